@@ -1,0 +1,173 @@
+/* libtopt: SIMP topology-optimization hot path for AMD Instinct MI355X (gfx950).
+ *
+ * C ABI only: plain device pointers, sizes and host scalars; every call is
+ * asynchronous on the given HIP stream (passed as void*) unless stated.
+ * Functions return 0 on success or a negative TOPT_E* code; topt_last_error()
+ * returns a thread-local message.
+ *
+ * Each entry point replaces a Taichi kernel or host driver of the reference
+ * (tomohiron907/toptimization, paths relative to src/toptimization/).
+ *
+ * Layout: node n = ix*(nely+1)+iy with DOFs interleaved (x, y) -> 2 scalars
+ * per node; element e = ex*nely+ey; fixed DOFs as a bitmask (bit 2n+c of a
+ * uint32 array of ceil(2*nodes/32) words).
+ */
+#ifndef TOPT_H
+#define TOPT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision of the PCG vectors */
+#define TOPT_F32 0   /* mixed: FP32 matvec/vectors, FP64 dots + FP64 residual refresh */
+#define TOPT_F64 1   /* plain FP64 */
+
+/* preconditioners (reference solver.py:12-15; BLOCK2 is new) */
+#define TOPT_PC_NONE 0
+#define TOPT_PC_JACOBI 1
+#define TOPT_PC_ELEMBLOCK 2   /* reference "block_jacobi" (preconditioner.py) */
+#define TOPT_PC_BLOCK2 3      /* nodal 2x2 block Jacobi */
+
+/* filter modes (reference optimizer/filter.py) */
+#define TOPT_FILTER_DENSITY 0     /* rhof = H rho / Hs          filter.py:106-124 */
+#define TOPT_FILTER_DCAVG 1       /* dcf  = H dc / Hs           filter.py:127-149 */
+#define TOPT_FILTER_SENS 2        /* dcf  = H(rho dc)/(rho Hs)  filter.py:152-171 */
+#define TOPT_FILTER_TRANSPOSE 3   /* dcf  = H^T (dc / Hs)       exact chain rule (new) */
+
+/* PCG status */
+#define TOPT_ST_RUN 0
+#define TOPT_ST_CONV 1
+#define TOPT_ST_BREAK 2    /* |pAp| < 1e-30 (solver.py:114-115): reports max_iter */
+#define TOPT_ST_MAXIT 3
+#define TOPT_ST_ZERO 4     /* |f| < 1e-30 (solver.py:60-63): u = 0, 0 iterations */
+
+/* error codes */
+#define TOPT_OK 0
+#define TOPT_EINVAL -1
+#define TOPT_EHIP -2
+#define TOPT_ENOMEM -3
+
+/* Per-problem workspace buffers (byte offsets from topt_problem_layout). */
+enum {
+  TOPT_BUF_U = 0,   /* double[2*nodes]   displacement (FP64 solution)             */
+  TOPT_BUF_F,       /* double[2*nodes]   load vector (caller writes)              */
+  TOPT_BUF_FIX,     /* uint32[words]     fixed-DOF bitmask (caller writes)        */
+  TOPT_BUF_RHO,     /* double[elems]     design density (caller writes initial)   */
+  TOPT_BUF_RHOF,    /* double[elems]     filtered (physical) density              */
+  TOPT_BUF_DC,      /* double[elems]     compliance sensitivity                   */
+  TOPT_BUF_DCF,     /* double[elems]     filtered sensitivity                     */
+  TOPT_BUF_HS,      /* double[elems]     filter row sums                          */
+  TOPT_BUF_X, TOPT_BUF_R, TOPT_BUF_Z, TOPT_BUF_P0, TOPT_BUF_P1, TOPT_BUF_AP,
+  TOPT_BUF_PRE, TOPT_BUF_PRE2, TOPT_BUF_S, TOPT_BUF_S64,
+  TOPT_BUF_TAPS, TOPT_BUF_CG, TOPT_BUF_OC, TOPT_BUF_PART, TOPT_BUF_OPART,
+  TOPT_BUF_COUNT
+};
+
+/* One design problem: 2-D Q4 plane-stress grid, SIMP parameters, workspace. */
+typedef struct topt_problem {
+  int32_t nelx, nely;
+  int32_t precond;       /* TOPT_PC_* */
+  int32_t max_cg_iter;   /* solver.max_cg_iterations (problem.py:117) */
+  double E_min, penal;   /* material / SIMP (problem.py:103-108) */
+  double cg_tol;         /* relative residual tolerance (solver.py:45, 65) */
+  double volfrac, move, rmin;
+  double fnorm2;         /* |f|^2 of the load the caller wrote into TOPT_BUF_F */
+  double delta;          /* mixed path: refresh residual when |r| drops by this factor (0.1) */
+  double Ke[64];         /* element stiffness, row-major, E folded in (material.py:20-86) */
+  void* ws;              /* device workspace of topt_problem_layout() bytes */
+} topt_problem;
+
+typedef struct topt_plan topt_plan;
+
+/* Per-problem results of the last SIMP iteration (optimizer/simp.py:192-197). */
+typedef struct topt_iter_stats {
+  double compliance;   /* sum_e s_e u_e^T Ke u_e            (kernels.py:234-272) */
+  double volume;       /* mean rho after the OC update      (kernels.py:330-336) */
+  double change;       /* max |rho_new - rho|                (oc.py:46, 80) */
+  double lambda;       /* last bisection midpoint            (oc.py:72) */
+  int32_t n_cg;        /* PCG iterations, reference count semantics (solver.py:149,159) */
+  int32_t cg_status;   /* TOPT_ST_* */
+  int32_t oc_steps;    /* bisection steps taken (<= 50)      (oc.py:71) */
+  int32_t refines;     /* FP64 residual refreshes (mixed path) */
+} topt_iter_stats;
+
+const char* topt_last_error(void);
+int topt_device_count(int* n);
+
+/* ---- per-problem workspace (replaces fields.allocate, fem/fields.py:53-156) ---- */
+/* Byte offsets of every TOPT_BUF_* buffer (256-byte aligned) and the total. */
+int topt_problem_layout(int32_t nelx, int32_t nely, int32_t dtype, int32_t precond, double rmin,
+                        int64_t* offsets, int64_t* total_bytes);
+/* Build the filter stencil and row sums, zero u, reset the PCG/OC state.
+ * Caller has written F, FIX and RHO. Synchronous on the stream. */
+int topt_problem_init(const topt_problem* prob, int32_t dtype, void* stream);
+
+/* ---- batched engine: one launch per kernel advances every problem ---- */
+int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, topt_plan** out);
+int topt_plan_destroy(topt_plan* plan);
+/* Density / sensitivity filter of every problem (filter.py:106-171). */
+int topt_plan_filter(topt_plan* plan, int32_t mode, void* stream);
+/* PCG solve K(rho_phys) u = f for every problem (fem/solver.py:27-159).
+ * warm: 0 cold, 1 warm start from u, 2 warm iff the problem finished an OC
+ * iteration (simp.py:160). use_filtered: rho_phys = RHOF (density filter) else RHO.
+ * Blocks until all problems finished; writes n_iter[i] and status[i]. */
+int topt_plan_solve(topt_plan* plan, int32_t warm, int32_t use_filtered, int32_t* n_iter,
+                    int32_t* status, void* stream);
+/* Compliance + sensitivities of every problem (fem/kernels.py:234-272). */
+int topt_plan_sensitivity(topt_plan* plan, int32_t use_filtered, void* stream);
+/* OC bisection + commit for every problem (optimizer/oc.py:52-93). */
+int topt_plan_oc(topt_plan* plan, void* stream);
+/* One SIMP iteration for every problem (optimizer/simp.py:141-197):
+ * density mode (sens_filter=0) or sensitivity mode (1). Blocks; fills stats[nprob]. */
+int topt_plan_iteration(topt_plan* plan, int32_t sens_filter, topt_iter_stats* stats, void* stream);
+/* Read back per-problem stats of the last iteration (blocks). */
+int topt_plan_stats(topt_plan* plan, topt_iter_stats* stats, void* stream);
+
+/* ---- op-level kernels (single problem, explicit buffers) ---- */
+typedef struct topt_grid {
+  int32_t nelx, nely;
+  double E_min, penal;
+  double Ke[64];
+} topt_grid;
+
+/* Workspace bytes for the op-level reductions of a grid. */
+int64_t topt_op_workspace_bytes(int32_t nelx, int32_t nely);
+/* s_e = E_min + rho_e^p (1 - E_min)  (fem/kernels.py:92). s in dtype. */
+int topt_scale(const topt_grid* g, int32_t dtype, const double* rho, void* s, void* stream);
+/* Ap = K(s) p, fixed rows identity (fem/kernels.py:143-176); if pAp != NULL
+ * also p.Ap (FP64, device) (fem/fused_kernels.py:21-56). */
+int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, const void* s,
+                const uint32_t* fix, double* pAp, void* ws, void* stream);
+/* diag K(s), fixed -> 1, clamp 1e-20 (fem/kernels.py:76-101). */
+int topt_diagonal(const topt_grid* g, int32_t dtype, const void* s, const uint32_t* fix,
+                  void* diag, void* stream);
+/* Nodal 2x2 block-Jacobi inverse: inv_diag[2*nodes] (xx, yy), inv_off[nodes] (xy). */
+int topt_block2_inverse(const topt_grid* g, int32_t dtype, const void* s, const uint32_t* fix,
+                        void* inv_diag, void* inv_off, void* stream);
+/* Element-block Jacobi z = M r, r.z and r.r into rz_rr[2] (preconditioner.py:47-108). */
+int topt_elemblock_apply(const topt_grid* g, int32_t dtype, const void* r, void* z,
+                         const void* s, const uint32_t* fix, double* rz_rr, void* ws,
+                         void* stream);
+/* Compliance + dc (FP64) (fem/kernels.py:234-272); dv filled with 1 if non-NULL. */
+int topt_sensitivity(const topt_grid* g, const double* u, const double* rho, double* dc,
+                     double* dv, double* compliance, void* ws, void* stream);
+/* Stencil filter (modes above). Hs: double[elems] scratch filled by this call. */
+int topt_filter(int32_t nelx, int32_t nely, double rmin, int32_t mode, const double* rho,
+                const double* dc, double* out, double* Hs, void* ws, void* stream);
+/* Expand the stencil into the reference's padded tables (filter.py:52-74):
+ * nb int32[elems*width] (-1 pad), wt float[elems*width] (0 pad), cnt int32[elems]. */
+int topt_filter_tables(int32_t nelx, int32_t nely, double rmin, int32_t width, int32_t* nb,
+                       float* wt, int32_t* cnt, void* ws, void* stream);
+/* OC bisection + commit on rho in place (optimizer/oc.py:52-93). dv may be NULL (= 1).
+ * out[4] (device): change, volume, last lambda, steps. */
+int topt_oc_update(int32_t nelx, int32_t nely, double* rho, const double* dcf, const double* dv,
+                   double volfrac, double move, double* out, void* ws, void* stream);
+/* mean of an element field (fem/kernels.py:330-336) into out[0] (device). */
+int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* ws, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOPT_H */

@@ -1,0 +1,197 @@
+"""GPU parity: every HIP kernel and the SIMP loop against the FP64 oracle.
+
+Tolerances: FP64 kernels 1e-12 relative (summation order only); FP32 kernels
+2e-5 relative to the vector norm; integer tables bit-exact.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import simp_np as O
+
+from helpers import bridge, cantilever, mbb, oracle_args, rng_density
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(prob, seed=0):
+    from toptimization_amd import ops
+    Ke = O.ke_q4(prob.E, prob.nu)
+    edof = O.edof_q4(prob.nelx, prob.nely)
+    fixed = O.fixed_mask(prob.n_dofs, prob.fixed_dofs)
+    rho = rng_density(prob.n_elem, seed)
+    g = ops.make_grid(prob.nelx, prob.nely, Ke, prob.E_min, prob.penalty)
+    fix = ops.fix_mask(prob.n_dofs, prob.fixed_dofs)
+    return Ke, edof, fixed, rho, g, fix
+
+
+def _cuda(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+@pytest.mark.parametrize("shape", [(8, 4), (60, 20), (33, 17)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_matvec_matches_oracle(shape, dtype):
+    from toptimization_amd import ops
+    prob = mbb(*shape)
+    Ke, edof, fixed, rho, g, fix = _setup(prob, seed=1)
+    s_ref = O.scale(rho, prob.E_min, prob.penalty)
+    p = np.random.default_rng(2).normal(size=prob.n_dofs)   # nonzero on fixed DOFs too
+    ref = O.matvec(p, s_ref, Ke, edof, fixed)
+    s = ops.scale(g, _cuda(rho), dtype)
+    Ap, dot = ops.matvec(g, _cuda(p, dtype), s, fix, with_dot=True)
+    got = Ap.double().cpu().numpy()
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+    assert float(dot) == pytest.approx(float(p @ ref), rel=tol * 10, abs=tol * np.abs(ref).max())
+    # bitwise deterministic run to run
+    Ap2 = ops.matvec(g, _cuda(p, dtype), s, fix)
+    assert torch.equal(Ap, Ap2)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_diagonal_block2_elemblock(dtype):
+    from toptimization_amd import ops
+    prob = mbb(21, 9)
+    Ke, edof, fixed, rho, g, fix = _setup(prob, seed=3)
+    s_ref = O.scale(rho, prob.E_min, prob.penalty)
+    s = ops.scale(g, _cuda(rho), dtype)
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    d = ops.diagonal(g, s, fix).double().cpu().numpy()
+    np.testing.assert_allclose(d, O.diagonal(s_ref, Ke, edof, fixed), rtol=tol)
+    dg, off = ops.block2_inverse(g, s, fix)
+    Dinv = O.block2x2_inverse(s_ref, Ke, edof, fixed)
+    np.testing.assert_allclose(dg.double().cpu().numpy().reshape(-1, 2),
+                               np.stack([Dinv[:, 0, 0], Dinv[:, 1, 1]], 1), rtol=tol * 100)
+    np.testing.assert_allclose(off.double().cpu().numpy(), Dinv[:, 0, 1], rtol=tol * 100,
+                               atol=tol * 100 * np.abs(Dinv).max())
+    r = np.random.default_rng(4).normal(size=prob.n_dofs)
+    r[fixed] = 0.0
+    Ki, a = O.ke_inverse_regularised(Ke)
+    zref = O.elemblock_apply(r, s_ref, Ki, a, edof, O.dof_count(edof, prob.n_dofs), fixed)
+    z, rz, rr = ops.elemblock_apply(g, _cuda(r, dtype), s, fix)
+    zz = z.double().cpu().numpy()
+    assert np.abs(zz - zref).max() <= tol * 10 * np.abs(zref).max()
+    assert float(rz) == pytest.approx(float(r @ zref), rel=tol * 100)
+    assert float(rr) == pytest.approx(float(r @ r), rel=tol * 100)
+
+
+def test_sensitivity_matches_oracle():
+    from toptimization_amd import ops
+    prob = cantilever(40, 15)
+    Ke, edof, fixed, rho, g, fix = _setup(prob, seed=5)
+    u = np.random.default_rng(6).normal(size=prob.n_dofs)
+    c_ref, dc_ref, _ = O.sensitivity(u, rho, Ke, edof, prob.E_min, prob.penalty)
+    c, dc = ops.sensitivity(g, _cuda(u), _cuda(rho))
+    assert float(c) == pytest.approx(c_ref, rel=1e-12)
+    np.testing.assert_allclose(dc.cpu().numpy(), dc_ref, rtol=1e-12)
+
+
+@pytest.mark.parametrize("rmin", [1.5, 2.5, 4.0, 6.0, 7.144])
+def test_filter_tables_bit_exact(rmin):
+    from toptimization_amd import ops
+    nelx, nely = 23, 13
+    nb, wt, cnt = ops.filter_tables(nelx, nely, rmin)
+    nb0, wt0, c0 = O.filter_tables_bruteforce(nelx, nely, rmin)
+    np.testing.assert_array_equal(nb.cpu().numpy(), nb0)
+    np.testing.assert_array_equal(wt.cpu().numpy().view(np.uint32), wt0.view(np.uint32))
+    np.testing.assert_array_equal(cnt.cpu().numpy(), c0)
+
+
+@pytest.mark.parametrize("rmin", [1.5, 4.0, 6.0])
+def test_filter_modes_match_oracle(rmin):
+    from toptimization_amd import ops, _lib
+    nelx, nely = 70, 30
+    nb, wt, cnt = O.filter_tables(nelx, nely, rmin)
+    rho = rng_density(nelx * nely, 7)
+    dc = -rng_density(nelx * nely, 8)
+    cases = [
+        (_lib.FILTER_DENSITY, O.density_filter(rho, nb, wt, cnt)),
+        (_lib.FILTER_DCAVG, O.dc_average_filter(dc, nb, wt, cnt)),
+        (_lib.FILTER_SENS, O.sensitivity_filter(rho, dc, nb, wt, cnt)),
+        (_lib.FILTER_TRANSPOSE, O.filter_transpose(dc, nb, wt, cnt)),
+    ]
+    for mode, ref in cases:
+        got = ops.filter_apply(nelx, nely, rmin, mode, _cuda(rho), _cuda(dc)).cpu().numpy()
+        np.testing.assert_allclose(got, ref, rtol=1e-13, err_msg=f"mode {mode}")
+    # density filter is summed in the reference table order: bit-identical to the oracle
+    got = ops.filter_apply(nelx, nely, rmin, _lib.FILTER_DENSITY, _cuda(rho)).cpu().numpy()
+    np.testing.assert_array_equal(got, O.density_filter(rho, nb, wt, cnt))
+
+
+@pytest.mark.parametrize("scale", [1.0, 1e-4, 1e-7])
+def test_oc_update_matches_oracle(scale):
+    from toptimization_amd import ops
+    nelx, nely = 50, 30
+    rho = rng_density(nelx * nely, 9, 0.05, 0.95)
+    dcf = -rng_density(nelx * nely, 10, 1e-3, 1.0) * scale
+    ref, change_ref, trace = O.oc_update(rho, dcf, np.ones_like(rho), 0.4, 0.2, return_trace=True)
+    r = _cuda(rho)
+    change, vol, lam, steps = ops.oc_update(nelx, nely, r, _cuda(dcf), 0.4, 0.2)
+    assert steps == len(trace)
+    assert lam == trace[-1][0]
+    np.testing.assert_array_equal(r.cpu().numpy(), ref)
+    assert change == change_ref
+    assert vol == pytest.approx(np.mean(ref), rel=1e-14)
+
+
+@pytest.mark.parametrize("pc", ["jacobi", "block_jacobi", "block_jacobi_2x2", "none"])
+def test_pcg_fp64_matches_oracle(pc):
+    from toptimization_amd.engine import DesignState, Plan
+    prob = cantilever(30, 12, preconditioner=pc, cg_tolerance=1e-10, max_cg_iterations=20000)
+    Ke, edof, fixed, rho, _, _ = _setup(prob, seed=11)
+    st = DesignState(prob, precision="fp64", rho0=rho)
+    st.rhof.copy_(st.rho)
+    plan = Plan([st])
+    it, status = plan.solve(warm=0, use_filtered=True)
+    f = O.force_vector(prob.n_dofs, prob.force_dofs, prob.force_values, prob.fixed_dofs)
+    u_ref, it_ref = O.pcg(f, O.scale(rho, prob.E_min, prob.penalty), Ke, edof, fixed,
+                          max_iter=20000, tol=1e-10, preconditioner=pc)
+    assert status[0] == 1
+    assert abs(it[0] - it_ref) <= max(2, it_ref // 50)
+    u = st.u.cpu().numpy()
+    assert np.abs(u - u_ref).max() <= 1e-7 * np.abs(u_ref).max()
+    plan.close()
+
+
+def test_pcg_mixed_matches_fp64():
+    from toptimization_amd.engine import DesignState, Plan
+    prob = mbb(60, 20, preconditioner="block_jacobi_2x2", cg_tolerance=1e-8, max_cg_iterations=20000)
+    Ke, edof, fixed, rho, _, _ = _setup(prob, seed=12)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        st = DesignState(prob, precision=prec, rho0=rho)
+        st.rhof.copy_(st.rho)
+        plan = Plan([st])
+        it, status = plan.solve(warm=0, use_filtered=True)
+        assert status[0] == 1, (prec, status)
+        plan.sensitivity(use_filtered=True)
+        res[prec] = (plan.stats()[0].compliance, st.u.cpu().numpy(), it[0])
+        plan.close()
+    c64, u64, _ = res["fp64"]
+    cmx, umx, _ = res["mixed"]
+    assert cmx == pytest.approx(c64, rel=1e-6)
+    assert np.abs(umx - u64).max() <= 1e-5 * np.abs(u64).max()
+
+
+def test_simp_loop_config0_matches_oracle():
+    """60x20 cantilever, FP64 Jacobi PCG rtol 1e-8, 50 OC iterations."""
+    from toptimization_amd.optimizer.simp import run
+    prob = cantilever(60, 20, volume_fraction=0.5, filter_radius=1.5, max_iterations=50,
+                      tolerance=0.0, cg_tolerance=1e-8, max_cg_iterations=20000)
+    args, kw = oracle_args(prob)
+    ref = O.simp_run(*args, **kw)
+    hist = {}
+    rho = run(prob, live_viz=False, verbose=False, history=hist)
+    assert len(hist["c"]) == 50
+    c = np.array(hist["c"])
+    cref = np.array(ref["c"])
+    assert abs(c[-1] - cref[-1]) / cref[-1] <= 1e-6
+    assert np.max(np.abs(c - cref) / cref) <= 1e-6
+    np.testing.assert_allclose(hist["vol"], ref["vol"], rtol=1e-9)
+    assert np.abs(rho.ravel() - ref["rho"]).max() <= 1e-6
+    ncg = np.array(hist["n_cg"])
+    assert np.all(np.abs(ncg - np.array(ref["n_cg"])) <= 2)

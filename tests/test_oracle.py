@@ -1,0 +1,256 @@
+"""Pins for the FP64 oracle (oracle/simp_np.py) against independent mathematics.
+
+Each check is chosen so that a plausible slip (wrong sign, swapped index,
+dropped term, wrong table order) fails: SciPy sparse assembly, spsolve,
+rigid-body modes, finite differences, dense filter matrices, brute-force
+neighbour tables, brute-force OC replay.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import simp_np as O
+from toptimization_amd.material import compute_Ke_2d
+from toptimization_amd.mesh import build_edof_2d
+
+from helpers import cantilever, mbb, rng_density
+
+
+def assemble(s, Ke, edof, n):
+    rows = np.repeat(edof, 8, axis=1).ravel()
+    cols = np.tile(edof, (1, 8)).ravel()
+    vals = (s[:, None, None] * Ke[None]).ravel()
+    return sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+
+
+def bc_operator(K, fixed):
+    """Identity rows at fixed DOFs, columns untouched (fem/kernels.py:174-176)."""
+    K = K.tolil()
+    for i in np.flatnonzero(fixed):
+        K.rows[i] = [i]
+        K.data[i] = [1.0]
+    return K.tocsr()
+
+
+def small_case(nelx=7, nely=5, seed=0, prob=None):
+    prob = prob or cantilever(nelx, nely)
+    Ke = O.ke_q4()
+    edof = O.edof_q4(prob.nelx, prob.nely)
+    fixed = O.fixed_mask(prob.n_dofs, prob.fixed_dofs)
+    f = O.force_vector(prob.n_dofs, prob.force_dofs, prob.force_values, prob.fixed_dofs)
+    rho = rng_density(prob.n_elem, seed)
+    s = O.scale(rho, prob.E_min, prob.penalty)
+    return prob, Ke, edof, fixed, f, rho, s
+
+
+def test_ke_closed_form_equals_quadrature():
+    assert np.abs(O.ke_q4(1.0, 0.3) - compute_Ke_2d(1.0, 0.3)).max() < 1e-15
+    assert np.abs(O.ke_q4(2.5, 0.25) - compute_Ke_2d(2.5, 0.25)).max() < 1e-14
+    assert O.ke_q4()[0, 0] == pytest.approx(0.4945054945054945, rel=1e-15)
+
+
+def test_edof_matches_product_and_reference_pin():
+    prob = cantilever(4, 3)
+    np.testing.assert_array_equal(O.edof_q4(4, 3), build_edof_2d(prob))
+    np.testing.assert_array_equal(O.edof_q4(4, 3)[0], [0, 1, 8, 9, 10, 11, 2, 3])
+
+
+def test_matvec_equals_sparse_assembly_with_bc_rule():
+    prob, Ke, edof, fixed, f, rho, s = small_case()
+    K = assemble(s, Ke, edof, prob.n_dofs)
+    A = bc_operator(K, fixed)
+    p = np.random.default_rng(3).normal(size=prob.n_dofs)  # nonzero at fixed DOFs too
+    np.testing.assert_allclose(O.matvec(p, s, Ke, edof, fixed), A @ p, rtol=1e-13, atol=1e-13)
+    # free rows still see fixed columns
+    assert np.abs((K @ p)[~fixed] - O.matvec(p, s, Ke, edof, fixed)[~fixed]).max() < 1e-12
+
+
+def test_matvec_rigid_body_modes_and_symmetry():
+    nelx, nely = 6, 4
+    n = 2 * (nelx + 1) * (nely + 1)
+    Ke, edof = O.ke_q4(), O.edof_q4(nelx, nely)
+    s = O.scale(rng_density(nelx * nely, 1), 1e-9, 3.0)
+    free = np.zeros(n, dtype=bool)
+    ix, iy = np.divmod(np.arange(n // 2), nely + 1)
+    tx = np.zeros(n); tx[0::2] = 1
+    ty = np.zeros(n); ty[1::2] = 1
+    rot = np.zeros(n); rot[0::2] = -iy; rot[1::2] = ix
+    for mode in (tx, ty, rot):
+        assert np.abs(O.matvec(mode, s, Ke, edof, free)).max() < 1e-12
+    K = assemble(s, Ke, edof, n)
+    assert abs(K - K.T).max() < 1e-15
+
+
+def test_diagonal_and_2x2_blocks_match_assembly():
+    prob, Ke, edof, fixed, f, rho, s = small_case(prob=mbb(7, 5))
+    K = assemble(s, Ke, edof, prob.n_dofs).toarray()
+    d = O.diagonal(s, Ke, edof, fixed)
+    np.testing.assert_allclose(d[~fixed], np.diag(K)[~fixed], rtol=1e-14)
+    assert np.all(d[fixed] == 1.0)
+    Dinv = O.block2x2_inverse(s, Ke, edof, fixed)
+    A = bc_operator(sp.csr_matrix(K), fixed).toarray()
+    for node in range(prob.n_nodes):
+        blk = A[2 * node:2 * node + 2, 2 * node:2 * node + 2].copy()
+        for c in range(2):
+            if fixed[2 * node + c]:
+                blk[c, :] = 0.0; blk[:, c] = 0.0; blk[c, c] = 1.0
+        np.testing.assert_allclose(Dinv[node] @ blk, np.eye(2), atol=1e-10)
+
+
+def test_elemblock_equals_dense_assembly():
+    prob, Ke, edof, fixed, f, rho, s = small_case()
+    Ke_inv, a = O.ke_inverse_regularised(Ke)
+    assert a == pytest.approx(0.1 * np.mean(np.diag(Ke)))
+    cnt = O.dof_count(edof, prob.n_dofs)
+    w = s / np.maximum(s, a) ** 2
+    M = assemble(w, Ke_inv, edof, prob.n_dofs).toarray()
+    Cs = np.diag(1.0 / np.sqrt(cnt))
+    M = Cs @ M @ Cs
+    M[fixed, :] = 0.0
+    r = np.random.default_rng(5).normal(size=prob.n_dofs)
+    np.testing.assert_allclose(O.elemblock_apply(r, s, Ke_inv, a, edof, cnt, fixed), M @ r,
+                               rtol=1e-12, atol=1e-12)
+    Mf = M[np.ix_(~fixed, ~fixed)]
+    assert np.abs(Mf - Mf.T).max() < 1e-12
+    # corner/edge/interior multiplicities
+    assert sorted(set(cnt.tolist())) == [1, 2, 4]
+
+
+@pytest.mark.parametrize("pc", ["jacobi", "block_jacobi", "block_jacobi_2x2", "none"])
+def test_pcg_matches_spsolve(pc):
+    prob, Ke, edof, fixed, f, rho, s = small_case(12, 6, prob=None)
+    K = assemble(s, Ke, edof, prob.n_dofs)
+    free = ~fixed
+    u_ref = np.zeros(prob.n_dofs)
+    u_ref[free] = spla.spsolve(K[free][:, free].tocsc(), f[free])
+    u, it = O.pcg(f, s, Ke, edof, fixed, max_iter=20000, tol=1e-12, preconditioner=pc)
+    assert 0 < it < 20000
+    np.testing.assert_allclose(u, u_ref, rtol=1e-7, atol=1e-9 * np.abs(u_ref).max())
+    c_fu = f @ u
+    c_e, _, _ = O.sensitivity(u, rho, Ke, edof, prob.E_min, prob.penalty)
+    assert c_e == pytest.approx(c_fu, rel=1e-9)
+
+
+def test_pcg_quirks():
+    prob, Ke, edof, fixed, f, rho, s = small_case()
+    # zero load -> 0 iterations, u = 0
+    u, it = O.pcg(np.zeros_like(f), s, Ke, edof, fixed)
+    assert it == 0 and not u.any()
+    # non-convergence reports max_iter; one iteration always runs
+    u, it = O.pcg(f, s, Ke, edof, fixed, max_iter=1, tol=1e-12)
+    assert it == 1
+    # warm start from the solution converges in one iteration
+    u_star, _ = O.pcg(f, s, Ke, edof, fixed, max_iter=5000, tol=1e-12)
+    _, it = O.pcg(f, s, Ke, edof, fixed, u_prev=u_star, warm_start=True, tol=1e-6)
+    assert it == 1
+
+
+def test_sensitivity_matches_finite_differences():
+    prob, Ke, edof, fixed, f, rho, s = small_case(6, 4, seed=7)
+    free = ~fixed
+
+    def comp(r):
+        K = assemble(O.scale(r, prob.E_min, prob.penalty), Ke, edof, prob.n_dofs)
+        u = np.zeros(prob.n_dofs)
+        u[free] = spla.spsolve(K[free][:, free].tocsc(), f[free])
+        return f @ u, u
+
+    c0, u = comp(rho)
+    c, dc, dv = O.sensitivity(u, rho, Ke, edof, prob.E_min, prob.penalty)
+    assert c == pytest.approx(c0, rel=1e-12)
+    h = 1e-6
+    for e in (0, 5, 11, 23):
+        rp, rm = rho.copy(), rho.copy()
+        rp[e] += h; rm[e] -= h
+        fd = (comp(rp)[0] - comp(rm)[0]) / (2 * h)
+        assert dc[e] == pytest.approx(fd, rel=2e-6)
+    assert np.all(dv == 1.0)
+
+
+@pytest.mark.parametrize("rmin", [1.5, 2.5, 4.0, 6.0, 7.144])
+def test_filter_tables_bit_exact(rmin):
+    nelx, nely = 19, 11
+    nb0, wt0, c0 = O.filter_tables_bruteforce(nelx, nely, rmin)
+    nb1, wt1, c1 = O.filter_tables(nelx, nely, rmin)
+    np.testing.assert_array_equal(nb0, nb1)
+    np.testing.assert_array_equal(wt0.view(np.uint32), wt1.view(np.uint32))
+    np.testing.assert_array_equal(c0, c1)
+    assert nb0.shape[1] == (2 * math.ceil(rmin) + 1) ** 2
+    interior = {1.5: 9, 2.5: 21, 4.0: 45, 6.0: 109, 7.144: 161}[rmin]
+    assert len(O.filter_stencil(rmin)) == interior
+
+
+def dense_H(nb, wt, cnt):
+    n = nb.shape[0]
+    H = np.zeros((n, n))
+    for e in range(n):
+        H[e, nb[e, :cnt[e]]] = wt[e, :cnt[e]]
+    return H
+
+
+def test_filters_against_dense_matrix():
+    nelx, nely, rmin = 9, 7, 2.5
+    nb, wt, cnt = O.filter_tables(nelx, nely, rmin)
+    H = dense_H(nb, wt, cnt)
+    assert np.abs(H - H.T).max() == 0.0
+    Hs = H.sum(1)
+    rho = rng_density(nelx * nely, 2)
+    dc = -rng_density(nelx * nely, 3)
+    np.testing.assert_allclose(O.density_filter(rho, nb, wt, cnt), H @ rho / (Hs + 1e-20), rtol=1e-14)
+    np.testing.assert_allclose(O.dc_average_filter(dc, nb, wt, cnt), H @ dc / (Hs + 1e-20), rtol=1e-14)
+    np.testing.assert_allclose(O.sensitivity_filter(rho, dc, nb, wt, cnt),
+                               H @ (rho * dc) / (rho * Hs + 1e-20), rtol=1e-14)
+    np.testing.assert_allclose(O.filter_transpose(dc, nb, wt, cnt), H.T @ (dc / (Hs + 1e-20)), rtol=1e-13)
+    # exact chain rule: d/d rho_j of sum_e g_e rhof_e
+    g = dc
+    eps = 1e-7
+    for j in (0, 17, 40):
+        rp, rm = rho.copy(), rho.copy()
+        rp[j] += eps; rm[j] -= eps
+        fd = (g @ O.density_filter(rp, nb, wt, cnt) - g @ O.density_filter(rm, nb, wt, cnt)) / (2 * eps)
+        assert O.filter_transpose(g, nb, wt, cnt)[j] == pytest.approx(fd, rel=1e-7)
+    # the reference's H dc / Hs equals the exact transpose only away from the boundary
+    diff = np.abs(O.dc_average_filter(dc, nb, wt, cnt) - O.filter_transpose(dc, nb, wt, cnt))
+    ex, ey = np.divmod(np.arange(nelx * nely), nely)
+    r = math.ceil(rmin)
+    deep = (ex >= 2 * r) & (ex < nelx - 2 * r) & (ey >= 2 * r) & (ey < nely - 2 * r)
+    assert diff[deep].max() < 1e-15 if deep.any() else True
+    assert diff.max() > 1e-6
+
+
+def test_oc_bisection_bruteforce_replay():
+    n = 500
+    rho = rng_density(n, 4, 0.1, 0.9)
+    dc = -rng_density(n, 5, 1e-3, 1.0)
+    dv = np.ones(n)
+    new, change, trace = O.oc_update(rho, dc, dv, 0.5, 0.2, return_trace=True)
+    lo, hi = 0.0, 1e9
+    for k, (lam, vol) in enumerate(trace):
+        assert lam == 0.5 * (lo + hi)
+        assert vol == pytest.approx(np.mean(O.oc_candidate(rho, dc, dv, lam, 0.2)), rel=1e-15)
+        lo, hi = (lam, hi) if vol > 0.5 else (lo, lam)
+        stop = (hi - lo) / (hi + lo + 1e-30) < 1e-4
+        assert stop == (k == len(trace) - 1) or (k == 49)
+    np.testing.assert_array_equal(new, O.oc_candidate(rho, dc, dv, trace[-1][0], 0.2))
+    assert change == np.max(np.abs(new - rho))
+    lams = np.logspace(-6, 3, 200)
+    vols = [np.mean(O.oc_candidate(rho, dc, dv, l, 0.2)) for l in lams]
+    assert np.all(np.diff(vols) <= 1e-15)
+    assert np.all(new >= 0.001) and np.all(new <= 1.0)
+    # 50-step cap binds for a tiny multiplier
+    _, _, tr = O.oc_update(rho, dc * 1e-6, dv, 0.5, 0.2, return_trace=True)
+    assert len(tr) == 50
+
+
+def test_simp_loop_small_cantilever_behaves():
+    prob = cantilever(20, 10)
+    h = O.simp_run(prob.nelx, prob.nely, prob.fixed_dofs, prob.force_dofs, prob.force_values,
+                   volfrac=0.5, rmin=1.5, max_iter=15, tol=0.0, cg_tol=1e-8, max_cg=5000)
+    assert len(h["c"]) == 15
+    assert h["c"][5] < h["c"][0]
+    assert abs(h["vol"][-1] - 0.5) < 5e-3
+    assert all(n > 0 for n in h["n_cg"])

@@ -1,0 +1,821 @@
+// libtopt host engine: workspace layout, batched plans, the PCG slot loop
+// with asynchronous convergence polling, and the C ABI of include/topt.h.
+//
+// The reference drives every kernel from Python with a host sync after each
+// scalar-returning kernel (fem/solver.py:107-156, optimizer/oc.py:71-88).
+// Here alpha/beta/convergence/bisection live on the device; the host only
+// enqueues fixed chunks of slot kernels and polls a status snapshot that is
+// copied back asynchronously, one chunk behind the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/topt.h"
+#include "topt_dev.h"
+#include "fem_kernels.h"
+#include "opt_kernels.h"
+
+using namespace topt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(TOPT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCH_TRY()                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess) return fail(TOPT_EHIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+struct Geom {
+  int64_t nn, ne, words;
+  int nb[4];
+};
+
+Geom geom(int nelx, int nely) {
+  Geom g;
+  g.nn = int64_t(nelx + 1) * (nely + 1);
+  g.ne = int64_t(nelx) * nely;
+  g.words = cdiv(2 * g.nn, 32);
+  g.nb[G_NODE] = (int)cdiv(g.nn, TOPT_BLOCK);
+  g.nb[G_ELEM] = (int)cdiv(g.ne, TOPT_BLOCK);
+  g.nb[G_FILT] = (int)(cdiv(nelx, TOPT_FILT_TX) * cdiv(nely, TOPT_FILT_TY));
+  g.nb[G_OC] = (int)cdiv(g.ne, int64_t(TOPT_BLOCK) * TOPT_OC_PER_THREAD);
+  return g;
+}
+
+// Filter stencil in the reference's table order (optimizer/filter.py:58-74):
+// dx outer, dy inner over [-ceil(r), ceil(r)], kept when w = rmin - sqrt(d2) > 0,
+// weight stored as float32 (filter.py:53,71).
+std::vector<Tap> build_taps(double rmin) {
+  std::vector<Tap> taps;
+  const int rc = (int)std::ceil(rmin);
+  for (int dx = -rc; dx <= rc; ++dx)
+    for (int dy = -rc; dy <= rc; ++dy) {
+      const double w = rmin - std::sqrt((double)(dx * dx + dy * dy));
+      if (w > 0.0) taps.push_back(Tap{dx, dy, (double)(float)w});
+    }
+  return taps;
+}
+
+int tap_radius(const std::vector<Tap>& taps) {
+  int R = 0;
+  for (const Tap& t : taps) R = std::max(R, std::max(std::abs(t.dx), std::abs(t.dy)));
+  return R;
+}
+
+struct Layout {
+  int64_t off[TOPT_BUF_COUNT];
+  int64_t total;
+};
+
+int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout& L) {
+  if (nelx < 1 || nely < 1) return fail(TOPT_EINVAL, "grid must have nelx, nely >= 1");
+  if (dtype != TOPT_F32 && dtype != TOPT_F64) return fail(TOPT_EINVAL, "dtype must be TOPT_F32 or TOPT_F64");
+  if (precond < TOPT_PC_NONE || precond > TOPT_PC_BLOCK2) return fail(TOPT_EINVAL, "unknown preconditioner");
+  if (!(rmin > 0.0)) return fail(TOPT_EINVAL, "rmin must be > 0");
+  const Geom g = geom(nelx, nely);
+  const int64_t ts = dtype == TOPT_F32 ? 4 : 8;
+  const int64_t ntaps = (int64_t)build_taps(rmin).size();
+  int64_t sz[TOPT_BUF_COUNT] = {0};
+  sz[TOPT_BUF_U] = g.nn * 16;
+  sz[TOPT_BUF_F] = g.nn * 16;
+  sz[TOPT_BUF_FIX] = g.words * 4;
+  sz[TOPT_BUF_RHO] = sz[TOPT_BUF_RHOF] = sz[TOPT_BUF_DC] = sz[TOPT_BUF_DCF] = sz[TOPT_BUF_HS] = g.ne * 8;
+  sz[TOPT_BUF_X] = dtype == TOPT_F32 ? g.nn * 2 * ts : 0;   // plain path: x aliases u
+  sz[TOPT_BUF_R] = sz[TOPT_BUF_Z] = sz[TOPT_BUF_P0] = sz[TOPT_BUF_P1] = sz[TOPT_BUF_AP] = g.nn * 2 * ts;
+  sz[TOPT_BUF_PRE] = (precond == TOPT_PC_JACOBI || precond == TOPT_PC_BLOCK2) ? g.nn * 2 * ts : 0;
+  sz[TOPT_BUF_PRE2] = precond == TOPT_PC_BLOCK2 ? g.nn * ts : 0;
+  sz[TOPT_BUF_S] = g.ne * ts;
+  sz[TOPT_BUF_S64] = dtype == TOPT_F32 ? g.ne * 8 : 0;     // plain path: s64 aliases s
+  sz[TOPT_BUF_TAPS] = std::max<int64_t>(ntaps, 1) * (int64_t)sizeof(Tap);
+  sz[TOPT_BUF_CG] = sizeof(CgState);
+  sz[TOPT_BUF_OC] = sizeof(OcState);
+  sz[TOPT_BUF_PART] = int64_t(g.nb[G_NODE]) * 2 * 8;
+  sz[TOPT_BUF_OPART] = std::max<int64_t>(g.nb[G_ELEM], int64_t(g.nb[G_OC]) * 64) * 8;
+  int64_t o = 0;
+  for (int i = 0; i < TOPT_BUF_COUNT; ++i) {
+    L.off[i] = o;
+    o += align256(sz[i]);
+  }
+  if (sz[TOPT_BUF_X] == 0) L.off[TOPT_BUF_X] = L.off[TOPT_BUF_U];
+  if (sz[TOPT_BUF_S64] == 0) L.off[TOPT_BUF_S64] = L.off[TOPT_BUF_S];
+  L.total = o;
+  return TOPT_OK;
+}
+
+// 8x8 inverse by Gauss-Jordan with partial pivoting (FP64).
+bool invert8(const double* A, double* Ainv) {
+  double M[8][16];
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 16; ++j) M[i][j] = j < 8 ? A[i * 8 + j] : (j - 8 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 8; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < 8; ++i)
+      if (std::fabs(M[i][c]) > std::fabs(M[piv][c])) piv = i;
+    if (M[piv][c] == 0.0) return false;
+    if (piv != c)
+      for (int j = 0; j < 16; ++j) std::swap(M[c][j], M[piv][j]);
+    const double d = M[c][c];
+    for (int j = 0; j < 16; ++j) M[c][j] /= d;
+    for (int i = 0; i < 8; ++i) {
+      if (i == c) continue;
+      const double f = M[i][c];
+      if (f != 0.0)
+        for (int j = 0; j < 16; ++j) M[i][j] -= f * M[c][j];
+    }
+  }
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) Ainv[i * 8 + j] = M[i][8 + j];
+  return true;
+}
+
+// Material part of a descriptor: Ke (FP32 + FP64) and the regularised
+// inverse used by the element-block preconditioner,
+// Ki = inv(Ke + a I), a = 0.1 mean|diag Ke| (fem/fields.py:139-143).
+int fill_material(ProbDev& P, const double* Ke, double E_min, double penal) {
+  P.Emin = E_min;
+  P.penal = penal;
+  double a = 0.0;
+  for (int i = 0; i < 8; ++i) a += std::fabs(Ke[i * 9]);
+  a = 0.1 * a / 8.0;
+  P.alpha_reg = a;
+  double Kr[64], Ki[64];
+  for (int i = 0; i < 64; ++i) {
+    P.Ke64[i] = Ke[i];
+    P.Ke32[i] = (float)Ke[i];
+    Kr[i] = Ke[i] + ((i % 9) == 0 ? a : 0.0);
+  }
+  if (!invert8(Kr, Ki)) return fail(TOPT_EINVAL, "Ke + alpha I is singular");
+  for (int i = 0; i < 64; ++i) {
+    P.Ki64[i] = Ki[i];
+    P.Ki32[i] = (float)Ki[i];
+  }
+  return TOPT_OK;
+}
+
+int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
+  std::memset(&P, 0, sizeof(P));
+  Layout L;
+  int rc = make_layout(p.nelx, p.nely, dtype, p.precond, p.rmin, L);
+  if (rc) return rc;
+  if (p.ws == nullptr) return fail(TOPT_EINVAL, "problem workspace is NULL");
+  const Geom g = geom(p.nelx, p.nely);
+  P.nelx = p.nelx;
+  P.nely = p.nely;
+  for (int k = 0; k < 4; ++k) P.nb[k] = g.nb[k];
+  P.precond = p.precond;
+  P.max_it = p.max_cg_iter;
+  const std::vector<Tap> taps = build_taps(p.rmin);
+  P.ntaps = (int)taps.size();
+  P.R = tap_radius(taps);
+  rc = fill_material(P, p.Ke, p.E_min, p.penal);
+  if (rc) return rc;
+  P.tol2 = (p.cg_tol * std::sqrt(p.fnorm2)) * (p.cg_tol * std::sqrt(p.fnorm2));
+  P.delta2 = p.delta * p.delta;
+  P.volfrac = p.volfrac;
+  P.move = p.move;
+  P.fnorm2 = p.fnorm2;
+  char* w = (char*)p.ws;
+  auto at = [&](int b) { return (void*)(w + L.off[b]); };
+  P.x = at(TOPT_BUF_X);
+  P.r = at(TOPT_BUF_R);
+  P.z = at(TOPT_BUF_Z);
+  P.p0 = at(TOPT_BUF_P0);
+  P.p1 = at(TOPT_BUF_P1);
+  P.Ap = at(TOPT_BUF_AP);
+  P.pre = at(TOPT_BUF_PRE);
+  P.pre2 = at(TOPT_BUF_PRE2);
+  P.s = at(TOPT_BUF_S);
+  P.s64 = (double*)at(TOPT_BUF_S64);
+  P.u64 = (double*)at(TOPT_BUF_U);
+  P.f = (const double*)at(TOPT_BUF_F);
+  P.fix = (const uint32_t*)at(TOPT_BUF_FIX);
+  P.rho = (double*)at(TOPT_BUF_RHO);
+  P.rhof = (double*)at(TOPT_BUF_RHOF);
+  P.dc = (double*)at(TOPT_BUF_DC);
+  P.dcf = (double*)at(TOPT_BUF_DCF);
+  P.dv = nullptr;
+  P.Hs = (double*)at(TOPT_BUF_HS);
+  P.taps = (const Tap*)at(TOPT_BUF_TAPS);
+  P.cg = (CgState*)at(TOPT_BUF_CG);
+  P.part = (double*)at(TOPT_BUF_PART);
+  P.oc = (OcState*)at(TOPT_BUF_OC);
+  P.opart = (double*)at(TOPT_BUF_OPART);
+  return TOPT_OK;
+}
+
+size_t filter_shmem(int R) {
+  return size_t(TOPT_FILT_TX + 2 * R) * size_t(TOPT_FILT_TY + 2 * R) * sizeof(double);
+}
+
+__global__ void k_gather_status(const ProbDev* tab, int n, int* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CgState* st = tab[i].cg;
+  out[2 * i] = st->status;
+  out[2 * i + 1] = st->it;
+}
+
+__global__ void k_gather_stats(const ProbDev* tab, int n, topt_iter_stats* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const CgState* st = tab[i].cg;
+  const OcState* o = tab[i].oc;
+  topt_iter_stats s;
+  s.compliance = o->c;
+  s.volume = o->vol;
+  s.change = o->change;
+  s.lambda = o->last_mid;
+  s.cg_status = st->status;
+  s.n_cg = (st->status == ST_BREAK || st->status == ST_MAXIT) ? tab[i].max_it
+           : (st->status == ST_ZERO ? 0 : st->it);
+  s.oc_steps = o->steps;
+  s.refines = st->refines;
+  out[i] = s;
+}
+
+__global__ void k_oc_out(const OcState* o, double* out) {
+  out[0] = o->change;
+  out[1] = o->vol;
+  out[2] = o->last_mid;
+  out[3] = (double)o->steps;
+}
+
+}  // namespace
+
+// ============================================================== plan
+struct topt_plan {
+  int n = 0;
+  int dtype = TOPT_F64;
+  bool any_elemblock = false;
+  int maxR = 0;
+  int max_cg = 0;
+  std::vector<ProbDev> host;
+  ProbDev* dtab = nullptr;
+  int* dmap[4] = {nullptr, nullptr, nullptr, nullptr};
+  int nblk[4] = {0, 0, 0, 0};
+  int* dstat = nullptr;
+  int* hstat[2] = {nullptr, nullptr};
+  topt_iter_stats* dstats = nullptr;
+  topt_iter_stats* hstats = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+template <typename T>
+int launch_slots(topt_plan* pl, int count, hipStream_t s) {
+  const ProbDev dummy{};
+  const int nb = pl->nblk[G_NODE];
+  for (int k = 0; k < count; ++k) {
+    hipLaunchKernelGGL((k_cg_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+    hipLaunchKernelGGL((k_cg_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+    if (pl->any_elemblock)
+      hipLaunchKernelGGL((k_cg_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+  }
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
+  const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, s) : launch_slots<double>(pl, count, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstat);
+  LAUNCH_TRY();
+  HIP_TRY(hipMemcpyAsync(pl->hstat[slot], pl->dstat, sizeof(int) * 2 * pl->n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipEventRecord(pl->ev[slot], s));
+  return TOPT_OK;
+}
+
+bool any_running(const topt_plan* pl, const int* st) {
+  for (int i = 0; i < pl->n; ++i)
+    if (st[2 * i] == ST_RUN) return true;
+  return false;
+}
+
+template <typename T>
+int launch_solve_setup(topt_plan* pl, int warm, int use_filtered, hipStream_t s) {
+  const ProbDev dummy{};
+  hipLaunchKernelGGL((k_scale<T, true>), dim3(pl->nblk[G_ELEM]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     pl->dmap[G_ELEM], dummy, use_filtered);
+  hipLaunchKernelGGL((k_precond<T, true>), dim3(pl->nblk[G_NODE]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     pl->dmap[G_NODE], dummy, 0);
+  hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(pl->nblk[G_NODE]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     pl->dmap[G_NODE], dummy, warm);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int plan_filter(topt_plan* pl, int mode, hipStream_t s) {
+  const ProbDev dummy{};
+  hipLaunchKernelGGL((k_filter<true>), dim3(pl->nblk[G_FILT]), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s,
+                     pl->dtab, pl->dmap[G_FILT], dummy, mode);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int plan_oc(topt_plan* pl, hipStream_t s) {
+  const ProbDev dummy{};
+  constexpr int D = 5;
+  hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, dummy);
+  for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
+    hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(pl->nblk[G_OC]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                       pl->dmap[G_OC], dummy);
+  hipLaunchKernelGGL((k_oc_apply<true>), dim3(pl->nblk[G_OC]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     pl->dmap[G_OC], dummy);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int plan_sens(topt_plan* pl, int use_filtered, hipStream_t s) {
+  const ProbDev dummy{};
+  hipLaunchKernelGGL((k_sens<true>), dim3(pl->nblk[G_ELEM]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     pl->dmap[G_ELEM], dummy, use_filtered);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* status, hipStream_t s) {
+  int rc = pl->dtype == TOPT_F32 ? launch_solve_setup<float>(pl, warm, use_filtered, s)
+                                 : launch_solve_setup<double>(pl, warm, use_filtered, s);
+  if (rc) return rc;
+  // slots issued so far; a solve needs at most max_cg CG slots plus refresh slots
+  const int64_t slot_cap = 4 * int64_t(pl->max_cg) + 4096;
+  int64_t issued = 0;
+  int chunk = 4;
+  for (int k = 0; k < 2; ++k) {
+    rc = enqueue_chunk(pl, chunk, k, s);
+    if (rc) return rc;
+    issued += chunk;
+  }
+  int wait = 0;
+  for (;;) {
+    HIP_TRY(hipEventSynchronize(pl->ev[wait]));
+    if (!any_running(pl, pl->hstat[wait])) break;
+    if (issued > slot_cap) return fail(TOPT_EINVAL, "PCG slot loop did not terminate");
+    chunk = std::min(chunk * 2, 32);
+    rc = enqueue_chunk(pl, chunk, wait, s);
+    if (rc) return rc;
+    issued += chunk;
+    wait ^= 1;
+  }
+  // drain the chunk still in flight and take a final snapshot
+  HIP_TRY(hipEventSynchronize(pl->ev[wait ^ 1]));
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
+  LAUNCH_TRY();
+  HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int i = 0; i < pl->n; ++i) {
+    if (n_iter) n_iter[i] = pl->hstats[i].n_cg;
+    if (status) status[i] = pl->hstats[i].cg_status;
+  }
+  return TOPT_OK;
+}
+
+int gather_stats(topt_plan* pl, topt_iter_stats* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
+  LAUNCH_TRY();
+  HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  std::memcpy(out, pl->hstats, sizeof(topt_iter_stats) * pl->n);
+  return TOPT_OK;
+}
+
+// Single-problem descriptor for op-level calls: geometry, material and the
+// reduction scratch carved from ws (partials + a CgState/OcState for counters).
+struct OpScratch {
+  int64_t part_off, opart_off, cg_off, oc_off, hs_off, taps_off, total;
+};
+
+OpScratch op_scratch(int nelx, int nely) {
+  const Geom g = geom(nelx, nely);
+  OpScratch o;
+  int64_t off = 0;
+  o.part_off = off;
+  off += align256(int64_t(g.nb[G_NODE]) * 2 * 8);
+  o.opart_off = off;
+  off += align256(std::max<int64_t>(g.nb[G_ELEM], int64_t(g.nb[G_OC]) * 64) * 8);
+  o.cg_off = off;
+  off += align256(sizeof(CgState));
+  o.oc_off = off;
+  off += align256(sizeof(OcState));
+  o.taps_off = off;
+  off += align256(int64_t(2 * 34 + 1) * (2 * 34 + 1) * (int64_t)sizeof(Tap));
+  o.total = off;
+  return o;
+}
+
+int op_prob(ProbDev& P, int nelx, int nely, void* ws, hipStream_t s) {
+  std::memset(&P, 0, sizeof(P));
+  if (nelx < 1 || nely < 1) return fail(TOPT_EINVAL, "grid must have nelx, nely >= 1");
+  if (ws == nullptr) return fail(TOPT_EINVAL, "workspace is NULL");
+  const Geom g = geom(nelx, nely);
+  P.nelx = nelx;
+  P.nely = nely;
+  for (int k = 0; k < 4; ++k) P.nb[k] = g.nb[k];
+  const OpScratch o = op_scratch(nelx, nely);
+  char* w = (char*)ws;
+  P.part = (double*)(w + o.part_off);
+  P.opart = (double*)(w + o.opart_off);
+  P.cg = (CgState*)(w + o.cg_off);
+  P.oc = (OcState*)(w + o.oc_off);
+  // counters must start at zero
+  HIP_TRY(hipMemsetAsync(P.cg, 0, sizeof(CgState), s));
+  HIP_TRY(hipMemsetAsync(P.oc, 0, sizeof(OcState), s));
+  return TOPT_OK;
+}
+
+int grid_prob(ProbDev& P, const topt_grid* g, void* ws, hipStream_t s) {
+  if (g == nullptr) return fail(TOPT_EINVAL, "grid is NULL");
+  int rc = op_prob(P, g->nelx, g->nely, ws, s);
+  if (rc) return rc;
+  return fill_material(P, g->Ke, g->E_min, g->penal);
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+const char* topt_last_error(void) { return g_err.c_str(); }
+
+int topt_device_count(int* n) {
+  HIP_TRY(hipGetDeviceCount(n));
+  return TOPT_OK;
+}
+
+int topt_problem_layout(int32_t nelx, int32_t nely, int32_t dtype, int32_t precond, double rmin,
+                        int64_t* offsets, int64_t* total_bytes) {
+  Layout L;
+  const int rc = make_layout(nelx, nely, dtype, precond, rmin, L);
+  if (rc) return rc;
+  if (offsets)
+    for (int i = 0; i < TOPT_BUF_COUNT; ++i) offsets[i] = L.off[i];
+  if (total_bytes) *total_bytes = L.total;
+  return TOPT_OK;
+}
+
+int topt_problem_init(const topt_problem* prob, int32_t dtype, void* stream) {
+  if (prob == nullptr) return fail(TOPT_EINVAL, "problem is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  ProbDev P;
+  int rc = fill_prob(P, *prob, dtype);
+  if (rc) return rc;
+  const std::vector<Tap> taps = build_taps(prob->rmin);
+  HIP_TRY(hipMemcpyAsync((void*)P.taps, taps.data(), taps.size() * sizeof(Tap), hipMemcpyHostToDevice, s));
+  const Geom g = geom(prob->nelx, prob->nely);
+  HIP_TRY(hipMemsetAsync(P.u64, 0, g.nn * 16, s));
+  HIP_TRY(hipMemsetAsync(P.cg, 0, sizeof(CgState), s));
+  HIP_TRY(hipMemsetAsync(P.oc, 0, sizeof(OcState), s));
+  P.b0[G_FILT] = 0;
+  hipLaunchKernelGGL((k_filter<false>), dim3(g.nb[G_FILT]), dim3(TOPT_BLOCK), filter_shmem(P.R), s, nullptr,
+                     nullptr, P, (int)F_HS);
+  LAUNCH_TRY();
+  HIP_TRY(hipStreamSynchronize(s));
+  return TOPT_OK;
+}
+
+int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, topt_plan** out) {
+  if (out == nullptr || probs == nullptr || nprob < 1) return fail(TOPT_EINVAL, "invalid plan arguments");
+  topt_plan* pl = new topt_plan();
+  pl->n = nprob;
+  pl->dtype = dtype;
+  pl->host.resize(nprob);
+  std::vector<int> maps[4];
+  for (int i = 0; i < nprob; ++i) {
+    ProbDev& P = pl->host[i];
+    const int rc = fill_prob(P, probs[i], dtype);
+    if (rc) {
+      delete pl;
+      return rc;
+    }
+    for (int g = 0; g < 4; ++g) {
+      P.b0[g] = (int)maps[g].size();
+      maps[g].insert(maps[g].end(), P.nb[g], i);
+    }
+    pl->any_elemblock |= (P.precond == PC_ELEMBLOCK);
+    pl->maxR = std::max(pl->maxR, P.R);
+    pl->max_cg = std::max(pl->max_cg, P.max_it);
+  }
+  for (int g = 0; g < 4; ++g) pl->nblk[g] = (int)maps[g].size();
+  auto cleanup = [&](int code) {
+    topt_plan_destroy(pl);
+    return code;
+  };
+  if (hipMalloc(&pl->dtab, sizeof(ProbDev) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc plan table"));
+  if (hipMemcpy(pl->dtab, pl->host.data(), sizeof(ProbDev) * nprob, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(TOPT_EHIP, "upload plan table"));
+  for (int g = 0; g < 4; ++g) {
+    if (hipMalloc(&pl->dmap[g], sizeof(int) * maps[g].size()) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipMalloc block map"));
+    if (hipMemcpy(pl->dmap[g], maps[g].data(), sizeof(int) * maps[g].size(), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(TOPT_EHIP, "upload block map"));
+  }
+  if (hipMalloc(&pl->dstat, sizeof(int) * 2 * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc status"));
+  if (hipMalloc(&pl->dstats, sizeof(topt_iter_stats) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc stats"));
+  for (int k = 0; k < 2; ++k) {
+    if (hipHostMalloc(&pl->hstat[k], sizeof(int) * 2 * nprob, hipHostMallocDefault) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc status"));
+    if (hipEventCreateWithFlags(&pl->ev[k], hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(TOPT_EHIP, "hipEventCreate"));
+  }
+  if (hipHostMalloc(&pl->hstats, sizeof(topt_iter_stats) * nprob, hipHostMallocDefault) != hipSuccess)
+    return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc stats"));
+  *out = pl;
+  return TOPT_OK;
+}
+
+int topt_plan_destroy(topt_plan* pl) {
+  if (pl == nullptr) return TOPT_OK;
+  (void)hipFree(pl->dtab);
+  for (int g = 0; g < 4; ++g) (void)hipFree(pl->dmap[g]);
+  (void)hipFree(pl->dstat);
+  (void)hipFree(pl->dstats);
+  for (int k = 0; k < 2; ++k) {
+    if (pl->hstat[k]) (void)hipHostFree(pl->hstat[k]);
+    if (pl->ev[k]) (void)hipEventDestroy(pl->ev[k]);
+  }
+  if (pl->hstats) (void)hipHostFree(pl->hstats);
+  delete pl;
+  return TOPT_OK;
+}
+
+int topt_plan_filter(topt_plan* pl, int32_t mode, void* stream) {
+  if (pl == nullptr || mode < 0 || mode > 3) return fail(TOPT_EINVAL, "invalid filter call");
+  return plan_filter(pl, mode, (hipStream_t)stream);
+}
+
+int topt_plan_solve(topt_plan* pl, int32_t warm, int32_t use_filtered, int32_t* n_iter, int32_t* status,
+                    void* stream) {
+  if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
+  return plan_solve(pl, warm, use_filtered, n_iter, status, (hipStream_t)stream);
+}
+
+int topt_plan_sensitivity(topt_plan* pl, int32_t use_filtered, void* stream) {
+  if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
+  return plan_sens(pl, use_filtered, (hipStream_t)stream);
+}
+
+int topt_plan_oc(topt_plan* pl, void* stream) {
+  if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
+  return plan_oc(pl, (hipStream_t)stream);
+}
+
+int topt_plan_iteration(topt_plan* pl, int32_t sens_filter, topt_iter_stats* stats, void* stream) {
+  if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  const int use_f = sens_filter ? 0 : 1;
+  if (!sens_filter && (rc = plan_filter(pl, F_DENSITY, s))) return rc;
+  if ((rc = plan_solve(pl, 2, use_f, nullptr, nullptr, s))) return rc;
+  if ((rc = plan_sens(pl, use_f, s))) return rc;
+  if ((rc = plan_filter(pl, sens_filter ? F_SENS : F_DCAVG, s))) return rc;
+  if ((rc = plan_oc(pl, s))) return rc;
+  if (stats) return gather_stats(pl, stats, s);
+  return TOPT_OK;
+}
+
+int topt_plan_stats(topt_plan* pl, topt_iter_stats* stats, void* stream) {
+  if (pl == nullptr || stats == nullptr) return fail(TOPT_EINVAL, "invalid stats call");
+  return gather_stats(pl, stats, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------- op level
+
+int64_t topt_op_workspace_bytes(int32_t nelx, int32_t nely) {
+  if (nelx < 1 || nely < 1) return -1;
+  return op_scratch(nelx, nely).total;
+}
+
+int topt_scale(const topt_grid* g, int32_t dtype, const double* rho, void* s, void* stream) {
+  if (g == nullptr || rho == nullptr || s == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  ProbDev P;
+  std::memset(&P, 0, sizeof(P));
+  P.nelx = g->nelx;
+  P.nely = g->nely;
+  P.Emin = g->E_min;
+  P.penal = g->penal;
+  P.rho = (double*)rho;
+  P.s = s;
+  const Geom gm = geom(g->nelx, g->nely);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TOPT_F32)
+    hipLaunchKernelGGL((k_scale<float, false>), dim3(gm.nb[G_ELEM]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P, 0);
+  else
+    hipLaunchKernelGGL((k_scale<double, false>), dim3(gm.nb[G_ELEM]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P, 0);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, const void* s, const uint32_t* fix,
+                double* pAp, void* ws, void* stream) {
+  if (p == nullptr || Ap == nullptr || s == nullptr || fix == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  int rc;
+  if (pAp != nullptr) {
+    if ((rc = grid_prob(P, g, ws, st))) return rc;
+  } else {
+    if (g == nullptr) return fail(TOPT_EINVAL, "grid is NULL");
+    std::memset(&P, 0, sizeof(P));
+    P.nelx = g->nelx;
+    P.nely = g->nely;
+    const Geom gm = geom(g->nelx, g->nely);
+    for (int k = 0; k < 4; ++k) P.nb[k] = gm.nb[k];
+    if ((rc = fill_material(P, g->Ke, g->E_min, g->penal))) return rc;
+  }
+  P.s = (void*)s;
+  P.fix = fix;
+  if (dtype == TOPT_F32)
+    hipLaunchKernelGGL((k_matvec<float>), dim3(P.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, P, (const float*)p,
+                       (float*)Ap, pAp);
+  else
+    hipLaunchKernelGGL((k_matvec<double>), dim3(P.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, P, (const double*)p,
+                       (double*)Ap, pAp);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+static int precond_op(const topt_grid* g, int32_t dtype, const void* s, const uint32_t* fix, void* pre,
+                      void* pre2, int kind, int raw, void* stream) {
+  if (g == nullptr || s == nullptr || fix == nullptr || pre == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  ProbDev P;
+  std::memset(&P, 0, sizeof(P));
+  P.nelx = g->nelx;
+  P.nely = g->nely;
+  const Geom gm = geom(g->nelx, g->nely);
+  for (int k = 0; k < 4; ++k) P.nb[k] = gm.nb[k];
+  int rc = fill_material(P, g->Ke, g->E_min, g->penal);
+  if (rc) return rc;
+  P.precond = kind;
+  P.s = (void*)s;
+  P.fix = fix;
+  P.pre = pre;
+  P.pre2 = pre2;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TOPT_F32)
+    hipLaunchKernelGGL((k_precond<float, false>), dim3(gm.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P, raw);
+  else
+    hipLaunchKernelGGL((k_precond<double, false>), dim3(gm.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P, raw);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int topt_diagonal(const topt_grid* g, int32_t dtype, const void* s, const uint32_t* fix, void* diag, void* stream) {
+  return precond_op(g, dtype, s, fix, diag, nullptr, PC_JACOBI, 1, stream);
+}
+
+int topt_block2_inverse(const topt_grid* g, int32_t dtype, const void* s, const uint32_t* fix, void* inv_diag,
+                        void* inv_off, void* stream) {
+  if (inv_off == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  return precond_op(g, dtype, s, fix, inv_diag, inv_off, PC_BLOCK2, 0, stream);
+}
+
+int topt_elemblock_apply(const topt_grid* g, int32_t dtype, const void* r, void* z, const void* s,
+                         const uint32_t* fix, double* rz_rr, void* ws, void* stream) {
+  if (r == nullptr || z == nullptr || s == nullptr || fix == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  int rc = grid_prob(P, g, ws, st);
+  if (rc) return rc;
+  P.s = (void*)s;
+  P.fix = fix;
+  if (dtype == TOPT_F32)
+    hipLaunchKernelGGL((k_elemblock<float>), dim3(P.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, P, (const float*)r,
+                       (float*)z, rz_rr);
+  else
+    hipLaunchKernelGGL((k_elemblock<double>), dim3(P.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, P, (const double*)r,
+                       (double*)z, rz_rr);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int topt_sensitivity(const topt_grid* g, const double* u, const double* rho, double* dc, double* dv,
+                     double* compliance, void* ws, void* stream) {
+  if (u == nullptr || rho == nullptr || dc == nullptr || compliance == nullptr)
+    return fail(TOPT_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  int rc = grid_prob(P, g, ws, st);
+  if (rc) return rc;
+  P.u64 = (double*)u;
+  P.rho = (double*)rho;
+  P.dc = dc;
+  hipLaunchKernelGGL((k_sens<false>), dim3(P.nb[G_ELEM]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P, 0);
+  LAUNCH_TRY();
+  HIP_TRY(hipMemcpyAsync(compliance, &P.oc->c, sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (dv != nullptr) {
+    std::vector<double> ones(size_t(g->nelx) * g->nely, 1.0);
+    HIP_TRY(hipMemcpyAsync(dv, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return TOPT_OK;
+}
+
+int topt_filter(int32_t nelx, int32_t nely, double rmin, int32_t mode, const double* rho, const double* dc,
+                double* out, double* Hs, void* ws, void* stream) {
+  if (out == nullptr || Hs == nullptr || mode < 0 || mode > 3) return fail(TOPT_EINVAL, "invalid filter arguments");
+  if (!(rmin > 0.0)) return fail(TOPT_EINVAL, "rmin must be > 0");
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  int rc = op_prob(P, nelx, nely, ws, st);
+  if (rc) return rc;
+  const std::vector<Tap> taps = build_taps(rmin);
+  if (tap_radius(taps) > 34) return fail(TOPT_EINVAL, "rmin too large for the op-level filter (max 34)");
+  const OpScratch o = op_scratch(nelx, nely);
+  P.taps = (const Tap*)((char*)ws + o.taps_off);
+  P.ntaps = (int)taps.size();
+  P.R = tap_radius(taps);
+  HIP_TRY(hipMemcpyAsync((void*)P.taps, taps.data(), taps.size() * sizeof(Tap), hipMemcpyHostToDevice, st));
+  P.rho = (double*)rho;
+  P.dc = (double*)dc;
+  P.Hs = Hs;
+  P.rhof = out;
+  P.dcf = out;
+  hipLaunchKernelGGL((k_filter<false>), dim3(P.nb[G_FILT]), dim3(TOPT_BLOCK), filter_shmem(P.R), st, nullptr,
+                     nullptr, P, (int)F_HS);
+  hipLaunchKernelGGL((k_filter<false>), dim3(P.nb[G_FILT]), dim3(TOPT_BLOCK), filter_shmem(P.R), st, nullptr,
+                     nullptr, P, (int)mode);
+  LAUNCH_TRY();
+  HIP_TRY(hipStreamSynchronize(st));   // taps upload came from pageable memory
+  return TOPT_OK;
+}
+
+int topt_filter_tables(int32_t nelx, int32_t nely, double rmin, int32_t width, int32_t* nb, float* wt,
+                       int32_t* cnt, void* ws, void* stream) {
+  if (nb == nullptr || wt == nullptr || cnt == nullptr || ws == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  if (nelx < 1 || nely < 1 || !(rmin > 0.0)) return fail(TOPT_EINVAL, "invalid grid or rmin");
+  const std::vector<Tap> taps = build_taps(rmin);
+  if ((int)taps.size() > width) return fail(TOPT_EINVAL, "table width smaller than the stencil");
+  if (tap_radius(taps) > 34) return fail(TOPT_EINVAL, "rmin too large (max 34)");
+  hipStream_t st = (hipStream_t)stream;
+  const OpScratch o = op_scratch(nelx, nely);
+  Tap* dt = (Tap*)((char*)ws + o.taps_off);
+  HIP_TRY(hipMemcpyAsync(dt, taps.data(), taps.size() * sizeof(Tap), hipMemcpyHostToDevice, st));
+  const int64_t ne = int64_t(nelx) * nely;
+  hipLaunchKernelGGL(k_filter_tables, dim3(cdiv(ne, TOPT_BLOCK)), dim3(TOPT_BLOCK), 0, st, nelx, nely, dt,
+                     (int)taps.size(), width, nb, wt, cnt);
+  LAUNCH_TRY();
+  HIP_TRY(hipStreamSynchronize(st));
+  return TOPT_OK;
+}
+
+int topt_oc_update(int32_t nelx, int32_t nely, double* rho, const double* dcf, const double* dv, double volfrac,
+                   double move, double* out, void* ws, void* stream) {
+  if (rho == nullptr || dcf == nullptr || out == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  int rc = op_prob(P, nelx, nely, ws, st);
+  if (rc) return rc;
+  P.rho = rho;
+  P.dcf = (double*)dcf;
+  P.dv = dv;
+  P.volfrac = volfrac;
+  P.move = move;
+  if (dv != nullptr) {
+    // non-unit dv: exact per-lambda evaluation is not implemented in the
+    // subtree pass; reject instead of silently assuming dv = 1
+    return fail(TOPT_EINVAL, "topt_oc_update: dv must be NULL (dv = 1, fem/kernels.py:270)");
+  }
+  constexpr int D = 5;
+  hipLaunchKernelGGL((k_oc_reset<false>), dim3(1), dim3(64), 0, st, nullptr, 1, P);
+  for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
+    hipLaunchKernelGGL((k_oc_eval<D, false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
+  hipLaunchKernelGGL((k_oc_apply<false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
+  LAUNCH_TRY();
+  hipLaunchKernelGGL(k_oc_out, dim3(1), dim3(1), 0, st, P.oc, out);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* ws, void* stream) {
+  if (v == nullptr || out == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  int rc = op_prob(P, nelx, nely, ws, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_volume, dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, P, v, out);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+}  // extern "C"

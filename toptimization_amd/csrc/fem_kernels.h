@@ -1,0 +1,596 @@
+#pragma once
+// FEM kernels of the SIMP hot path: element scales, preconditioner setup,
+// node-centric K(rho) matvec, the PCG slot kernels with device-side
+// alpha/beta/convergence, and the compliance sensitivity.
+//
+// Reference counterparts (src/toptimization/):
+//   fem/kernels.py:143-176 compute_matvec_2d      -> q4 gather (no atomics)
+//   fem/kernels.py:76-101  compute_diagonal_2d    -> k_precond (Jacobi)
+//   fem/preconditioner.py:47-108 apply_and_rz_2d  -> k_cg_c / refine B (element-block)
+//   fem/fused_kernels.py:21-56,110-187            -> k_cg_a / k_cg_b
+//   fem/solver.py:27-167 solve                    -> slot sequence driven by engine.hip (host)
+//   fem/kernels.py:234-272 compute_sensitivity_2d -> k_sens
+#include "topt_dev.h"
+
+namespace topt {
+
+// ---------------------------------------------------------------- helpers
+
+struct NodeIdx {
+  long n;      // node id (clamped to a valid node for inactive lanes)
+  int ix, iy;
+  bool act;
+};
+
+__device__ __forceinline__ NodeIdx node_of(const ProbDev& P, int lb) {
+  NodeIdx id;
+  const int nny = P.nely + 1;
+  const long nn = (long)(P.nelx + 1) * nny;
+  const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
+  id.act = n < nn;
+  id.n = id.act ? n : nn - 1;
+  id.ix = (int)(id.n / nny);
+  id.iy = (int)(id.n - (long)id.ix * nny);
+  return id;
+}
+
+// Gather a T2 vector over the 3x3 node neighbourhood.
+template <typename T>
+__device__ __forceinline__ void gather9(const vec2<T>* __restrict__ v, const NodeIdx& id,
+                                        const ProbDev& P, vec2<T> (&q)[3][3]) {
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) q[dx][dy] = v[nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely)];
+}
+
+// Element-block Jacobi (reference fem/preconditioner.py:47-108) in gather
+// form: z_n = c_n^-1/2 sum_{e ni n} w_e (Ki (r/sqrt(c))_e)[n], fixed -> 0.
+// c = number of elements sharing the node (4 interior, 2 edge, 1 corner).
+template <typename T>
+__device__ __forceinline__ vec2<T> elemblock_node(const ProbDev& P, const NodeIdx& id,
+                                                  const vec2<T>* __restrict__ r) {
+  const T* s = (const T*)P.s;
+  T sc[2][2];
+  load_scales<T>(s, id.ix, id.iy, P.nelx, P.nely, sc);
+  const T areg = (T)P.alpha_reg;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const T c = sc[a][b] > areg ? sc[a][b] : areg;
+      sc[a][b] = sc[a][b] / (c * c);   // zero scale (outside) stays zero
+    }
+  vec2<T> q[3][3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const int jx = id.ix + dx - 1, jy = id.iy + dy - 1;
+      const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
+      const int cx = (jx == 0 || jx == P.nelx) ? 1 : 2;
+      const int cy = (jy == 0 || jy == P.nely) ? 1 : 2;
+      const T inv = T(1) / sqrt((T)(cx * cy));
+      vec2<T> v = r[m];
+      v.x *= inv;
+      v.y *= inv;
+      q[dx][dy] = v;
+    }
+  T zx, zy;
+  q4_gather<T>(kiptr<T>(P), sc, q, zx, zy);
+  const int cx = (id.ix == 0 || id.ix == P.nelx) ? 1 : 2;
+  const int cy = (id.iy == 0 || id.iy == P.nely) ? 1 : 2;
+  const T inv = T(1) / sqrt((T)(cx * cy));
+  zx *= inv;
+  zy *= inv;
+  const unsigned fb = fixbits(P.fix, id.n);
+  vec2<T> z;
+  z.x = (fb & 1u) ? T(0) : zx;
+  z.y = (fb & 2u) ? T(0) : zy;
+  return z;
+}
+
+// z = M^-1 r at one node for the elementwise preconditioners.
+template <typename T>
+__device__ __forceinline__ vec2<T> precond_local(const ProbDev& P, long n, vec2<T> r) {
+  vec2<T> z;
+  if (P.precond == PC_JACOBI) {
+    const vec2<T> d = ((const vec2<T>*)P.pre)[n];
+    z.x = r.x * d.x;
+    z.y = r.y * d.y;
+  } else if (P.precond == PC_BLOCK2) {
+    const vec2<T> d = ((const vec2<T>*)P.pre)[n];
+    const T o = ((const T*)P.pre2)[n];
+    z.x = d.x * r.x + o * r.y;
+    z.y = o * r.x + d.y * r.y;
+  } else {
+    z = r;
+  }
+  return z;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// s_e = E_min + rho_e^p (1 - E_min) from the physical density
+// (reference fem/kernels.py:92). Writes s in T and, for the mixed path, FP64.
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_scale(const ProbDev* tab, const int* bmap,
+                                                      ProbDev single, int use_filtered) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  const long ne = (long)P.nelx * P.nely;
+  const long e = (long)(b - P.b0[G_ELEM]) * TOPT_BLOCK + threadIdx.x;
+  if (e >= ne) return;
+  const double* rho = use_filtered ? P.rhof : P.rho;
+  const double sv = P.Emin + pow(rho[e], P.penal) * (1.0 - P.Emin);
+  ((T*)P.s)[e] = (T)sv;
+  if (P.s64 != nullptr && (void*)P.s64 != P.s) P.s64[e] = sv;
+}
+
+// Preconditioner data from the element scales.
+// Jacobi: 1/diag, diag = sum s_e Ke_ii, fixed -> 1, clamp 1e-20 (fem/kernels.py:76-101).
+// 2x2 block: inverse of the assembled 2x2 diagonal block, fixed DOF -> identity
+// row/column (no reference counterpart).
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, const int* bmap,
+                                                        ProbDev single, int raw) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  if (P.precond != PC_JACOBI && P.precond != PC_BLOCK2) return;
+  const NodeIdx id = node_of(P, b - P.b0[G_NODE]);
+  if (!id.act) return;
+  const T* s = (const T*)P.s;
+  T sc[2][2];
+  load_scales<T>(s, id.ix, id.iy, P.nelx, P.nely, sc);
+  const T* K = keptr<T>(P);
+  T dxx = 0, dyy = 0, dxy = 0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) {
+      const int me = (a == 1) ? ((bb == 1) ? 0 : 3) : ((bb == 1) ? 1 : 2);
+      dxx += sc[a][bb] * K[(2 * me) * 8 + 2 * me];
+      dyy += sc[a][bb] * K[(2 * me + 1) * 8 + 2 * me + 1];
+      dxy += sc[a][bb] * K[(2 * me) * 8 + 2 * me + 1];
+    }
+  const unsigned fb = fixbits(P.fix, id.n);
+  vec2<T> out;
+  if (P.precond == PC_JACOBI) {
+    dxx = (fb & 1u) ? T(1) : (dxx < T(1e-20) ? T(1e-20) : dxx);
+    dyy = (fb & 2u) ? T(1) : (dyy < T(1e-20) ? T(1e-20) : dyy);
+    out.x = raw ? dxx : T(1) / dxx;
+    out.y = raw ? dyy : T(1) / dyy;
+    ((vec2<T>*)P.pre)[id.n] = out;
+  } else {
+    if (fb & 1u) { dxx = T(1); dxy = T(0); }
+    if (fb & 2u) { dyy = T(1); dxy = T(0); }
+    dxx = dxx < T(1e-20) ? T(1e-20) : dxx;
+    dyy = dyy < T(1e-20) ? T(1e-20) : dyy;
+    T det = dxx * dyy - dxy * dxy;
+    if (!(det > T(0))) { dxy = T(0); det = dxx * dyy; }
+    const T id_ = T(1) / det;
+    out.x = dyy * id_;
+    out.y = dxx * id_;
+    ((vec2<T>*)P.pre)[id.n] = out;
+    ((T*)P.pre2)[id.n] = -dxy * id_;
+  }
+}
+
+// Start of a solve: u[fixed] = 0 (or u = 0 cold / zero load), x = 0 (mixed),
+// p buffers = 0, CG state reset. warm: 0 cold, 1 warm, 2 warm iff the
+// problem has finished at least one OC iteration (fem/solver.py:77-83;
+// optimizer/simp.py:160 passes warm_start = it > 1).
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, const int* bmap,
+                                                           ProbDev single, int warm) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  const int lb = b - P.b0[G_NODE];
+  const bool zero_load = P.fnorm2 < 1e-60;
+  const int w = (warm == 2) ? (P.oc != nullptr && P.oc->iter > 0) : warm;
+  const NodeIdx id = node_of(P, lb);
+  if (id.act) {
+    double2* u = (double2*)P.u64;
+    double2 v = u[id.n];
+    const unsigned fb = fixbits(P.fix, id.n);
+    if (!w || zero_load) { v.x = 0.0; v.y = 0.0; }
+    if (fb & 1u) v.x = 0.0;
+    if (fb & 2u) v.y = 0.0;
+    u[id.n] = v;
+    vec2<T> zero;
+    zero.x = T(0);
+    zero.y = T(0);
+    if ((void*)P.x != (void*)P.u64) ((vec2<T>*)P.x)[id.n] = zero;
+    ((vec2<T>*)P.p0)[id.n] = zero;
+    ((vec2<T>*)P.p1)[id.n] = zero;
+  }
+  if (lb == 0 && threadIdx.x == 0) {
+    CgState* st = P.cg;
+    st->rz = st->rz_prev = st->alpha = st->beta = st->pAp = st->rr = st->rr_true = 0.0;
+    st->rmax2 = 0.0;
+    st->it = 0;
+    st->status = zero_load ? ST_ZERO : ST_RUN;
+    st->mode = MODE_REFINE;
+    st->pending = 0;
+    st->pcur = 0;
+    st->first = 1;
+    st->refines = 0;
+    st->pad = 0;
+  }
+}
+
+// Residual refresh r = f - A(u64 [+ x]) in FP64 (stored in T), ||r||^2 in
+// FP64. Used to start every solve and, on the mixed path, as the reliable
+// update that replaces the recursive residual.
+template <typename T>
+__device__ void refine_a(const ProbDev& P, int lb) {
+  __shared__ double sh[16];
+  CgState* st = P.cg;
+  const NodeIdx id = node_of(P, lb);
+  constexpr bool MIXED = sizeof(T) == 4;
+  const double2* u = (const double2*)P.u64;
+  const vec2<T>* x = (const vec2<T>*)P.x;
+  double2 q[3][3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
+      double2 v = u[m];
+      if (MIXED) {
+        const vec2<T> c = x[m];
+        v.x += (double)c.x;
+        v.y += (double)c.y;
+      }
+      q[dx][dy] = v;
+    }
+  double sc[2][2];
+  load_scales<double>(P.s64, id.ix, id.iy, P.nelx, P.nely, sc);
+  double ax, ay;
+  q4_gather<double>(P.Ke64, sc, q, ax, ay);
+  const unsigned fb = fixbits(P.fix, id.n);
+  if (fb & 1u) ax = q[1][1].x;
+  if (fb & 2u) ay = q[1][1].y;
+  const double2 f = ((const double2*)P.f)[id.n];
+  const double rx = f.x - ax, ry = f.y - ay;
+  double v[1] = {id.act ? rx * rx + ry * ry : 0.0};
+  if (id.act) {
+    vec2<T> r;
+    r.x = (T)rx;
+    r.y = (T)ry;
+    ((vec2<T>*)P.r)[id.n] = r;
+  }
+  block_sum<1>(v, sh);
+  double tot[1];
+  if (last_block_sum<1>(v, P.part, &st->cnt[0], lb, P.nb[G_NODE], tot, sh)) {
+    if (threadIdx.x == 0) {
+      st->rr_true = tot[0];
+      if (!st->first && tot[0] < P.tol2) st->pending = ST_CONV;
+    }
+  }
+}
+
+// Slot A: p = z + beta p (p buffers ping-pong), Ap = K(s) p, p.Ap (FP64),
+// alpha = rz / pAp by the last block; |pAp| < 1e-30 -> breakdown
+// (fem/fused_kernels.py:21-56 + fem/solver.py:110-117, 151-154).
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_a(const ProbDev* tab, const int* bmap,
+                                                     ProbDev single) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  CgState* st = P.cg;
+  if (st->status != ST_RUN) return;
+  const int lb = b - P.b0[G_NODE];
+  if (st->mode == MODE_REFINE) {
+    refine_a<T>(P, lb);
+    return;
+  }
+  __shared__ double sh[16];
+  const NodeIdx id = node_of(P, lb);
+  const int pc = st->pcur;
+  const T beta = (T)st->beta;
+  const vec2<T>* pold = (const vec2<T>*)(pc ? P.p1 : P.p0);
+  vec2<T>* pnew = (vec2<T>*)(pc ? P.p0 : P.p1);
+  const vec2<T>* z = (const vec2<T>*)P.z;
+  vec2<T> q[3][3];
+#pragma unroll
+  for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
+      const vec2<T> zv = z[m], pv = pold[m];
+      vec2<T> v;
+      v.x = zv.x + beta * pv.x;
+      v.y = zv.y + beta * pv.y;
+      q[dx][dy] = v;
+    }
+  T sc[2][2];
+  load_scales<T>((const T*)P.s, id.ix, id.iy, P.nelx, P.nely, sc);
+  T ax, ay;
+  q4_gather<T>(keptr<T>(P), sc, q, ax, ay);
+  const unsigned fb = fixbits(P.fix, id.n);
+  if (fb & 1u) ax = q[1][1].x;
+  if (fb & 2u) ay = q[1][1].y;
+  double v[1] = {0.0};
+  if (id.act) {
+    pnew[id.n] = q[1][1];
+    vec2<T> o;
+    o.x = ax;
+    o.y = ay;
+    ((vec2<T>*)P.Ap)[id.n] = o;
+    v[0] = (double)q[1][1].x * (double)ax + (double)q[1][1].y * (double)ay;
+  }
+  block_sum<1>(v, sh);
+  double tot[1];
+  if (last_block_sum<1>(v, P.part, &st->cnt[1], lb, P.nb[G_NODE], tot, sh)) {
+    if (threadIdx.x == 0) {
+      st->pcur = pc ^ 1;
+      st->pAp = tot[0];
+      if (fabs(tot[0]) < 1e-30) {
+        if (sizeof(T) == 4) {   // mixed: commit the correction before stopping
+          st->pending = ST_BREAK;
+          st->mode = MODE_REFINE;
+        } else {
+          st->status = ST_BREAK;
+        }
+      } else {
+        st->alpha = st->rz / tot[0];
+      }
+    }
+  }
+}
+
+// Convergence bookkeeping after a CG update (fem/solver.py:146-156).
+// rzn < 0 means "rz computed later by slot C" (element-block Jacobi).
+template <typename T>
+__device__ __forceinline__ void cg_after_update(const ProbDev& P, CgState* st, double rr,
+                                                double rzn, bool have_rz) {
+  constexpr bool MIXED = sizeof(T) == 4;
+  st->rr = rr;
+  st->it += 1;
+  if (rr < P.tol2) {
+    if (!MIXED) { st->status = ST_CONV; return; }
+    st->mode = MODE_REFINE;     // confirm on the true FP64 residual
+  } else if (st->it >= P.max_it) {
+    if (!MIXED) { st->status = ST_MAXIT; return; }
+    st->pending = ST_MAXIT;
+    st->mode = MODE_REFINE;
+  } else if (MIXED && rr < P.delta2 * st->rmax2) {
+    st->mode = MODE_REFINE;     // reliable update
+  }
+  if (have_rz) {
+    st->beta = rzn / (st->rz + 1e-60);
+    st->rz_prev = st->rz;
+    st->rz = rzn;
+  } else {
+    st->pad = 1;                // slot C must run
+  }
+}
+
+// Slot B. CG mode: x += alpha p, r -= alpha Ap, z = M^-1 r, r.z and r.r in
+// FP64, beta and convergence by the last block
+// (fem/fused_kernels.py:110-166 update_and_precond_*; element-block Jacobi
+// defers z to slot C as fem/fused_kernels.py:169-180 + preconditioner.py do).
+// Refine mode: u64 += x, x = 0 (mixed), z = M^-1 r, r.z; beta from the
+// previous r.z; after the very first refresh p = z (beta = 0).
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_b(const ProbDev* tab, const int* bmap,
+                                                     ProbDev single) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  CgState* st = P.cg;
+  if (st->status != ST_RUN) return;
+  constexpr bool MIXED = sizeof(T) == 4;
+  __shared__ double sh[16];
+  const int lb = b - P.b0[G_NODE];
+  const NodeIdx id = node_of(P, lb);
+  vec2<T>* r = (vec2<T>*)P.r;
+  vec2<T>* z = (vec2<T>*)P.z;
+  vec2<T>* x = (vec2<T>*)P.x;
+
+  if (st->mode == MODE_REFINE) {
+    double v[1] = {0.0};
+    vec2<T> zz;
+    if (P.precond == PC_ELEMBLOCK) zz = elemblock_node<T>(P, id, r);
+    if (id.act) {
+      if (MIXED) {
+        double2* u = (double2*)P.u64;
+        double2 uv = u[id.n];
+        const vec2<T> c = x[id.n];
+        uv.x += (double)c.x;
+        uv.y += (double)c.y;
+        u[id.n] = uv;
+        vec2<T> zero;
+        zero.x = T(0);
+        zero.y = T(0);
+        x[id.n] = zero;
+      }
+      const vec2<T> rv = r[id.n];
+      if (P.precond != PC_ELEMBLOCK) zz = precond_local<T>(P, id.n, rv);
+      z[id.n] = zz;
+      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+    }
+    block_sum<1>(v, sh);
+    double tot[1];
+    if (last_block_sum<1>(v, P.part, &st->cnt[2], lb, P.nb[G_NODE], tot, sh)) {
+      if (threadIdx.x == 0) {
+        if (st->pending) {
+          st->status = st->pending;
+        } else {
+          st->beta = st->first ? 0.0 : tot[0] / (st->rz_prev + 1e-60);
+          st->rz = tot[0];
+          st->first = 0;
+          st->mode = MODE_CG;
+          st->rmax2 = st->rr_true;
+          st->refines += 1;
+        }
+      }
+    }
+    return;
+  }
+
+  const T alpha = (T)st->alpha;
+  const int pc = st->pcur;
+  const vec2<T>* p = (const vec2<T>*)(pc ? P.p1 : P.p0);
+  double v[2] = {0.0, 0.0};
+  if (id.act) {
+    const vec2<T> pv = p[id.n], ap = ((const vec2<T>*)P.Ap)[id.n];
+    vec2<T> xv = x[id.n], rv = r[id.n];
+    xv.x += alpha * pv.x;
+    xv.y += alpha * pv.y;
+    rv.x -= alpha * ap.x;
+    rv.y -= alpha * ap.y;
+    x[id.n] = xv;
+    r[id.n] = rv;
+    v[1] = (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+    if (P.precond != PC_ELEMBLOCK) {
+      const vec2<T> zz = precond_local<T>(P, id.n, rv);
+      z[id.n] = zz;
+      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+    }
+  }
+  block_sum<2>(v, sh);
+  double tot[2];
+  if (last_block_sum<2>(v, P.part, &st->cnt[2], lb, P.nb[G_NODE], tot, sh)) {
+    if (threadIdx.x == 0) cg_after_update<T>(P, st, tot[1], tot[0], P.precond != PC_ELEMBLOCK);
+  }
+}
+
+// Slot C (element-block Jacobi only): z = M^-1 r by gather, r.z, beta
+// (fem/preconditioner.py:47-108 apply_and_rz_2d).
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_c(const ProbDev* tab, const int* bmap,
+                                                     ProbDev single) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  CgState* st = P.cg;
+  if (st->status != ST_RUN || st->pad == 0) return;
+  __shared__ double sh[16];
+  const int lb = b - P.b0[G_NODE];
+  const NodeIdx id = node_of(P, lb);
+  const vec2<T>* r = (const vec2<T>*)P.r;
+  const vec2<T> zz = elemblock_node<T>(P, id, r);
+  double v[1] = {0.0};
+  if (id.act) {
+    ((vec2<T>*)P.z)[id.n] = zz;
+    const vec2<T> rv = r[id.n];
+    v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+  }
+  block_sum<1>(v, sh);
+  double tot[1];
+  if (last_block_sum<1>(v, P.part, &st->cnt[3], lb, P.nb[G_NODE], tot, sh)) {
+    if (threadIdx.x == 0) {
+      st->beta = tot[0] / (st->rz + 1e-60);
+      st->rz_prev = st->rz;
+      st->rz = tot[0];
+      st->pad = 0;
+    }
+  }
+}
+
+// Compliance and sensitivities (fem/kernels.py:234-272), FP64:
+// dc_e = -p rho^(p-1) (1-E_min) u_e^T Ke u_e, dv_e = 1, c = sum s_e u_e^T Ke u_e.
+// rho is the physical density (filtered in density mode).
+template <bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const int* bmap,
+                                                     ProbDev single, int use_filtered) {
+  __shared__ double sh[16];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  const int lb = b - P.b0[G_ELEM];
+  const long ne = (long)P.nelx * P.nely;
+  const long e0 = (long)lb * TOPT_BLOCK + threadIdx.x;
+  const bool act = e0 < ne;
+  const long e = act ? e0 : ne - 1;
+  const int ex = (int)(e / P.nely), ey = (int)(e - (long)ex * P.nely);
+  const int nny = P.nely + 1;
+  const double2* u = (const double2*)P.u64;
+  const long n1 = (long)ex * nny + ey;
+  double ue[8];
+  const double2 a0 = u[n1], a1 = u[n1 + nny], a2 = u[n1 + nny + 1], a3 = u[n1 + 1];
+  ue[0] = a0.x; ue[1] = a0.y; ue[2] = a1.x; ue[3] = a1.y;
+  ue[4] = a2.x; ue[5] = a2.y; ue[6] = a3.x; ue[7] = a3.y;
+  double uKu = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t += P.Ke64[i * 8 + j] * ue[j];
+    uKu += ue[i] * t;
+  }
+  const double* rho = use_filtered ? P.rhof : P.rho;
+  const double re = rho[e];
+  const double sv = P.Emin + pow(re, P.penal) * (1.0 - P.Emin);
+  double v[1] = {0.0};
+  if (act) {
+    P.dc[e] = -P.penal * pow(re, P.penal - 1.0) * (1.0 - P.Emin) * uKu;
+    v[0] = sv * uKu;
+  }
+  block_sum<1>(v, sh);
+  double tot[1];
+  if (last_block_sum<1>(v, P.opart, &P.oc->cnt[0], lb, P.nb[G_ELEM], tot, sh)) {
+    if (threadIdx.x == 0) P.oc->c = tot[0];
+  }
+}
+
+// Op-level K(s) p with the reference BC rule (fixed rows -> identity), and
+// optionally p.Ap into out[0] (fem/kernels.py:143-176, fused_kernels.py:21-56).
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev P, const T* __restrict__ pin,
+                                                       T* __restrict__ Ap, double* out) {
+  __shared__ double sh[16];
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const NodeIdx id = node_of(P, lb);
+  vec2<T> q[3][3];
+  gather9<T>((const vec2<T>*)pin, id, P, q);
+  T sc[2][2];
+  load_scales<T>((const T*)P.s, id.ix, id.iy, P.nelx, P.nely, sc);
+  T ax, ay;
+  q4_gather<T>(keptr<T>(P), sc, q, ax, ay);
+  const unsigned fb = fixbits(P.fix, id.n);
+  if (fb & 1u) ax = q[1][1].x;
+  if (fb & 2u) ay = q[1][1].y;
+  double v[1] = {0.0};
+  if (id.act) {
+    vec2<T> o;
+    o.x = ax;
+    o.y = ay;
+    ((vec2<T>*)Ap)[id.n] = o;
+    v[0] = (double)q[1][1].x * (double)ax + (double)q[1][1].y * (double)ay;
+  }
+  if (out == nullptr) return;
+  block_sum<1>(v, sh);
+  double tot[1];
+  if (last_block_sum<1>(v, P.part, &P.cg->cnt[0], lb, P.nb[G_NODE], tot, sh)) {
+    if (threadIdx.x == 0) out[0] = tot[0];
+  }
+}
+
+// Op-level element-block Jacobi apply (+ r.z, r.r) for drop-in callers.
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_elemblock(ProbDev P, const T* __restrict__ rin,
+                                                          T* __restrict__ zout, double* out) {
+  __shared__ double sh[16];
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const NodeIdx id = node_of(P, lb);
+  const vec2<T>* r = (const vec2<T>*)rin;
+  const vec2<T> zz = elemblock_node<T>(P, id, r);
+  double v[2] = {0.0, 0.0};
+  if (id.act) {
+    ((vec2<T>*)zout)[id.n] = zz;
+    const vec2<T> rv = r[id.n];
+    v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+    v[1] = (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+  }
+  if (out == nullptr) return;
+  block_sum<2>(v, sh);
+  double tot[2];
+  if (last_block_sum<2>(v, P.part, &P.cg->cnt[0], lb, P.nb[G_NODE], tot, sh)) {
+    if (threadIdx.x == 0) {
+      out[0] = tot[0];
+      out[1] = tot[1];
+    }
+  }
+}
+
+}  // namespace topt
