@@ -1,0 +1,217 @@
+"""Benchmark: SIMP OC iterations/s on one MI355X (1200x400 bridge, mixed precision).
+
+A step is one OC iteration of the SIMP loop (density filter -> PCG ->
+sensitivity -> dc filter -> OC bisection) over the 1200x400 Q4 bridge:
+bottom corners pinned, total unit load spread over the top row, volfrac 0.3,
+rmin 6, p 3, PCG rtol 1e-6 with FP32 vectors, FP64 dots and FP64 residual
+refresh, 2x2 block Jacobi. Inputs live in HBM before the timed region.
+
+Also reported (same JSON line): PCG iterations/s, matvec GDOF/s, the plain
+FP64 GPU path on the same iterations, the compliance agreement of the two
+paths on one density, the roofline of the dominant kernel (PCG slot A:
+p = z + beta p, Ap = K p, p.Ap) timed with HIP events, and the C
+restatement of the reference algorithm (FP32, Jacobi, as the reference) on
+the host cores as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NELX, NELY = 1200, 400
+
+
+def bridge(nelx, nely, precision, preconditioner="block_jacobi_2x2", tol=1e-6):
+    from toptimization_amd.problem import make_problem
+    return make_problem(
+        nelx, nely,
+        fixed=[{"nodes": {"type": "point", "corner": "bottom-left"}, "dofs": ["x", "y"]},
+               {"nodes": {"type": "point", "corner": "bottom-right"}, "dofs": ["x", "y"]}],
+        loads=[{"nodes": {"type": "edge", "edge": "top"}, "force": [0.0, -1.0 / (nelx + 1)]}],
+        name="bridge", volume_fraction=0.3, filter_radius=6.0, penalty=3.0, max_iterations=200,
+        tolerance=0.0, cg_tolerance=tol, max_cg_iterations=50000, preconditioner=preconditioner,
+        precision=precision)
+
+
+def run_path(precision, warmup, steps, log):
+    import torch
+    from toptimization_amd.engine import DesignState, Plan
+    prob = bridge(NELX, NELY, precision)
+    st = DesignState(prob)
+    plan = Plan([st])
+    for k in range(warmup):
+        s = plan.iteration()[0]
+        log(f"  [{precision}] warmup it {k + 1}: c={s.compliance:.6e} n_cg={s.n_cg} refines={s.refines}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [plan.iteration()[0] for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ncg = sum(s.n_cg for s in stats)
+    log(f"  [{precision}] timed {steps} it: {dt:.3f} s, {ncg} CG iterations, "
+        f"last c={stats[-1].compliance:.6e}")
+    return {"seconds": dt, "n_cg": ncg, "stats": stats, "state": st, "plan": plan, "problem": prob}
+
+
+def same_rho_compliance(rho, log):
+    """Solve the same density with the mixed and the plain FP64 path; compare c."""
+    from toptimization_amd.engine import DesignState, Plan
+    out = {}
+    for prec in ("mixed", "fp64"):
+        st = DesignState(bridge(NELX, NELY, prec), rho0=rho)
+        plan = Plan([st])
+        plan.filter(0)
+        it, status = plan.solve(warm=0, use_filtered=True)
+        plan.sensitivity(use_filtered=True)
+        out[prec] = (plan.stats()[0].compliance, it[0])
+        plan.close()
+    rel = abs(out["mixed"][0] - out["fp64"][0]) / abs(out["fp64"][0])
+    log(f"  same-rho compliance: mixed {out['mixed'][0]:.12e} ({out['mixed'][1]} CG) vs fp64 "
+        f"{out['fp64'][0]:.12e} ({out['fp64'][1]} CG): rel {rel:.2e}")
+    return rel
+
+
+def kernel_roofline(plan, nn, ne, log, reps=200):
+    """Average duration of PCG slot A over back-to-back launches (HIP events)."""
+    ms = plan.time_slot(0, reps)
+    # FP32 slot A moves: read z, p_old (8 B/node each), s (4 B/elem); write p_new, Ap (8 B/node each)
+    bytes_a = 32 * nn + 4 * ne
+    gbs = bytes_a / (ms * 1e-3) / 1e9
+    log(f"  slot A: {ms * 1e3:.2f} us/launch, {bytes_a / 1e6:.2f} MB -> {gbs:.0f} GB/s")
+    return ms, bytes_a, gbs
+
+
+def matvec_gdofs(nelx, nely, log, reps=200):
+    """Op-level FP32 K(s) p (no dot) back to back, torch events on the launch stream."""
+    import numpy as np
+    import torch
+    from toptimization_amd import ops
+    from toptimization_amd.material import compute_Ke_2d
+    nn = (nelx + 1) * (nely + 1)
+    g = ops.make_grid(nelx, nely, compute_Ke_2d(1.0, 0.3), 1e-9, 3.0)
+    rng = np.random.default_rng(0)
+    rho = torch.as_tensor(rng.uniform(0.001, 1.0, nelx * nely), device="cuda")
+    s = ops.scale(g, rho, torch.float32)
+    p = torch.as_tensor(rng.normal(size=2 * nn), dtype=torch.float32, device="cuda")
+    fix = ops.fix_mask(2 * nn, [0, 1])
+    Ap = torch.empty_like(p)
+    for _ in range(5):
+        ops.matvec(g, p, s, fix, Ap=Ap)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ops.matvec(g, p, s, fix, Ap=Ap)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gdofs = 2 * nn / (ms * 1e-3) / 1e9
+    bytes_mv = 4 * (2 * 2 * nn + nelx * nely)
+    log(f"  matvec {nelx}x{nely}: {ms * 1e3:.2f} us, {gdofs:.2f} GDOF/s, "
+        f"{bytes_mv / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
+    return ms, gdofs, bytes_mv / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(log, budget_s=25.0):
+    """C restatement of the reference (FP32, Jacobi, host cores) on the first
+    OC iterations of the same workload, bounded to about budget_s."""
+    from oracle import build as obuild
+    from oracle import c_oracle
+    obuild.build() if not c_oracle.LIB.exists() else None
+    cores = len(os.sched_getaffinity(0))
+    c_oracle.set_threads(cores)
+    prob = bridge(NELX, NELY, "fp64", preconditioner="jacobi")
+    n_it = 1
+    res = c_oracle.simp(prob, "f32", max_iter=n_it, preconditioner="jacobi")
+    if res["seconds"] < budget_s / 3:
+        n_it = max(1, min(4, int(budget_s / max(res["seconds"], 1e-3))))
+        res = c_oracle.simp(prob, "f32", max_iter=n_it, preconditioner="jacobi")
+    rate = res["iterations"] / res["seconds"]
+    log(f"  CPU (C restatement, FP32 Jacobi, {cores} threads): {res['iterations']} OC it in "
+        f"{res['seconds']:.2f} s, CG {res['n_cg']}")
+    return {"value": rate, "unit": "OC it/s", "cores": cores, "kind": "port",
+            "sample": f"first {res['iterations']} OC iteration(s) of the {NELX}x{NELY} bridge from rho=0.3, "
+                      f"FP32 Jacobi PCG rtol 1e-6 ({sum(res['n_cg'])} CG iterations), C+OpenMP "
+                      f"restatement of the reference algorithm"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a design problem is "
+              "solved on one device and independent problems shard by problem, so there is no "
+              "multi-GPU figure to report", file=sys.stderr)
+        return 2
+    log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
+    import torch
+    torch.cuda.set_device(0)
+    nn, ne = (NELX + 1) * (NELY + 1), NELX * NELY
+
+    log(f"[bench] {NELX}x{NELY} bridge, warmup {args.warmup}, steps {args.steps}")
+    mixed = run_path("mixed", args.warmup, args.steps, log)
+    ms_a, bytes_a, gbs_a = kernel_roofline(mixed["plan"], nn, ne, log)
+    rho_probe = mixed["state"].rho.cpu().numpy()
+    mixed["plan"].close()
+    fp64 = run_path("fp64", args.warmup, args.steps, log)
+    fp64["plan"].close()
+    rel_c = same_rho_compliance(rho_probe, log)
+    mv_ms, mv_gdofs, mv_gbs = matvec_gdofs(NELX, NELY, log)
+    mvb_ms, mvb_gdofs, mvb_gbs = matvec_gdofs(8192, 4096, log, reps=20)
+    cpu = None if args.no_cpu else cpu_baseline(log)
+
+    value = args.steps / mixed["seconds"]
+    fp64_value = args.steps / fp64["seconds"]
+    line = {
+        "metric": "FEM matvec GDOF/s, OC iters/s, serving p50/p99 latency at compliance err <=1e-6",
+        "value": round(value, 3),
+        "unit": "OC it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 vectors + f64 dots/refresh/design fields",
+        "data": "synthetic (bridge load case, rho0 = volfrac)",
+        "config": {"workload": f"SIMP bridge {NELX}x{NELY} Q4, volfrac 0.3, rmin 6, p 3, "
+                               "PCG rtol 1e-6 mixed FP32/FP64 2x2 block Jacobi",
+                   "nelx": NELX, "nely": NELY, "dofs": 2 * nn, "elements": ne,
+                   "parallelism": "single-gpu"},
+        "pcg_its_per_s": round(mixed["n_cg"] / mixed["seconds"], 1),
+        "cg_per_oc_it": round(mixed["n_cg"] / args.steps, 1),
+        "fp64_oc_its_per_s": round(fp64_value, 3),
+        "fp64_cg_per_oc_it": round(fp64["n_cg"] / args.steps, 1),
+        "mixed_over_fp64": round(value / fp64_value, 3),
+        "same_rho_compliance_rel_err": rel_c,
+        "matvec_gdofs": round(mv_gdofs, 2),
+        "matvec_gbs_algorithmic": round(mv_gbs, 1),
+        "matvec_8192x4096_gdofs": round(mvb_gdofs, 2),
+        "matvec_8192x4096_gbs": round(mvb_gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_cg_a<float> (PCG slot A)",
+                     "achieved": round(gbs_a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs_a / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_a, "us_per_launch": round(ms_a * 1e3, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -125,6 +125,12 @@ int topt_plan_iteration(topt_plan* plan, int32_t sens_filter, topt_iter_stats* s
 /* Read back per-problem stats of the last iteration (blocks). */
 int topt_plan_stats(topt_plan* plan, topt_iter_stats* stats, void* stream);
 
+/* Benchmark support: run `reps` back-to-back launches of PCG slot kernel A
+ * (which must be 0: p = z + beta p, Ap = K p, p.Ap) on the plan's current
+ * vectors and return the elapsed GPU time (hipEvents on the stream) in *ms.
+ * Leaves every problem marked converged. No reference counterpart. Blocks. */
+int topt_plan_time_slot(topt_plan* plan, int32_t which, int32_t reps, float* ms, void* stream);
+
 /* ---- op-level kernels (single problem, explicit buffers) ---- */
 typedef struct topt_grid {
   int32_t nelx, nely;
@@ -166,6 +172,25 @@ int topt_oc_update(int32_t nelx, int32_t nely, double* rho, const double* dcf, c
                    double volfrac, double move, double* out, void* ws, void* stream);
 /* mean of an element field (fem/kernels.py:330-336) into out[0] (device). */
 int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* ws, void* stream);
+
+/* Vector ops of the drop-in API, op codes (buffers by role x, y, z, w, v):
+ *  0 FILL      x = a                              fem/kernels.py:19-23
+ *  1 COPY      y = x                              fem/kernels.py:26-30
+ *  2 AXPY      z = a x + y                        fem/kernels.py:33-37
+ *  3 SCALE     z = a x                            fem/kernels.py:40-44
+ *  4 DIV       z = x / y                          fem/kernels.py:65-73
+ *  5 DOT       red[0] = x.y (FP64)                fem/kernels.py:47-62
+ *  6 MAXDIFF   red[0] = max|x - y|                fem/kernels.py:339-345
+ *  7 SUM       red[0] = sum x                     fem/kernels.py:330-336
+ *  8 UPD_JAC   x=u += a w(=p); y=r -= a v(=Ap); z = r / d(=diag);
+ *              red = (r.z, r.r)                   fem/fused_kernels.py:110-139
+ *  9 UPD_NONE  as 8 with z = r                    fem/fused_kernels.py:142-166
+ * 10 UPD_UR    x=u += a w; y=r -= a v             fem/fused_kernels.py:169-180
+ * 11 UPD_P     w=p = z + a w                      fem/fused_kernels.py:183-187
+ * 12 ZERO_FIXED x[i] = 0 where bit i of fix set   fem/solver.py:162-167 */
+int64_t topt_blas1_workspace_bytes(void);
+int topt_blas1(int32_t op, int32_t dtype, int64_t n, double a, void* x, void* y, void* z, void* w, void* v,
+               const void* d, const uint32_t* fix, double* red, void* ws, void* stream);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,7 @@ _SIGS = {
     "topt_plan_oc": (C.c_int, [_vp, _vp]),
     "topt_plan_iteration": (C.c_int, [_vp, _i32, C.POINTER(TIterStats), _vp]),
     "topt_plan_stats": (C.c_int, [_vp, C.POINTER(TIterStats), _vp]),
+    "topt_plan_time_slot": (C.c_int, [_vp, _i32, _i32, C.POINTER(C.c_float), _vp]),
     "topt_op_workspace_bytes": (_i64, [_i32, _i32]),
     "topt_scale": (C.c_int, [C.POINTER(TGrid), _i32, _vp, _vp, _vp]),
     "topt_matvec": (C.c_int, [C.POINTER(TGrid), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -67,7 +68,12 @@ _SIGS = {
     "topt_filter_tables": (C.c_int, [_i32, _i32, _d, _i32, _vp, _vp, _vp, _vp, _vp]),
     "topt_oc_update": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp]),
     "topt_volume": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _vp]),
+    "topt_blas1_workspace_bytes": (_i64, []),
+    "topt_blas1": (C.c_int, [_i32, _i32, _i64, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
+
+B1 = {"fill": 0, "copy": 1, "axpy": 2, "scale": 3, "div": 4, "dot": 5, "maxdiff": 6, "sum": 7,
+      "upd_jacobi": 8, "upd_none": 9, "upd_ur": 10, "upd_p": 11, "zero_fixed": 12}
 
 _LIB: C.CDLL | None = None
 

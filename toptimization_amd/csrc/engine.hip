@@ -253,6 +253,25 @@ __global__ void k_gather_stats(const ProbDev* tab, int n, topt_iter_stats* out) 
   out[i] = s;
 }
 
+// Bench support: put every problem back into CG mode so slot kernels can be
+// timed back to back on the current vectors; k_cg_finish ends it again.
+__global__ void k_cg_force_run(const ProbDev* tab, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CgState* st = tab[i].cg;
+  st->status = ST_RUN;
+  st->mode = MODE_CG;
+  st->pending = 0;
+  st->pad = 0;
+  st->it = 0;
+}
+
+__global__ void k_cg_finish(const ProbDev* tab, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tab[i].cg->status = ST_CONV;
+}
+
 __global__ void k_oc_out(const OcState* o, double* out) {
   out[0] = o->change;
   out[1] = o->vol;
@@ -594,6 +613,38 @@ int topt_plan_iteration(topt_plan* pl, int32_t sens_filter, topt_iter_stats* sta
   return TOPT_OK;
 }
 
+int topt_plan_time_slot(topt_plan* pl, int32_t which, int32_t reps, float* ms, void* stream) {
+  // Only slot A can be replayed: slot B would re-detect convergence and stop.
+  if (pl == nullptr || ms == nullptr || reps < 1 || which != 0)
+    return fail(TOPT_EINVAL, "invalid slot timing call (only which = 0 is supported)");
+  hipStream_t s = (hipStream_t)stream;
+  const ProbDev dummy{};
+  const int nb = pl->nblk[G_NODE];
+  // max_it is never reached: the timed launches do not count iterations past max_cg
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_cg_force_run, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
+  HIP_TRY(hipEventRecord(e0, s));
+  for (int k = 0; k < reps; ++k) {
+    if (pl->dtype == TOPT_F32) {
+      if (which == 0) hipLaunchKernelGGL((k_cg_a<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+      else hipLaunchKernelGGL((k_cg_b<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+    } else {
+      if (which == 0) hipLaunchKernelGGL((k_cg_a<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+      else hipLaunchKernelGGL((k_cg_b<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+    }
+  }
+  HIP_TRY(hipEventRecord(e1, s));
+  hipLaunchKernelGGL(k_cg_finish, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
+  LAUNCH_TRY();
+  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return TOPT_OK;
+}
+
 int topt_plan_stats(topt_plan* pl, topt_iter_stats* stats, void* stream) {
   if (pl == nullptr || stats == nullptr) return fail(TOPT_EINVAL, "invalid stats call");
   return gather_stats(pl, stats, (hipStream_t)stream);
@@ -814,6 +865,35 @@ int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* 
   int rc = op_prob(P, nelx, nely, ws, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_volume, dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, P, v, out);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+// Vector ops of the drop-in API (fem/kernels.py:19-73,330-345,
+// fem/fused_kernels.py:110-187, fem/solver.py:162-167). Buffers by role:
+// x, y, z, w, v as documented per op in include/topt.h; red[2] (device)
+// receives reductions. ws: >= topt_blas1_workspace_bytes().
+int64_t topt_blas1_workspace_bytes(void) { return 1024 * 16 + 256; }
+
+int topt_blas1(int32_t op, int32_t dtype, int64_t n, double a, void* x, void* y, void* z, void* w, void* v,
+               const void* d, const uint32_t* fix, double* red, void* ws, void* stream) {
+  if (op < B1_FILL || op > B1_ZERO_FIXED || n < 0) return fail(TOPT_EINVAL, "invalid blas1 op");
+  if (n == 0) return TOPT_OK;
+  const bool reduce = op == B1_DOT || op == B1_MAXDIFF || op == B1_SUM || op == B1_UPD_JAC || op == B1_UPD_NONE;
+  if (reduce && (red == nullptr || ws == nullptr)) return fail(TOPT_EINVAL, "reduction needs red and ws");
+  if (op == B1_ZERO_FIXED && fix == nullptr) return fail(TOPT_EINVAL, "zero_fixed needs the bitmask");
+  if (op == B1_UPD_JAC && d == nullptr) return fail(TOPT_EINVAL, "jacobi update needs the diagonal");
+  hipStream_t s = (hipStream_t)stream;
+  const int grid = (int)std::min<int64_t>(1024, cdiv(n, TOPT_BLOCK));
+  double* part = (double*)ws;
+  unsigned* counter = (unsigned*)((char*)ws + 1024 * 16);
+  if (reduce) HIP_TRY(hipMemsetAsync(counter, 0, sizeof(unsigned), s));
+  if (dtype == TOPT_F32)
+    hipLaunchKernelGGL((k_blas1<float>), dim3(grid), dim3(TOPT_BLOCK), 0, s, (int)op, (long)n, a, (float*)x,
+                       (float*)y, (float*)z, (float*)w, (float*)v, (const float*)d, fix, part, counter, red);
+  else
+    hipLaunchKernelGGL((k_blas1<double>), dim3(grid), dim3(TOPT_BLOCK), 0, s, (int)op, (long)n, a, (double*)x,
+                       (double*)y, (double*)z, (double*)w, (double*)v, (const double*)d, fix, part, counter, red);
   LAUNCH_TRY();
   return TOPT_OK;
 }

@@ -593,4 +593,104 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_elemblock(ProbDev P, const T* __
   }
 }
 
+// ---------------------------------------------------------------- BLAS-1
+// Op-level vector kernels behind the drop-in API of fem/kernels.py:19-73,
+// 330-345 and fem/fused_kernels.py:110-187. One grid-stride kernel, op
+// selected by `op`; reductions are FP64 with the last-block pattern.
+enum Blas1Op : int {
+  B1_FILL = 0,      // x[i] = a                                 (kernels.py:19-23)
+  B1_COPY = 1,      // y[i] = x[i]                              (kernels.py:26-30)
+  B1_AXPY = 2,      // out[i] = a x[i] + y[i]                    (kernels.py:33-37)
+  B1_SCALE = 3,     // out[i] = a x[i]                           (kernels.py:40-44)
+  B1_DIV = 4,       // out[i] = x[i] / y[i]                      (kernels.py:65-73)
+  B1_DOT = 5,       // red[0] = sum x y                          (kernels.py:47-62)
+  B1_MAXDIFF = 6,   // red[0] = max |x - y|                      (kernels.py:339-345)
+  B1_SUM = 7,       // red[0] = sum x                            (kernels.py:330-336)
+  B1_UPD_JAC = 8,   // u += a p; r -= a Ap; z = r / d; red = (r.z, r.r)  (fused_kernels.py:110-139)
+  B1_UPD_NONE = 9,  // u += a p; r -= a Ap; z = r;     red = (r.z, r.r)  (fused_kernels.py:142-166)
+  B1_UPD_UR = 10,   // u += a p; r -= a Ap                         (fused_kernels.py:169-180)
+  B1_UPD_P = 11,    // p = z + a p                                 (fused_kernels.py:183-187)
+  B1_ZERO_FIXED = 12,  // x[i] = 0 where fixed bit i set           (solver.py:162-167)
+};
+
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_blas1(int op, long n, double a, T* x, T* y, T* z, T* w,
+                                                      T* v, const T* d, const uint32_t* fix, double* part,
+                                                      unsigned* counter, double* red) {
+  __shared__ double sh[16];
+  const long stride = (long)gridDim.x * TOPT_BLOCK;
+  const T at = (T)a;
+  double acc[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x; i < n; i += stride) {
+    switch (op) {
+      case B1_FILL: x[i] = at; break;
+      case B1_COPY: y[i] = x[i]; break;
+      case B1_AXPY: z[i] = at * x[i] + y[i]; break;
+      case B1_SCALE: z[i] = at * x[i]; break;
+      case B1_DIV: z[i] = x[i] / y[i]; break;
+      case B1_DOT: acc[0] += (double)x[i] * (double)y[i]; break;
+      case B1_MAXDIFF: {
+        const double d = fabs((double)x[i] - (double)y[i]);
+        acc[0] = d > acc[0] ? d : acc[0];
+        break;
+      }
+      case B1_SUM: acc[0] += (double)x[i]; break;
+      case B1_UPD_JAC:
+      case B1_UPD_NONE:
+      case B1_UPD_UR: {   // x = u, y = r, z = z, w = p, v = Ap, d = diag
+        x[i] += at * w[i];
+        const T rv = y[i] - at * v[i];
+        y[i] = rv;
+        if (op == B1_UPD_UR) break;
+        const T zv = op == B1_UPD_JAC ? rv / d[i] : rv;
+        z[i] = zv;
+        acc[0] += (double)rv * (double)zv;
+        acc[1] += (double)rv * (double)rv;
+        break;
+      }
+      case B1_UPD_P: w[i] = z[i] + at * w[i]; break;
+      case B1_ZERO_FIXED:
+        if ((fix[i >> 5] >> (i & 31)) & 1u) x[i] = T(0);
+        break;
+    }
+  }
+  const bool reduce = op == B1_DOT || op == B1_MAXDIFF || op == B1_SUM || op == B1_UPD_JAC || op == B1_UPD_NONE;
+  if (!reduce || red == nullptr) return;
+  if (op == B1_MAXDIFF) {
+    acc[0] = wave_max(acc[0]);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sh[wv] = acc[0];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = sh[0];
+      for (int k = 1; k < TOPT_BLOCK / 64; ++k) m = sh[k] > m ? sh[k] : m;
+      acc[0] = m;
+    }
+    __syncthreads();
+  } else {
+    block_sum<2>(acc, sh);
+  }
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&part[2L * blockIdx.x], acc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&part[2L * blockIdx.x + 1], acc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last || threadIdx.x != 0) return;
+  __threadfence();
+  double t0 = 0.0, t1 = 0.0;
+  for (unsigned b = 0; b < gridDim.x; ++b) {
+    const double p0 = __hip_atomic_load(&part[2L * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double p1 = __hip_atomic_load(&part[2L * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (op == B1_MAXDIFF) t0 = p0 > t0 ? p0 : t0;
+    else t0 += p0;
+    t1 += p1;
+  }
+  red[0] = t0;
+  red[1] = t1;
+  *counter = 0u;
+}
+
 }  // namespace topt

@@ -200,6 +200,14 @@ class Plan:
                                               self._stream()), "topt_plan_iteration")
         return [_stats(o) for o in out]
 
+    def time_slot(self, which: int, reps: int) -> float:
+        """Average GPU ms of PCG slot kernel A (0) or B (1) over `reps`
+        back-to-back launches (benchmark only; leaves the solve finished)."""
+        ms = C.c_float()
+        check(_lib.load().topt_plan_time_slot(self.handle, int(which), int(reps), C.byref(ms),
+                                              self._stream()), "topt_plan_time_slot")
+        return ms.value / reps
+
     def stats(self) -> list[IterStats]:
         out = (TIterStats * len(self))()
         check(_lib.load().topt_plan_stats(self.handle, out, self._stream()), "topt_plan_stats")
