@@ -61,7 +61,7 @@ enum {
   TOPT_BUF_HS,      /* double[elems]     filter row sums                          */
   TOPT_BUF_X, TOPT_BUF_R, TOPT_BUF_Z, TOPT_BUF_P0, TOPT_BUF_P1, TOPT_BUF_AP,
   TOPT_BUF_PRE, TOPT_BUF_PRE2, TOPT_BUF_S, TOPT_BUF_S64,
-  TOPT_BUF_TAPS, TOPT_BUF_CG, TOPT_BUF_OC, TOPT_BUF_PART, TOPT_BUF_OPART,
+  TOPT_BUF_TAPS, TOPT_BUF_CG, TOPT_BUF_OC, TOPT_BUF_PART, TOPT_BUF_OPART, TOPT_BUF_CTR,
   TOPT_BUF_COUNT
 };
 

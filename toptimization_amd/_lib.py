@@ -19,7 +19,7 @@ FILTER_DENSITY, FILTER_DCAVG, FILTER_SENS, FILTER_TRANSPOSE = 0, 1, 2, 3
 ST_RUN, ST_CONV, ST_BREAK, ST_MAXIT, ST_ZERO = 0, 1, 2, 3, 4
 
 BUF_NAMES = ["u", "f", "fix", "rho", "rhof", "dc", "dcf", "Hs", "x", "r", "z", "p0", "p1", "Ap",
-             "pre", "pre2", "s", "s64", "taps", "cg", "oc", "part", "opart"]
+             "pre", "pre2", "s", "s64", "taps", "cg", "oc", "part", "opart", "ctr"]
 BUF = {n: i for i, n in enumerate(BUF_NAMES)}
 
 

@@ -82,6 +82,18 @@ int tap_radius(const std::vector<Tap>& taps) {
   return R;
 }
 
+// Counter words of the three reduction trees (node, element, OC grid).
+int64_t ctr_words(const Geom& g) {
+  return (tree_ctrs(g.nb[G_NODE]) + tree_ctrs(g.nb[G_ELEM]) + tree_ctrs(g.nb[G_OC]) + 1) * TOPT_CTR_STRIDE;
+}
+
+void set_counters(ProbDev& P, unsigned* base, const Geom& g) {
+  P.ctr[G_NODE] = base;
+  P.ctr[G_ELEM] = base + tree_ctrs(g.nb[G_NODE]) * TOPT_CTR_STRIDE;
+  P.ctr[G_OC] = P.ctr[G_ELEM] + tree_ctrs(g.nb[G_ELEM]) * TOPT_CTR_STRIDE;
+  P.ctr[G_FILT] = nullptr;
+}
+
 struct Layout {
   int64_t off[TOPT_BUF_COUNT];
   int64_t total;
@@ -109,8 +121,9 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   sz[TOPT_BUF_TAPS] = std::max<int64_t>(ntaps, 1) * (int64_t)sizeof(Tap);
   sz[TOPT_BUF_CG] = sizeof(CgState);
   sz[TOPT_BUF_OC] = sizeof(OcState);
-  sz[TOPT_BUF_PART] = int64_t(g.nb[G_NODE]) * 2 * 8;
-  sz[TOPT_BUF_OPART] = std::max<int64_t>(g.nb[G_ELEM], int64_t(g.nb[G_OC]) * 64) * 8;
+  sz[TOPT_BUF_PART] = tree_parts(g.nb[G_NODE]) * 2 * 8;
+  sz[TOPT_BUF_OPART] = std::max<int64_t>(tree_parts(g.nb[G_ELEM]), tree_parts(g.nb[G_OC]) * 32) * 8;
+  sz[TOPT_BUF_CTR] = ctr_words(g) * 4;
   int64_t o = 0;
   for (int i = 0; i < TOPT_BUF_COUNT; ++i) {
     L.off[i] = o;
@@ -220,6 +233,7 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
   P.part = (double*)at(TOPT_BUF_PART);
   P.oc = (OcState*)at(TOPT_BUF_OC);
   P.opart = (double*)at(TOPT_BUF_OPART);
+  set_counters(P, (unsigned*)at(TOPT_BUF_CTR), g);
   return TOPT_OK;
 }
 
@@ -305,11 +319,16 @@ template <typename T>
 int launch_slots(topt_plan* pl, int count, hipStream_t s) {
   const ProbDev dummy{};
   const int nb = pl->nblk[G_NODE];
+  const int np = pl->n;
   for (int k = 0; k < count; ++k) {
     hipLaunchKernelGGL((k_cg_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+    hipLaunchKernelGGL((k_cg_fin<T, true>), dim3(np), dim3(TOPT_BLOCK), 0, s, pl->dtab, np, dummy, 0);
     hipLaunchKernelGGL((k_cg_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-    if (pl->any_elemblock)
+    hipLaunchKernelGGL((k_cg_fin<T, true>), dim3(np), dim3(TOPT_BLOCK), 0, s, pl->dtab, np, dummy, 1);
+    if (pl->any_elemblock) {
       hipLaunchKernelGGL((k_cg_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
+      hipLaunchKernelGGL((k_cg_fin<T, true>), dim3(np), dim3(TOPT_BLOCK), 0, s, pl->dtab, np, dummy, 2);
+    }
   }
   LAUNCH_TRY();
   return TOPT_OK;
@@ -422,7 +441,7 @@ int gather_stats(topt_plan* pl, topt_iter_stats* out, hipStream_t s) {
 // Single-problem descriptor for op-level calls: geometry, material and the
 // reduction scratch carved from ws (partials + a CgState/OcState for counters).
 struct OpScratch {
-  int64_t part_off, opart_off, cg_off, oc_off, hs_off, taps_off, total;
+  int64_t part_off, opart_off, cg_off, oc_off, ctr_off, taps_off, total;
 };
 
 OpScratch op_scratch(int nelx, int nely) {
@@ -430,13 +449,15 @@ OpScratch op_scratch(int nelx, int nely) {
   OpScratch o;
   int64_t off = 0;
   o.part_off = off;
-  off += align256(int64_t(g.nb[G_NODE]) * 2 * 8);
+  off += align256(tree_parts(g.nb[G_NODE]) * 2 * 8);
   o.opart_off = off;
-  off += align256(std::max<int64_t>(g.nb[G_ELEM], int64_t(g.nb[G_OC]) * 64) * 8);
+  off += align256(std::max<int64_t>(tree_parts(g.nb[G_ELEM]), tree_parts(g.nb[G_OC]) * 32) * 8);
   o.cg_off = off;
   off += align256(sizeof(CgState));
   o.oc_off = off;
   off += align256(sizeof(OcState));
+  o.ctr_off = off;
+  off += align256(ctr_words(g) * 4);
   o.taps_off = off;
   off += align256(int64_t(2 * 34 + 1) * (2 * 34 + 1) * (int64_t)sizeof(Tap));
   o.total = off;
@@ -457,9 +478,11 @@ int op_prob(ProbDev& P, int nelx, int nely, void* ws, hipStream_t s) {
   P.opart = (double*)(w + o.opart_off);
   P.cg = (CgState*)(w + o.cg_off);
   P.oc = (OcState*)(w + o.oc_off);
-  // counters must start at zero
+  set_counters(P, (unsigned*)(w + o.ctr_off), g);
+  // reduction counters must start at zero
   HIP_TRY(hipMemsetAsync(P.cg, 0, sizeof(CgState), s));
   HIP_TRY(hipMemsetAsync(P.oc, 0, sizeof(OcState), s));
+  HIP_TRY(hipMemsetAsync(P.ctr[G_NODE], 0, ctr_words(g) * 4, s));
   return TOPT_OK;
 }
 
@@ -505,6 +528,7 @@ int topt_problem_init(const topt_problem* prob, int32_t dtype, void* stream) {
   HIP_TRY(hipMemsetAsync(P.u64, 0, g.nn * 16, s));
   HIP_TRY(hipMemsetAsync(P.cg, 0, sizeof(CgState), s));
   HIP_TRY(hipMemsetAsync(P.oc, 0, sizeof(OcState), s));
+  HIP_TRY(hipMemsetAsync(P.ctr[G_NODE], 0, ctr_words(g) * 4, s));
   P.b0[G_FILT] = 0;
   hipLaunchKernelGGL((k_filter<false>), dim3(g.nb[G_FILT]), dim3(TOPT_BLOCK), filter_shmem(P.R), s, nullptr,
                      nullptr, P, (int)F_HS);
@@ -873,7 +897,9 @@ int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* 
 // fem/fused_kernels.py:110-187, fem/solver.py:162-167). Buffers by role:
 // x, y, z, w, v as documented per op in include/topt.h; red[2] (device)
 // receives reductions. ws: >= topt_blas1_workspace_bytes().
-int64_t topt_blas1_workspace_bytes(void) { return 1024 * 16 + 256; }
+int64_t topt_blas1_workspace_bytes(void) {
+  return align256(tree_parts(1024) * 2 * 8) + tree_ctrs(1024) * TOPT_CTR_STRIDE * 4;
+}
 
 int topt_blas1(int32_t op, int32_t dtype, int64_t n, double a, void* x, void* y, void* z, void* w, void* v,
                const void* d, const uint32_t* fix, double* red, void* ws, void* stream) {
@@ -886,8 +912,8 @@ int topt_blas1(int32_t op, int32_t dtype, int64_t n, double a, void* x, void* y,
   hipStream_t s = (hipStream_t)stream;
   const int grid = (int)std::min<int64_t>(1024, cdiv(n, TOPT_BLOCK));
   double* part = (double*)ws;
-  unsigned* counter = (unsigned*)((char*)ws + 1024 * 16);
-  if (reduce) HIP_TRY(hipMemsetAsync(counter, 0, sizeof(unsigned), s));
+  unsigned* counter = (unsigned*)((char*)ws + align256(tree_parts(1024) * 2 * 8));
+  if (reduce) HIP_TRY(hipMemsetAsync(counter, 0, tree_ctrs(1024) * TOPT_CTR_STRIDE * 4, s));
   if (dtype == TOPT_F32)
     hipLaunchKernelGGL((k_blas1<float>), dim3(grid), dim3(TOPT_BLOCK), 0, s, (int)op, (long)n, a, (float*)x,
                        (float*)y, (float*)z, (float*)w, (float*)v, (const float*)d, fix, part, counter, red);

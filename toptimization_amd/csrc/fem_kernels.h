@@ -22,7 +22,7 @@ struct NodeIdx {
   bool act;
 };
 
-__device__ __forceinline__ NodeIdx node_of(const ProbDev& P, int lb) {
+__device__ __forceinline__ NodeIdx node_of(PD& P, int lb) {
   NodeIdx id;
   const int nny = P.nely + 1;
   const long nn = (long)(P.nelx + 1) * nny;
@@ -36,8 +36,8 @@ __device__ __forceinline__ NodeIdx node_of(const ProbDev& P, int lb) {
 
 // Gather a T2 vector over the 3x3 node neighbourhood.
 template <typename T>
-__device__ __forceinline__ void gather9(const vec2<T>* __restrict__ v, const NodeIdx& id,
-                                        const ProbDev& P, vec2<T> (&q)[3][3]) {
+__device__ __forceinline__ void gather9(const GLB vec2<T>* v, const NodeIdx& id,
+                                        PD& P, vec2<T> (&q)[3][3]) {
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
@@ -48,8 +48,8 @@ __device__ __forceinline__ void gather9(const vec2<T>* __restrict__ v, const Nod
 // form: z_n = c_n^-1/2 sum_{e ni n} w_e (Ki (r/sqrt(c))_e)[n], fixed -> 0.
 // c = number of elements sharing the node (4 interior, 2 edge, 1 corner).
 template <typename T>
-__device__ __forceinline__ vec2<T> elemblock_node(const ProbDev& P, const NodeIdx& id,
-                                                  const vec2<T>* __restrict__ r) {
+__device__ __forceinline__ vec2<T> elemblock_node(PD& P, const NodeIdx& id,
+                                                  const GLB vec2<T>* r) {
   const T* s = (const T*)P.s;
   T sc[2][2];
   load_scales<T>(s, id.ix, id.iy, P.nelx, P.nely, sc);
@@ -77,7 +77,7 @@ __device__ __forceinline__ vec2<T> elemblock_node(const ProbDev& P, const NodeId
       q[dx][dy] = v;
     }
   T zx, zy;
-  q4_gather<T>(kiptr<T>(P), sc, q, zx, zy);
+  q4_gather<T>(KeSel<T>::ki(P), sc, q, zx, zy);
   const int cx = (id.ix == 0 || id.ix == P.nelx) ? 1 : 2;
   const int cy = (id.iy == 0 || id.iy == P.nely) ? 1 : 2;
   const T inv = T(1) / sqrt((T)(cx * cy));
@@ -92,21 +92,40 @@ __device__ __forceinline__ vec2<T> elemblock_node(const ProbDev& P, const NodeId
 
 // z = M^-1 r at one node for the elementwise preconditioners.
 template <typename T>
-__device__ __forceinline__ vec2<T> precond_local(const ProbDev& P, long n, vec2<T> r) {
+__device__ __forceinline__ vec2<T> precond_local(PD& P, long n, vec2<T> r) {
   vec2<T> z;
   if (P.precond == PC_JACOBI) {
-    const vec2<T> d = ((const vec2<T>*)P.pre)[n];
+    const vec2<T> d = (gp((const vec2<T>*)P.pre))[n];
     z.x = r.x * d.x;
     z.y = r.y * d.y;
   } else if (P.precond == PC_BLOCK2) {
-    const vec2<T> d = ((const vec2<T>*)P.pre)[n];
-    const T o = ((const T*)P.pre2)[n];
+    const vec2<T> d = (gp((const vec2<T>*)P.pre))[n];
+    const T o = (gp((const T*)P.pre2))[n];
     z.x = d.x * r.x + o * r.y;
     z.y = o * r.x + d.y * r.y;
   } else {
     z = r;
   }
   return z;
+}
+
+// K(s) q at one node with FP64 Ke and FP64 accumulation; q and s in T.
+template <typename T>
+__device__ __forceinline__ void stencil_fp64(PD& P, const NodeIdx& id,
+                                             const vec2<T> (&q)[3][3], double& ax, double& ay) {
+  T sc[2][2];
+  load_scales<T>((const T*)P.s, id.ix, id.iy, P.nelx, P.nely, sc);
+  double scd[2][2];
+  d2v qd[3][3];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) scd[a][b] = (double)sc[a][b];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) qd[a][b] = d2v{(double)q[a][b].x, (double)q[a][b].y};
+  q4_gather<double>(P.Ke64, scd, qd, ax, ay);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -117,14 +136,14 @@ template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_scale(const ProbDev* tab, const int* bmap,
                                                       ProbDev single, int use_filtered) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   const long ne = (long)P.nelx * P.nely;
   const long e = (long)(b - P.b0[G_ELEM]) * TOPT_BLOCK + threadIdx.x;
   if (e >= ne) return;
-  const double* rho = use_filtered ? P.rhof : P.rho;
+  const GLB double* rho = gp(use_filtered ? P.rhof : P.rho);
   const double sv = P.Emin + pow(rho[e], P.penal) * (1.0 - P.Emin);
-  ((T*)P.s)[e] = (T)sv;
-  if (P.s64 != nullptr && (void*)P.s64 != P.s) P.s64[e] = sv;
+  (gp((T*)P.s))[e] = (T)sv;
+  if (P.s64 != nullptr && (void*)P.s64 != P.s) gp(P.s64)[e] = sv;
 }
 
 // Preconditioner data from the element scales.
@@ -135,14 +154,14 @@ template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, const int* bmap,
                                                         ProbDev single, int raw) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   if (P.precond != PC_JACOBI && P.precond != PC_BLOCK2) return;
   const NodeIdx id = node_of(P, b - P.b0[G_NODE]);
   if (!id.act) return;
   const T* s = (const T*)P.s;
   T sc[2][2];
   load_scales<T>(s, id.ix, id.iy, P.nelx, P.nely, sc);
-  const T* K = keptr<T>(P);
+  const auto K = KeSel<T>::ke(P);
   T dxx = 0, dyy = 0, dxy = 0;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -160,7 +179,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, cons
     dyy = (fb & 2u) ? T(1) : (dyy < T(1e-20) ? T(1e-20) : dyy);
     out.x = raw ? dxx : T(1) / dxx;
     out.y = raw ? dyy : T(1) / dyy;
-    ((vec2<T>*)P.pre)[id.n] = out;
+    (gp((vec2<T>*)P.pre))[id.n] = out;
   } else {
     if (fb & 1u) { dxx = T(1); dxy = T(0); }
     if (fb & 2u) { dyy = T(1); dxy = T(0); }
@@ -171,8 +190,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, cons
     const T id_ = T(1) / det;
     out.x = dyy * id_;
     out.y = dxx * id_;
-    ((vec2<T>*)P.pre)[id.n] = out;
-    ((T*)P.pre2)[id.n] = -dxy * id_;
+    (gp((vec2<T>*)P.pre))[id.n] = out;
+    (gp((T*)P.pre2))[id.n] = -dxy * id_;
   }
 }
 
@@ -184,14 +203,14 @@ template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, const int* bmap,
                                                            ProbDev single, int warm) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_NODE];
   const bool zero_load = P.fnorm2 < 1e-60;
   const int w = (warm == 2) ? (P.oc != nullptr && P.oc->iter > 0) : warm;
   const NodeIdx id = node_of(P, lb);
   if (id.act) {
-    double2* u = (double2*)P.u64;
-    double2 v = u[id.n];
+    GLB d2v* u = gp((d2v*)P.u64);
+    d2v v = u[id.n];
     const unsigned fb = fixbits(P.fix, id.n);
     if (!w || zero_load) { v.x = 0.0; v.y = 0.0; }
     if (fb & 1u) v.x = 0.0;
@@ -200,9 +219,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
     vec2<T> zero;
     zero.x = T(0);
     zero.y = T(0);
-    if ((void*)P.x != (void*)P.u64) ((vec2<T>*)P.x)[id.n] = zero;
-    ((vec2<T>*)P.p0)[id.n] = zero;
-    ((vec2<T>*)P.p1)[id.n] = zero;
+    if ((void*)P.x != (void*)P.u64) (gp((vec2<T>*)P.x))[id.n] = zero;
+    (gp((vec2<T>*)P.p0))[id.n] = zero;
+    (gp((vec2<T>*)P.p1))[id.n] = zero;
   }
   if (lb == 0 && threadIdx.x == 0) {
     CgState* st = P.cg;
@@ -219,24 +238,37 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
   }
 }
 
-// Residual refresh r = f - A(u64 [+ x]) in FP64 (stored in T), ||r||^2 in
-// FP64. Used to start every solve and, on the mixed path, as the reliable
-// update that replaces the recursive residual.
+// PCG slot kernels write one set of FP64 partials per block (plain stores,
+// no atomics, no state changes); the finalize kernel launched right after
+// each slot (one block per problem) sums them in a fixed order and updates
+// alpha / beta / convergence. Kernel boundaries order the hand-off, so there
+// is no cross-block synchronisation inside a kernel at all.
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], PD& P, int lb, double* sh) {
+  block_sum<NV>(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) gp(P.part)[(long)lb * NV + k] = v[k];
+  }
+}
+
+// Residual refresh r = f - A(u64 [+ x]) in FP64 (stored in T), partial
+// ||r||^2 in FP64. Starts every solve and, on the mixed path, is the
+// reliable update that replaces the recursive residual.
 template <typename T>
-__device__ void refine_a(const ProbDev& P, int lb) {
+__device__ void refine_a(PD& P, int lb) {
   __shared__ double sh[16];
-  CgState* st = P.cg;
   const NodeIdx id = node_of(P, lb);
   constexpr bool MIXED = sizeof(T) == 4;
-  const double2* u = (const double2*)P.u64;
-  const vec2<T>* x = (const vec2<T>*)P.x;
-  double2 q[3][3];
+  const GLB d2v* u = gp((const d2v*)P.u64);
+  const GLB vec2<T>* x = gp((const vec2<T>*)P.x);
+  d2v q[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy) {
       const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
-      double2 v = u[m];
+      d2v v = u[m];
       if (MIXED) {
         const vec2<T> c = x[m];
         v.x += (double)c.x;
@@ -251,33 +283,25 @@ __device__ void refine_a(const ProbDev& P, int lb) {
   const unsigned fb = fixbits(P.fix, id.n);
   if (fb & 1u) ax = q[1][1].x;
   if (fb & 2u) ay = q[1][1].y;
-  const double2 f = ((const double2*)P.f)[id.n];
+  const d2v f = gp((const d2v*)P.f)[id.n];
   const double rx = f.x - ax, ry = f.y - ay;
   double v[1] = {id.act ? rx * rx + ry * ry : 0.0};
   if (id.act) {
     vec2<T> r;
     r.x = (T)rx;
     r.y = (T)ry;
-    ((vec2<T>*)P.r)[id.n] = r;
+    (gp((vec2<T>*)P.r))[id.n] = r;
   }
-  block_sum<1>(v, sh);
-  double tot[1];
-  if (last_block_sum<1>(v, P.part, &st->cnt[0], lb, P.nb[G_NODE], tot, sh)) {
-    if (threadIdx.x == 0) {
-      st->rr_true = tot[0];
-      if (!st->first && tot[0] < P.tol2) st->pending = ST_CONV;
-    }
-  }
+  store_partials<1>(v, P, lb, sh);
 }
 
-// Slot A: p = z + beta p (p buffers ping-pong), Ap = K(s) p, p.Ap (FP64),
-// alpha = rz / pAp by the last block; |pAp| < 1e-30 -> breakdown
-// (fem/fused_kernels.py:21-56 + fem/solver.py:110-117, 151-154).
+// Slot A: p = z + beta p (p buffers ping-pong), Ap = K(s) p, partial p.Ap
+// (fem/fused_kernels.py:21-56; p update of fem/solver.py:151-154 fused in).
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_a(const ProbDev* tab, const int* bmap,
                                                      ProbDev single) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   CgState* st = P.cg;
   if (st->status != ST_RUN) return;
   const int lb = b - P.b0[G_NODE];
@@ -289,9 +313,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_a(const ProbDev* tab, const i
   const NodeIdx id = node_of(P, lb);
   const int pc = st->pcur;
   const T beta = (T)st->beta;
-  const vec2<T>* pold = (const vec2<T>*)(pc ? P.p1 : P.p0);
-  vec2<T>* pnew = (vec2<T>*)(pc ? P.p0 : P.p1);
-  const vec2<T>* z = (const vec2<T>*)P.z;
+  const GLB vec2<T>* pold = gp((const vec2<T>*)(pc ? P.p1 : P.p0));
+  GLB vec2<T>* pnew = gp((vec2<T>*)(pc ? P.p0 : P.p1));
+  const GLB vec2<T>* z = gp((const vec2<T>*)P.z);
   vec2<T> q[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
@@ -304,47 +328,124 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_a(const ProbDev* tab, const i
       v.y = zv.y + beta * pv.y;
       q[dx][dy] = v;
     }
-  T sc[2][2];
-  load_scales<T>((const T*)P.s, id.ix, id.iy, P.nelx, P.nely, sc);
-  T ax, ay;
-  q4_gather<T>(keptr<T>(P), sc, q, ax, ay);
+  // FP32 storage, FP64 registers: the stencil of the stored p_new is summed
+  // with the FP64 Ke so the inner operator is K(s32) up to the final
+  // rounding of Ap (an FP32 Ke and FP32 sums break the rigid-body
+  // cancellation and stall the FP64 residual refresh).
+  double ax, ay;
+  stencil_fp64<T>(P, id, q, ax, ay);
   const unsigned fb = fixbits(P.fix, id.n);
-  if (fb & 1u) ax = q[1][1].x;
-  if (fb & 2u) ay = q[1][1].y;
+  if (fb & 1u) ax = (double)q[1][1].x;
+  if (fb & 2u) ay = (double)q[1][1].y;
   double v[1] = {0.0};
   if (id.act) {
     pnew[id.n] = q[1][1];
     vec2<T> o;
-    o.x = ax;
-    o.y = ay;
-    ((vec2<T>*)P.Ap)[id.n] = o;
-    v[0] = (double)q[1][1].x * (double)ax + (double)q[1][1].y * (double)ay;
+    o.x = (T)ax;
+    o.y = (T)ay;
+    (gp((vec2<T>*)P.Ap))[id.n] = o;
+    v[0] = (double)q[1][1].x * (double)o.x + (double)q[1][1].y * (double)o.y;
   }
-  block_sum<1>(v, sh);
-  double tot[1];
-  if (last_block_sum<1>(v, P.part, &st->cnt[1], lb, P.nb[G_NODE], tot, sh)) {
-    if (threadIdx.x == 0) {
-      st->pcur = pc ^ 1;
-      st->pAp = tot[0];
-      if (fabs(tot[0]) < 1e-30) {
-        if (sizeof(T) == 4) {   // mixed: commit the correction before stopping
-          st->pending = ST_BREAK;
-          st->mode = MODE_REFINE;
-        } else {
-          st->status = ST_BREAK;
-        }
-      } else {
-        st->alpha = st->rz / tot[0];
+  store_partials<1>(v, P, lb, sh);
+}
+
+// Slot B. CG mode: x += alpha p, r -= alpha Ap, z = M^-1 r, partial r.z and
+// r.r (fem/fused_kernels.py:110-166 update_and_precond_*; element-block
+// Jacobi defers z to slot C, as fem/fused_kernels.py:169-180 with
+// preconditioner.py do). Refine mode: u64 += x, x = 0 (mixed),
+// z = M^-1 r, partial r.z.
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_b(const ProbDev* tab, const int* bmap,
+                                                     ProbDev single) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  PD& P = get_prob<B>(tab, bmap, single, b);
+  CgState* st = P.cg;
+  if (st->status != ST_RUN) return;
+  constexpr bool MIXED = sizeof(T) == 4;
+  __shared__ double sh[16];
+  const int lb = b - P.b0[G_NODE];
+  const NodeIdx id = node_of(P, lb);
+  GLB vec2<T>* r = gp((vec2<T>*)P.r);
+  GLB vec2<T>* z = gp((vec2<T>*)P.z);
+  GLB vec2<T>* x = gp((vec2<T>*)P.x);
+
+  if (st->mode == MODE_REFINE) {
+    double v[1] = {0.0};
+    vec2<T> zz;
+    if (P.precond == PC_ELEMBLOCK) zz = elemblock_node<T>(P, id, r);
+    if (id.act) {
+      if (MIXED) {
+        GLB d2v* u = gp((d2v*)P.u64);
+        d2v uv = u[id.n];
+        const vec2<T> c = x[id.n];
+        uv.x += (double)c.x;
+        uv.y += (double)c.y;
+        u[id.n] = uv;
+        vec2<T> zero;
+        zero.x = T(0);
+        zero.y = T(0);
+        x[id.n] = zero;
       }
+      const vec2<T> rv = r[id.n];
+      if (P.precond != PC_ELEMBLOCK) zz = precond_local<T>(P, id.n, rv);
+      z[id.n] = zz;
+      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+    }
+    store_partials<1>(v, P, lb, sh);
+    return;
+  }
+
+  const T alpha = (T)st->alpha;
+  const int pc = st->pcur;
+  const GLB vec2<T>* p = gp((const vec2<T>*)(pc ? P.p1 : P.p0));
+  double v[2] = {0.0, 0.0};
+  if (id.act) {
+    const vec2<T> pv = p[id.n], ap = (gp((const vec2<T>*)P.Ap))[id.n];
+    vec2<T> xv = x[id.n], rv = r[id.n];
+    xv.x += alpha * pv.x;
+    xv.y += alpha * pv.y;
+    rv.x -= alpha * ap.x;
+    rv.y -= alpha * ap.y;
+    x[id.n] = xv;
+    r[id.n] = rv;
+    v[1] = (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+    if (P.precond != PC_ELEMBLOCK) {
+      const vec2<T> zz = precond_local<T>(P, id.n, rv);
+      z[id.n] = zz;
+      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
     }
   }
+  store_partials<2>(v, P, lb, sh);
+}
+
+// Slot C (element-block Jacobi only): z = M^-1 r by gather, partial r.z
+// (fem/preconditioner.py:47-108 apply_and_rz_2d).
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_c(const ProbDev* tab, const int* bmap,
+                                                     ProbDev single) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  PD& P = get_prob<B>(tab, bmap, single, b);
+  CgState* st = P.cg;
+  if (st->status != ST_RUN || st->pad == 0) return;
+  __shared__ double sh[16];
+  const int lb = b - P.b0[G_NODE];
+  const NodeIdx id = node_of(P, lb);
+  const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
+  const vec2<T> zz = elemblock_node<T>(P, id, r);
+  double v[1] = {0.0};
+  if (id.act) {
+    (gp((vec2<T>*)P.z))[id.n] = zz;
+    const vec2<T> rv = r[id.n];
+    v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+  }
+  store_partials<1>(v, P, lb, sh);
 }
 
 // Convergence bookkeeping after a CG update (fem/solver.py:146-156).
-// rzn < 0 means "rz computed later by slot C" (element-block Jacobi).
+// have_rz = false: r.z comes later from slot C (element-block Jacobi).
 template <typename T>
-__device__ __forceinline__ void cg_after_update(const ProbDev& P, CgState* st, double rr,
-                                                double rzn, bool have_rz) {
+__device__ __forceinline__ void cg_after_update(PD& P, CgState* st, double rr, double rzn,
+                                                bool have_rz) {
   constexpr bool MIXED = sizeof(T) == 4;
   st->rr = rr;
   st->it += 1;
@@ -367,124 +468,76 @@ __device__ __forceinline__ void cg_after_update(const ProbDev& P, CgState* st, d
   }
 }
 
-// Slot B. CG mode: x += alpha p, r -= alpha Ap, z = M^-1 r, r.z and r.r in
-// FP64, beta and convergence by the last block
-// (fem/fused_kernels.py:110-166 update_and_precond_*; element-block Jacobi
-// defers z to slot C as fem/fused_kernels.py:169-180 + preconditioner.py do).
-// Refine mode: u64 += x, x = 0 (mixed), z = M^-1 r, r.z; beta from the
-// previous r.z; after the very first refresh p = z (beta = 0).
+// Finalize after slot `slot` (0 = A, 1 = B, 2 = C): one block per problem
+// sums that slot's block partials in a fixed order and applies the PCG
+// scalar logic (fem/solver.py:107-156):
+//   A/CG     alpha = rz / pAp, |pAp| < 1e-30 -> breakdown (solver.py:114-117)
+//   A/refine ||f - A u||^2 of the FP64 residual refresh
+//   B/CG     beta, convergence on the recursive residual (solver.py:146-151)
+//   B/refine beta after the refresh (first refresh: p = z)
+//   C        beta from the element-block r.z
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_b(const ProbDev* tab, const int* bmap,
-                                                     ProbDev single) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_fin(const ProbDev* tab, int nprob, ProbDev single,
+                                                       int slot) {
+  __shared__ double sh[16];
+  PD& P = B ? *(PD*)(tab + blockIdx.x) : *(PD*)&single;
   CgState* st = P.cg;
   if (st->status != ST_RUN) return;
+  if (slot == 2 && st->pad == 0) return;
+  const int mode = st->mode;
+  const int nv = (slot == 1 && mode == MODE_CG) ? 2 : 1;
+  const GLB double* part = gp((const double*)P.part);
+  const int nb = P.nb[G_NODE];
+  double acc[2] = {0.0, 0.0};
+  if (nv == 2) {
+    for (int j = threadIdx.x; j < nb; j += TOPT_BLOCK) {
+      acc[0] += part[2L * j];
+      acc[1] += part[2L * j + 1];
+    }
+  } else {
+    for (int j = threadIdx.x; j < nb; j += TOPT_BLOCK) acc[0] += part[j];
+  }
+  block_sum<2>(acc, sh);
+  if (threadIdx.x != 0) return;
   constexpr bool MIXED = sizeof(T) == 4;
-  __shared__ double sh[16];
-  const int lb = b - P.b0[G_NODE];
-  const NodeIdx id = node_of(P, lb);
-  vec2<T>* r = (vec2<T>*)P.r;
-  vec2<T>* z = (vec2<T>*)P.z;
-  vec2<T>* x = (vec2<T>*)P.x;
-
-  if (st->mode == MODE_REFINE) {
-    double v[1] = {0.0};
-    vec2<T> zz;
-    if (P.precond == PC_ELEMBLOCK) zz = elemblock_node<T>(P, id, r);
-    if (id.act) {
-      if (MIXED) {
-        double2* u = (double2*)P.u64;
-        double2 uv = u[id.n];
-        const vec2<T> c = x[id.n];
-        uv.x += (double)c.x;
-        uv.y += (double)c.y;
-        u[id.n] = uv;
-        vec2<T> zero;
-        zero.x = T(0);
-        zero.y = T(0);
-        x[id.n] = zero;
-      }
-      const vec2<T> rv = r[id.n];
-      if (P.precond != PC_ELEMBLOCK) zz = precond_local<T>(P, id.n, rv);
-      z[id.n] = zz;
-      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
-    }
-    block_sum<1>(v, sh);
-    double tot[1];
-    if (last_block_sum<1>(v, P.part, &st->cnt[2], lb, P.nb[G_NODE], tot, sh)) {
-      if (threadIdx.x == 0) {
-        if (st->pending) {
-          st->status = st->pending;
+  if (slot == 0) {
+    if (mode == MODE_REFINE) {
+      st->rr_true = acc[0];
+      if (!st->first && acc[0] < P.tol2) st->pending = ST_CONV;
+    } else {
+      st->pcur ^= 1;
+      st->pAp = acc[0];
+      if (fabs(acc[0]) < 1e-30) {
+        if (MIXED) {            // commit the correction before stopping
+          st->pending = ST_BREAK;
+          st->mode = MODE_REFINE;
         } else {
-          st->beta = st->first ? 0.0 : tot[0] / (st->rz_prev + 1e-60);
-          st->rz = tot[0];
-          st->first = 0;
-          st->mode = MODE_CG;
-          st->rmax2 = st->rr_true;
-          st->refines += 1;
+          st->status = ST_BREAK;
         }
+      } else {
+        st->alpha = st->rz / acc[0];
       }
     }
-    return;
-  }
-
-  const T alpha = (T)st->alpha;
-  const int pc = st->pcur;
-  const vec2<T>* p = (const vec2<T>*)(pc ? P.p1 : P.p0);
-  double v[2] = {0.0, 0.0};
-  if (id.act) {
-    const vec2<T> pv = p[id.n], ap = ((const vec2<T>*)P.Ap)[id.n];
-    vec2<T> xv = x[id.n], rv = r[id.n];
-    xv.x += alpha * pv.x;
-    xv.y += alpha * pv.y;
-    rv.x -= alpha * ap.x;
-    rv.y -= alpha * ap.y;
-    x[id.n] = xv;
-    r[id.n] = rv;
-    v[1] = (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
-    if (P.precond != PC_ELEMBLOCK) {
-      const vec2<T> zz = precond_local<T>(P, id.n, rv);
-      z[id.n] = zz;
-      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+  } else if (slot == 1) {
+    if (mode == MODE_REFINE) {
+      if (st->pending) {
+        st->status = st->pending;
+      } else {
+        st->beta = st->first ? 0.0 : acc[0] / (st->rz_prev + 1e-60);
+        st->rz = acc[0];
+        st->first = 0;
+        st->mode = MODE_CG;
+        st->rmax2 = st->rr_true;
+        st->refines += 1;
+      }
+    } else {
+      cg_after_update<T>(P, st, acc[1], acc[0], P.precond != PC_ELEMBLOCK);
     }
-  }
-  block_sum<2>(v, sh);
-  double tot[2];
-  if (last_block_sum<2>(v, P.part, &st->cnt[2], lb, P.nb[G_NODE], tot, sh)) {
-    if (threadIdx.x == 0) cg_after_update<T>(P, st, tot[1], tot[0], P.precond != PC_ELEMBLOCK);
-  }
-}
-
-// Slot C (element-block Jacobi only): z = M^-1 r by gather, r.z, beta
-// (fem/preconditioner.py:47-108 apply_and_rz_2d).
-template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_c(const ProbDev* tab, const int* bmap,
-                                                     ProbDev single) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
-  CgState* st = P.cg;
-  if (st->status != ST_RUN || st->pad == 0) return;
-  __shared__ double sh[16];
-  const int lb = b - P.b0[G_NODE];
-  const NodeIdx id = node_of(P, lb);
-  const vec2<T>* r = (const vec2<T>*)P.r;
-  const vec2<T> zz = elemblock_node<T>(P, id, r);
-  double v[1] = {0.0};
-  if (id.act) {
-    ((vec2<T>*)P.z)[id.n] = zz;
-    const vec2<T> rv = r[id.n];
-    v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
-  }
-  block_sum<1>(v, sh);
-  double tot[1];
-  if (last_block_sum<1>(v, P.part, &st->cnt[3], lb, P.nb[G_NODE], tot, sh)) {
-    if (threadIdx.x == 0) {
-      st->beta = tot[0] / (st->rz + 1e-60);
-      st->rz_prev = st->rz;
-      st->rz = tot[0];
-      st->pad = 0;
-    }
+  } else {
+    st->beta = acc[0] / (st->rz + 1e-60);
+    st->rz_prev = st->rz;
+    st->rz = acc[0];
+    st->pad = 0;
   }
 }
 
@@ -496,7 +549,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
                                                      ProbDev single, int use_filtered) {
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_ELEM];
   const long ne = (long)P.nelx * P.nely;
   const long e0 = (long)lb * TOPT_BLOCK + threadIdx.x;
@@ -504,10 +557,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
   const long e = act ? e0 : ne - 1;
   const int ex = (int)(e / P.nely), ey = (int)(e - (long)ex * P.nely);
   const int nny = P.nely + 1;
-  const double2* u = (const double2*)P.u64;
+  const GLB d2v* u = gp((const d2v*)P.u64);
   const long n1 = (long)ex * nny + ey;
   double ue[8];
-  const double2 a0 = u[n1], a1 = u[n1 + nny], a2 = u[n1 + nny + 1], a3 = u[n1 + 1];
+  const d2v a0 = u[n1], a1 = u[n1 + nny], a2 = u[n1 + nny + 1], a3 = u[n1 + 1];
   ue[0] = a0.x; ue[1] = a0.y; ue[2] = a1.x; ue[3] = a1.y;
   ue[4] = a2.x; ue[5] = a2.y; ue[6] = a3.x; ue[7] = a3.y;
   double uKu = 0.0;
@@ -518,17 +571,17 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
     for (int j = 0; j < 8; ++j) t += P.Ke64[i * 8 + j] * ue[j];
     uKu += ue[i] * t;
   }
-  const double* rho = use_filtered ? P.rhof : P.rho;
+  const GLB double* rho = gp(use_filtered ? P.rhof : P.rho);
   const double re = rho[e];
   const double sv = P.Emin + pow(re, P.penal) * (1.0 - P.Emin);
   double v[1] = {0.0};
   if (act) {
-    P.dc[e] = -P.penal * pow(re, P.penal - 1.0) * (1.0 - P.Emin) * uKu;
+    gp(P.dc)[e] = -P.penal * pow(re, P.penal - 1.0) * (1.0 - P.Emin) * uKu;
     v[0] = sv * uKu;
   }
   block_sum<1>(v, sh);
   double tot[1];
-  if (last_block_sum<1>(v, P.opart, &P.oc->cnt[0], lb, P.nb[G_ELEM], tot, sh)) {
+  if (tree_reduce<1>(v, P.opart, P.ctr[G_ELEM], lb, P.nb[G_ELEM], tot)) {
     if (threadIdx.x == 0) P.oc->c = tot[0];
   }
 }
@@ -536,48 +589,48 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
 // Op-level K(s) p with the reference BC rule (fixed rows -> identity), and
 // optionally p.Ap into out[0] (fem/kernels.py:143-176, fused_kernels.py:21-56).
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev P, const T* __restrict__ pin,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* __restrict__ pin,
                                                        T* __restrict__ Ap, double* out) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   __shared__ double sh[16];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const NodeIdx id = node_of(P, lb);
   vec2<T> q[3][3];
-  gather9<T>((const vec2<T>*)pin, id, P, q);
-  T sc[2][2];
-  load_scales<T>((const T*)P.s, id.ix, id.iy, P.nelx, P.nely, sc);
-  T ax, ay;
-  q4_gather<T>(keptr<T>(P), sc, q, ax, ay);
+  gather9<T>(gp((const vec2<T>*)pin), id, P, q);
+  double ax, ay;
+  stencil_fp64<T>(P, id, q, ax, ay);
   const unsigned fb = fixbits(P.fix, id.n);
-  if (fb & 1u) ax = q[1][1].x;
-  if (fb & 2u) ay = q[1][1].y;
+  if (fb & 1u) ax = (double)q[1][1].x;
+  if (fb & 2u) ay = (double)q[1][1].y;
   double v[1] = {0.0};
   if (id.act) {
     vec2<T> o;
-    o.x = ax;
-    o.y = ay;
-    ((vec2<T>*)Ap)[id.n] = o;
-    v[0] = (double)q[1][1].x * (double)ax + (double)q[1][1].y * (double)ay;
+    o.x = (T)ax;
+    o.y = (T)ay;
+    gp((vec2<T>*)Ap)[id.n] = o;
+    v[0] = (double)q[1][1].x * (double)o.x + (double)q[1][1].y * (double)o.y;
   }
   if (out == nullptr) return;
   block_sum<1>(v, sh);
   double tot[1];
-  if (last_block_sum<1>(v, P.part, &P.cg->cnt[0], lb, P.nb[G_NODE], tot, sh)) {
+  if (tree_reduce<1>(v, P.part, P.ctr[G_NODE], lb, P.nb[G_NODE], tot)) {
     if (threadIdx.x == 0) out[0] = tot[0];
   }
 }
 
 // Op-level element-block Jacobi apply (+ r.z, r.r) for drop-in callers.
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_elemblock(ProbDev P, const T* __restrict__ rin,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_elemblock(ProbDev single, const T* __restrict__ rin,
                                                           T* __restrict__ zout, double* out) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   __shared__ double sh[16];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const NodeIdx id = node_of(P, lb);
-  const vec2<T>* r = (const vec2<T>*)rin;
+  const GLB vec2<T>* r = gp((const vec2<T>*)rin);
   const vec2<T> zz = elemblock_node<T>(P, id, r);
   double v[2] = {0.0, 0.0};
   if (id.act) {
-    ((vec2<T>*)zout)[id.n] = zz;
+    gp((vec2<T>*)zout)[id.n] = zz;
     const vec2<T> rv = r[id.n];
     v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
     v[1] = (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
@@ -585,7 +638,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_elemblock(ProbDev P, const T* __
   if (out == nullptr) return;
   block_sum<2>(v, sh);
   double tot[2];
-  if (last_block_sum<2>(v, P.part, &P.cg->cnt[0], lb, P.nb[G_NODE], tot, sh)) {
+  if (tree_reduce<2>(v, P.part, P.ctr[G_NODE], lb, P.nb[G_NODE], tot)) {
     if (threadIdx.x == 0) {
       out[0] = tot[0];
       out[1] = tot[1];
@@ -670,27 +723,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_blas1(int op, long n, double a, 
   } else {
     block_sum<2>(acc, sh);
   }
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&part[2L * blockIdx.x], acc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&part[2L * blockIdx.x + 1], acc[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  double tot[2];
+  if (tree_reduce<2>(acc, part, counter, blockIdx.x, gridDim.x, tot, op == B1_MAXDIFF ? 1u : 0u)) {
+    if (threadIdx.x == 0) {
+      red[0] = tot[0];
+      red[1] = tot[1];
+    }
   }
-  __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
-  __threadfence();
-  double t0 = 0.0, t1 = 0.0;
-  for (unsigned b = 0; b < gridDim.x; ++b) {
-    const double p0 = __hip_atomic_load(&part[2L * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double p1 = __hip_atomic_load(&part[2L * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (op == B1_MAXDIFF) t0 = p0 > t0 ? p0 : t0;
-    else t0 += p0;
-    t1 += p1;
-  }
-  red[0] = t0;
-  red[1] = t1;
-  *counter = 0u;
 }
 
 }  // namespace topt

@@ -27,15 +27,15 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_filter(const ProbDev* tab, const
                                                        ProbDev single, int mode) {
   extern __shared__ double tile[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_FILT];
   const int nelx = P.nelx, nely = P.nely, R = P.R;
   const int nty = (nely + TOPT_FILT_TY - 1) / TOPT_FILT_TY;
   const int tx0 = (lb / nty) * TOPT_FILT_TX, ty0 = (lb % nty) * TOPT_FILT_TY;
   const int H = TOPT_FILT_TY + 2 * R, W = TOPT_FILT_TX + 2 * R;
-  const double* rho = P.rho;
-  const double* dc = P.dc;
-  const double* Hs = P.Hs;
+  const GLB double* rho = gp((const double*)P.rho);
+  const GLB double* dc = gp((const double*)P.dc);
+  const GLB double* Hs = gp((const double*)P.Hs);
   for (int c = threadIdx.x; c < H * W; c += TOPT_BLOCK) {
     const int lx = c / H, ly = c - lx * H;
     const int ex = tx0 - R + lx, ey = ty0 - R + ly;
@@ -57,14 +57,14 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_filter(const ProbDev* tab, const
   const int txa = threadIdx.x / TOPT_FILT_TY;     // 0..3, second element at txa + 4
   const int ey = ty0 + ty;
   double num0 = 0.0, num1 = 0.0;
-  const Tap* taps = P.taps;
+  const CST Tap* taps = cp(P.taps);
   const int base0 = (txa + R) * H + ty + R;
   const int base1 = (txa + 4 + R) * H + ty + R;
   for (int k = 0; k < P.ntaps; ++k) {
-    const Tap t = taps[k];
-    const int off = t.dx * H + t.dy;
-    num0 += t.w * tile[base0 + off];
-    num1 += t.w * tile[base1 + off];
+    const int off = taps[k].dx * H + taps[k].dy;
+    const double w = taps[k].w;
+    num0 += w * tile[base0 + off];
+    num1 += w * tile[base1 + off];
   }
   if (ey >= nely) return;
 #pragma unroll
@@ -74,11 +74,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_filter(const ProbDev* tab, const
     const long e = (long)ex * nely + ey;
     const double num = h ? num1 : num0;
     switch (mode) {
-      case F_HS: P.Hs[e] = num; break;
-      case F_DENSITY: P.rhof[e] = num / (Hs[e] + 1e-20); break;
-      case F_DCAVG: P.dcf[e] = num / (Hs[e] + 1e-20); break;
-      case F_SENS: P.dcf[e] = num / (rho[e] * Hs[e] + 1e-20); break;
-      default: P.dcf[e] = num; break;
+      case F_HS: gp(P.Hs)[e] = num; break;
+      case F_DENSITY: gp(P.rhof)[e] = num / (Hs[e] + 1e-20); break;
+      case F_DCAVG: gp(P.dcf)[e] = num / (Hs[e] + 1e-20); break;
+      case F_SENS: gp(P.dcf)[e] = num / (rho[e] * Hs[e] + 1e-20); break;
+      default: gp(P.dcf)[e] = num; break;
     }
   }
 }
@@ -120,7 +120,7 @@ template <bool B>
 __global__ void k_oc_reset(const ProbDev* tab, int nprob, ProbDev single) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (B ? nprob : 1)) return;
-  const ProbDev& P = B ? tab[i] : single;
+  PD& P = B ? *(PD*)(tab + i) : *(PD*)&single;
   OcState* o = P.oc;
   o->lo = 0.0;
   o->hi = 1e9;
@@ -154,9 +154,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
   constexpr int NL = (1 << D) - 1;
   __shared__ double qs[NL];
   __shared__ double red[4 * NL];
-  __shared__ int s_last;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   OcState* o = P.oc;
   if (o->stopped) return;
   const double lo = o->lo, hi = o->hi;
@@ -173,8 +172,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
   for (int k = 0; k < TOPT_OC_PER_THREAD; ++k) {
     const long e = base + (long)k * TOPT_BLOCK;
     if (e >= ne) break;
-    const double r = P.rho[e];
-    const double g = -P.dcf[e];
+    const double r = gp(P.rho)[e];
+    const double g = -gp(P.dcf)[e];
     const double a = r * sqrt(g > 0.0 ? g : 0.0);
     const double rl = (r - mv) > 0.001 ? (r - mv) : 0.001;
     const double rh = (r + mv) < 1.0 ? (r + mv) : 1.0;
@@ -185,7 +184,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
       acc[j] += (c > rl ? c : rl);
     }
   }
-  // block reduction of NL sums
+  // block reduction of NL sums (fixed order), then the grid tree
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int j = 0; j < NL; ++j) {
@@ -193,39 +192,20 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
     if (lane == 0) red[wv * NL + j] = t;
   }
   __syncthreads();
-  double* part = P.opart;
-  if (threadIdx.x < NL) {
-    const double t = red[threadIdx.x] + red[NL + threadIdx.x] + red[2 * NL + threadIdx.x] +
-                     red[3 * NL + threadIdx.x];
-    __hip_atomic_store(&part[(long)lb * NL + threadIdx.x], t, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
+  double vsum[NL];
   if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(&o->cnt[1], 1u);
-    s_last = (t == (unsigned)(P.nb[G_OC] - 1));
+#pragma unroll
+    for (int j = 0; j < NL; ++j) vsum[j] = red[j] + red[NL + j] + red[2 * NL + j] + red[3 * NL + j];
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // last block: fixed-order totals, one wave per group of lambdas
-  const int nbk = P.nb[G_OC];
-  for (int j = wv; j < NL; j += TOPT_BLOCK / 64) {
-    double t = 0.0;
-    for (int i = lane; i < nbk; i += 64)
-      t += __hip_atomic_load(&part[(long)i * NL + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t = wave_sum(t);
-    if (lane == 0) red[j] = t;
-  }
-  __syncthreads();
+  double tot[NL];
+  if (!tree_reduce<NL>(vsum, P.opart, P.ctr[G_OC], lb, P.nb[G_OC], tot)) return;
   if (threadIdx.x == 0) {
     double l = lo, h = hi;
     int node = 0, steps = o->steps;
     const double nd = (double)ne;
     for (int d = 0; d < D; ++d) {
       const double mid = 0.5 * (l + h);
-      const double vol = red[node] / nd;
+      const double vol = tot[node] / nd;
       if (vol > P.volfrac) {
         l = mid;
         node = 2 * node + 2;
@@ -243,7 +223,6 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
     o->lo = l;
     o->hi = h;
     o->steps = steps;
-    o->cnt[1] = 0u;
   }
 }
 
@@ -254,9 +233,8 @@ template <bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_apply(const ProbDev* tab, const int* bmap,
                                                          ProbDev single) {
   __shared__ double red[8];
-  __shared__ int s_last;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const ProbDev& P = get_prob<B>(tab, bmap, single, b);
+  PD& P = get_prob<B>(tab, bmap, single, b);
   OcState* o = P.oc;
   const int lb = b - P.b0[G_OC];
   const long ne = (long)P.nelx * P.nely;
@@ -266,16 +244,16 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_apply(const ProbDev* tab, con
   for (int k = 0; k < TOPT_OC_PER_THREAD; ++k) {
     const long e = base + (long)k * TOPT_BLOCK;
     if (e >= ne) break;
-    const double r = P.rho[e];
-    const double dv = P.dv ? P.dv[e] : 1.0;
-    double be = -P.dcf[e] / (lam * dv + 1e-20);
+    const double r = gp(P.rho)[e];
+    const double dv = P.dv ? gp(P.dv)[e] : 1.0;
+    double be = -gp(P.dcf)[e] / (lam * dv + 1e-20);
     be = be > 0.0 ? be : 0.0;
     const double cand = r * sqrt(be);
     const double rl = (r - mv) > 0.001 ? (r - mv) : 0.001;
     const double rh = (r + mv) < 1.0 ? (r + mv) : 1.0;
     const double c = cand < rh ? cand : rh;
     const double rn = c > rl ? c : rl;
-    P.rho[e] = rn;
+    gp(P.rho)[e] = rn;
     vsum += rn;
     const double d = fabs(rn - r);
     vmax = d > vmax ? d : vmax;
@@ -288,48 +266,27 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_apply(const ProbDev* tab, con
     red[4 + wv] = vmax;
   }
   __syncthreads();
-  double* part = P.opart;
+  double v2[2] = {0.0, 0.0};
   if (threadIdx.x == 0) {
-    const double s = red[0] + red[1] + red[2] + red[3];
+    v2[0] = red[0] + red[1] + red[2] + red[3];
     double m = red[4];
     for (int w = 5; w < 8; ++w) m = red[w] > m ? red[w] : m;
-    __hip_atomic_store(&part[2L * lb], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&part[2L * lb + 1], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned t = atomicAdd(&o->cnt[2], 1u);
-    s_last = (t == (unsigned)(P.nb[G_OC] - 1));
+    v2[1] = m;
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double ts = 0.0, tm = 0.0;
-  for (int i = threadIdx.x; i < P.nb[G_OC]; i += TOPT_BLOCK) {
-    ts += __hip_atomic_load(&part[2L * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double m = __hip_atomic_load(&part[2L * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    tm = m > tm ? m : tm;
-  }
-  ts = wave_sum(ts);
-  tm = wave_max(tm);
-  __syncthreads();
-  if (lane == 0) {
-    red[wv] = ts;
-    red[4 + wv] = tm;
-  }
-  __syncthreads();
+  double tot[2];
+  if (!tree_reduce<2>(v2, P.opart, P.ctr[G_OC], lb, P.nb[G_OC], tot, 2u)) return;
   if (threadIdx.x == 0) {
-    const double s = red[0] + red[1] + red[2] + red[3];
-    double m = red[4];
-    for (int w = 5; w < 8; ++w) m = red[w] > m ? red[w] : m;
+    const double s = tot[0], m = tot[1];
     o->vol = s / (double)ne;
     o->change = m;
     o->iter += 1;
-    o->cnt[2] = 0u;
   }
 }
 
 // Mean of an element field (fem/kernels.py:330-336) into out[0].
-__global__ __launch_bounds__(TOPT_BLOCK) void k_volume(ProbDev P, const double* __restrict__ v,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_volume(ProbDev single, const double* __restrict__ v,
                                                        double* out) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   __shared__ double sh[16];
   const int lb = blockIdx.x;
   const long ne = (long)P.nelx * P.nely;
@@ -341,7 +298,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_volume(ProbDev P, const double* 
   }
   block_sum<1>(acc, sh);
   double tot[1];
-  if (last_block_sum<1>(acc, P.opart, &P.oc->cnt[3], lb, P.nb[G_OC], tot, sh)) {
+  if (tree_reduce<1>(acc, P.opart, P.ctr[G_OC], lb, P.nb[G_OC], tot)) {
     if (threadIdx.x == 0) out[0] = tot[0] / (double)ne;
   }
 }

@@ -32,14 +32,12 @@ struct CgState {
   double rz, rz_prev, alpha, beta, pAp, rr, rr_true, rmax2;
   int it, status, mode, pending;
   int pcur, first, refines, pad;
-  unsigned cnt[4];          // arrival counters for last-block reductions
 };
 
 // OC / SIMP per-iteration scalars of one problem.
 struct OcState {
   double lo, hi, last_mid, c, vol, change;
   int steps, stopped, iter, pad;
-  unsigned cnt[4];
 };
 
 // Block grids a batch is launched over; each problem owns nb[g] consecutive
@@ -72,22 +70,22 @@ struct ProbDev {
   // design fields (FP64, per element)
   double* rho; double* rhof; double* dc; double* dcf; const double* dv; double* Hs;
   const Tap* taps;
-  CgState* cg; double* part;   // part: nb[G_NODE] * 2 doubles
-  OcState* oc; double* opart;  // opart: max(nb[G_ELEM], nb[G_OC] * 64) doubles
+  CgState* cg; double* part;   // node-grid tree partials: tree_parts(nb[G_NODE]) * 2
+  OcState* oc; double* opart;  // elem / OC-grid tree partials
+  unsigned* ctr[4];            // tree counters per grid (G_NODE, G_ELEM, -, G_OC)
 };
 
 // ------------------------------------------------------------ vector types
+// Two-component node vectors as native clang vectors (not HIP's class
+// types): assignable through address-space-qualified pointers and loaded
+// as one dwordx2 / dwordx4.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
 template <typename T> struct V2;
-template <> struct V2<float> { using t = float2; };
-template <> struct V2<double> { using t = double2; };
+template <> struct V2<float> { using t = f2v; };
+template <> struct V2<double> { using t = d2v; };
 template <typename T> using vec2 = typename V2<T>::t;
 
-template <typename T> __device__ __forceinline__ const T* keptr(const ProbDev& P);
-template <> __device__ __forceinline__ const float* keptr<float>(const ProbDev& P) { return P.Ke32; }
-template <> __device__ __forceinline__ const double* keptr<double>(const ProbDev& P) { return P.Ke64; }
-template <typename T> __device__ __forceinline__ const T* kiptr(const ProbDev& P);
-template <> __device__ __forceinline__ const float* kiptr<float>(const ProbDev& P) { return P.Ki32; }
-template <> __device__ __forceinline__ const double* kiptr<double>(const ProbDev& P) { return P.Ki64; }
 
 // ------------------------------------------------------------ block mapping
 // The hardware hands workgroup b to XCD (b % 8). Remap so each XCD works on a
@@ -100,17 +98,44 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return x * q + (x < r ? x : r) + j;
 }
 
-// Problem descriptor + local block index for this workgroup.
+// Descriptors are read through the constant address space: the table is
+// never written by a kernel, so every field comes in with scalar loads
+// (one per wave) instead of per-lane vector loads.
+typedef const __attribute__((address_space(4))) ProbDev PD;
+// Data pointers taken from a descriptor are generic; casting them to the
+// global address space gives global_load/store instead of flat_load/store.
+#define GLB __attribute__((address_space(1)))
+template <typename X>
+__device__ __forceinline__ GLB X* gp(X* p) { return (GLB X*)p; }
+template <typename X>
+__device__ __forceinline__ const GLB X* gp(const X* p) { return (const GLB X*)p; }
+
+// Read-only tables (filter taps) through the constant address space: scalar loads.
+#define CST __attribute__((address_space(4)))
+template <typename X>
+__device__ __forceinline__ const CST X* cp(const X* p) { return (const CST X*)p; }
+
+// Problem descriptor of this workgroup (table entry or the by-value single).
 template <bool BATCHED>
-__device__ __forceinline__ const ProbDev& get_prob(const ProbDev* tab, const int* bmap,
-                                                   const ProbDev& single, int b) {
-  if constexpr (BATCHED) return tab[bmap[b]];
-  else return single;
+__device__ __forceinline__ PD& get_prob(const ProbDev* tab, const int* bmap,
+                                        const ProbDev& single, int b) {
+  if constexpr (BATCHED) return *(PD*)(tab + bmap[b]);
+  else return *(PD*)&single;
 }
 
 __device__ __forceinline__ unsigned fixbits(const uint32_t* fix, long n) {
-  return (fix[n >> 4] >> (2 * (unsigned)(n & 15))) & 3u;
+  return (gp(fix)[n >> 4] >> (2 * (unsigned)(n & 15))) & 3u;
 }
+
+template <typename T> struct KeSel;
+template <> struct KeSel<float> {
+  static __device__ __forceinline__ const __attribute__((address_space(4))) float* ke(PD& P) { return P.Ke32; }
+  static __device__ __forceinline__ const __attribute__((address_space(4))) float* ki(PD& P) { return P.Ki32; }
+};
+template <> struct KeSel<double> {
+  static __device__ __forceinline__ const __attribute__((address_space(4))) double* ke(PD& P) { return P.Ke64; }
+  static __device__ __forceinline__ const __attribute__((address_space(4))) double* ki(PD& P) { return P.Ki64; }
+};
 
 // ------------------------------------------------------------ reductions
 template <typename T>
@@ -149,44 +174,122 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh /* >= 4*NV
   __syncthreads();
 }
 
-// Last-block pattern: every block deposits NV partials (thread 0 holds them),
-// the block that arrives last sums all partials of its problem in a fixed
-// order (independent of arrival order and of other problems in the batch) and
-// returns true with the totals in tot[] (valid in every thread).
+// Cross-block hand-off without an L2 flush per block. A block publishes its
+// partials with agent-scope atomic stores (written through the XCD-local
+// L2), waits until they are performed (s_waitcnt vmcnt(0)), then counts its
+// arrival with an agent-scope atomic. Only the block that arrives last
+// executes an agent-scope acquire (one L2 invalidate per kernel) and then
+// reads the partials with ordinary loads, which the compiler batches.
+// __threadfence() in every block would instead write back and invalidate
+// the whole L2 once per block. Data other than the partials is only read by
+// later kernels (kernel boundaries order it).
+__device__ __forceinline__ void publish(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Called by one thread after its publish() calls; true for the last arrival.
+__device__ __forceinline__ bool arrive(unsigned* counter, unsigned nb) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return t == nb - 1;
+}
+// Every thread of the last block calls this once before reading partials.
+__device__ __forceinline__ void acquire_partials() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+__device__ __forceinline__ double collect(const double* p) { return gp(p)[0]; }
+// The last block re-arms the counter for the next launch.
+__device__ __forceinline__ void rearm(unsigned* counter) {
+  __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Deterministic grid reduction over a 64-ary tree of blocks.
+// Level 0 holds one partial per block; blocks [64g, 64g+64) form group g
+// with its own arrival counter (counters 256 B apart, so at most 64
+// atomics ever meet on one address). The last arriver of a group folds the
+// group's partials in lane order with a wave-level tree, publishes the
+// result one level up and arrives there; the block that folds the last
+// level returns true with the totals. The summation order depends only on
+// the number of blocks, never on arrival order or on other problems in a
+// batch. maxmask: bit k set -> component k is combined with max, else sum.
+// part: tree_parts(nb) * NV doubles; ctr: tree_ctrs(nb) * TOPT_CTR_STRIDE
+// uints, zero between launches (every group re-arms its counter).
+#define TOPT_TREE 64
+#define TOPT_CTR_STRIDE 64
+
+__host__ __device__ inline long tree_parts(long nb) {
+  long t = 0, n = nb;
+  for (;;) {
+    t += n;
+    if (n <= 1) return t;
+    n = (n + TOPT_TREE - 1) / TOPT_TREE;
+  }
+}
+__host__ __device__ inline long tree_ctrs(long nb) {
+  long t = 0, n = nb;
+  while (n > 1) {
+    n = (n + TOPT_TREE - 1) / TOPT_TREE;
+    t += n;
+  }
+  return t;
+}
+
 template <int NV>
-__device__ bool last_block_sum(const double (&v)[NV], double* part, unsigned* counter, int lb,
-                               int nb, double (&tot)[NV], double* sh) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
+__device__ bool tree_reduce(double (&v)[NV], double* part, unsigned* ctr, int lb, int nb,
+                            double (&tot)[NV], unsigned maxmask = 0u) {
+  __shared__ double tsh[NV];
+  __shared__ int s_go;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  long idx = lb, n = nb;
+  double* lvl = part;
+  unsigned* cl = ctr;
+  for (;;) {
+    if (n <= 1) {
+      if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&part[(long)lb * NV + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    s_last = (t == (unsigned)(nb - 1));
+        for (int k = 0; k < NV; ++k) tsh[k] = v[k];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < NV; ++k) tot[k] = tsh[k];
+      __syncthreads();
+      return true;
+    }
+    const long g = idx / TOPT_TREE, first = g * TOPT_TREE;
+    const int cnt = (int)((n - first) < TOPT_TREE ? (n - first) : TOPT_TREE);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) publish(&lvl[idx * NV + k], v[k]);
+      s_go = arrive(&cl[g * TOPT_CTR_STRIDE], (unsigned)cnt);
+    }
+    __syncthreads();
+    if (!s_go) return false;
+    acquire_partials();
+    for (int k = wv; k < NV; k += nw) {
+      const bool mx = (maxmask >> k) & 1u;
+      double x = lane < cnt ? collect(&lvl[(first + lane) * NV + k]) : (mx ? -1.0e308 : 0.0);
+      x = mx ? wave_max(x) : wave_sum(x);
+      if (lane == 0) tsh[k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[k] = tsh[k];
+      rearm(&cl[g * TOPT_CTR_STRIDE]);
+    }
+    __syncthreads();
+    lvl += n * NV;
+    cl += ((n + TOPT_TREE - 1) / TOPT_TREE) * TOPT_CTR_STRIDE;
+    idx = g;
+    n = (n + TOPT_TREE - 1) / TOPT_TREE;
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-      acc[k] += __hip_atomic_load(&part[(long)j * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  block_sum<NV>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) sh[k] = acc[k];
-    *counter = 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) tot[k] = sh[k];
-  __syncthreads();
-  return true;
+}
+
+// Block sum followed by the grid tree (the usual pattern of the kernels).
+template <int NV>
+__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* part, unsigned* ctr, int lb, int nb,
+                                         double (&tot)[NV], double* sh) {
+  block_sum<NV>(v, sh);
+  return tree_reduce<NV>(v, part, ctr, lb, nb, tot);
 }
 
 // ------------------------------------------------------------ Q4 stencil
@@ -195,8 +298,8 @@ __device__ bool last_block_sum(const double (&v)[NV], double* part, unsigned* co
 // four adjacent elements sc[a][b] = s(ix-1+a, iy-1+b) (0 outside the grid).
 // Element (ex, ey) has local nodes BL(0,0) BR(1,0) TR(1,1) TL(0,1); our node
 // is its local node (1-a, 1-b).
-template <typename T>
-__device__ __forceinline__ void q4_gather(const T* __restrict__ K, const T (&sc)[2][2],
+template <typename T, typename KP>
+__device__ __forceinline__ void q4_gather(KP K, const T (&sc)[2][2],
                                           const vec2<T> (&q)[3][3], T& ox, T& oy) {
   constexpr int CX[4] = {0, 1, 1, 0};
   constexpr int CY[4] = {0, 0, 1, 1};
@@ -224,8 +327,9 @@ __device__ __forceinline__ void q4_gather(const T* __restrict__ K, const T (&sc)
 
 // Scales of the four elements around node (ix, iy); zero outside the grid.
 template <typename T>
-__device__ __forceinline__ void load_scales(const T* __restrict__ s, int ix, int iy, int nelx,
+__device__ __forceinline__ void load_scales(const T* s_generic, int ix, int iy, int nelx,
                                             int nely, T (&sc)[2][2]) {
+  const GLB T* s = gp(s_generic);
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
 #pragma unroll
