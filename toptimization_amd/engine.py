@@ -74,7 +74,8 @@ class DesignState:
 
     def __init__(self, problem: Problem, precision: str | None = None,
                  preconditioner: str | None = None, delta: float = 0.1,
-                 device: torch.device | None = None, rho0: np.ndarray | None = None):
+                 device: torch.device | None = None, rho0: np.ndarray | None = None,
+                 Ke: np.ndarray | None = None, f: np.ndarray | None = None):
         if problem.dim != 2:
             raise NotImplementedError("the GPU path covers 2-D Q4 problems only")
         self.problem = problem
@@ -105,14 +106,14 @@ class DesignState:
         self.dcf = self._view("dcf", torch.float64, self.n_elem)
         self.Hs = self._view("Hs", torch.float64, self.n_elem)
 
-        f_np = force_vector(problem)
+        f_np = force_vector(problem) if f is None else np.asarray(f, dtype=np.float64).ravel()
         self.f.copy_(torch.from_numpy(f_np))
         self.fix.copy_(torch.from_numpy(fixed_bitmask(nd, problem.fixed_dofs).view(np.int32)))
         if rho0 is None:
             self.rho.fill_(float(problem.volfrac))
         else:
             self.rho.copy_(torch.as_tensor(np.asarray(rho0, dtype=np.float64).ravel()))
-        self.Ke = compute_Ke_2d(problem.E, problem.nu)
+        self.Ke = compute_Ke_2d(problem.E, problem.nu) if Ke is None else np.asarray(Ke, dtype=np.float64)
         self.desc = TProblem()
         d = self.desc
         d.nelx, d.nely = nelx, nely

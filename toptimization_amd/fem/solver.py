@@ -1,0 +1,63 @@
+"""Matrix-free PCG solve with the reference's signature (``fem/solver.py:27-159``).
+
+``solve`` works on the module fields (``fields.u, f, rho, u_prev``) exactly as
+the reference does: rho is the physical density, the result lands in
+``fields.u`` and is copied to ``fields.u_prev``, the return value is the
+reference's iteration count (``max_iter`` on non-convergence or breakdown,
+0 for a zero load). The whole loop runs on the GPU engine: the device
+computes alpha/beta/convergence, the host polls one status word per chunk.
+
+Extra keyword ``precision``: "fp64" (default) or "mixed".
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+from ..engine import DesignState, Plan
+from ..problem import PRECONDITIONERS, Problem
+from . import fields as F
+
+_cache: dict = {}
+
+
+def _state_for(E_min, penalty, max_iter, tol, preconditioner, precision):
+    key = (id(F.u), F.NELX, F.NELY, float(E_min), float(penalty), int(max_iter), float(tol),
+           preconditioner, precision)
+    hit = _cache.get(key)
+    if hit is not None:
+        return hit
+    nelx, nely = F.grid()
+    fixed = np.flatnonzero(F.is_fixed.cpu().numpy())
+    f_np = F.f.double().cpu().numpy()
+    nz = np.flatnonzero(f_np)
+    prob = Problem(name="fields", dim=2, nelx=nelx, nely=nely, nelz=1, E=1.0, nu=0.3, E_min=float(E_min),
+                   fixed_dofs=fixed, force_dofs=nz, force_values=f_np[nz], volfrac=0.5,
+                   penalty=float(penalty), rmin=1.5, filter_type="density", max_iter=1, tol=0.0,
+                   move_limit=0.2, max_cg_iter=int(max_iter), cg_tol=float(tol),
+                   preconditioner=preconditioner, output_dir=Path("."), save_interval=1,
+                   output_format="png", precision=precision)
+    st = DesignState(prob, Ke=F.KE_NP, f=f_np)
+    hit = _cache[key] = (st, Plan([st]))
+    return hit
+
+
+def solve(E_min: float, penalty: float, dim: int, max_iter: int = 2000, tol: float = 1e-8,
+          preconditioner: str = "jacobi", warm_start: bool = True, *, precision: str = "fp64") -> int:
+    """Solve K(F.rho) F.u = F.f; returns the number of CG iterations."""
+    if dim != 2:
+        raise NotImplementedError("3-D H8 solves are not on the GPU path")
+    if preconditioner not in PRECONDITIONERS:
+        raise ValueError(f"unknown preconditioner {preconditioner!r}")
+    st, plan = _state_for(E_min, penalty, max_iter, tol, preconditioner, precision)
+    st.f.copy_(F.f)
+    st.rhof.copy_(F.rho)
+    if warm_start:
+        st.u.copy_(F.u_prev)
+    it, status = plan.solve(warm=1 if warm_start else 0, use_filtered=True)
+    F.u.copy_(st.u)
+    if status[0] != 4:            # zero load returns before u_prev is written (solver.py:60-63)
+        F.u_prev.copy_(st.u)
+    return int(it[0])

@@ -159,3 +159,16 @@ def volume(nelx: int, nely: int, v: torch.Tensor) -> float:
     ws = workspace(nelx, nely, v.device)
     check(_lib.load().topt_volume(int(nelx), int(nely), _p(v), _p(out), _p(ws), _s()), "topt_volume")
     return float(out.item())
+
+
+def blas1(op: str, n: int, a: float = 0.0, x=None, y=None, z=None, w=None, v=None, d=None,
+          fix: torch.Tensor | None = None, reduce: bool = False):
+    """Vector op of ``topt_blas1`` (see include/topt.h); returns the 2-vector
+    of reductions (device tensor) when reduce=True."""
+    ref = next(t for t in (x, y, z, w, v) if t is not None)
+    red = torch.zeros(2, dtype=torch.float64, device=ref.device) if reduce else None
+    ws = (torch.empty(int(_lib.load().topt_blas1_workspace_bytes()), dtype=torch.uint8, device=ref.device)
+          if reduce else None)
+    check(_lib.load().topt_blas1(_lib.B1[op], _dtype_id(ref), int(n), float(a), _p(x), _p(y), _p(z), _p(w),
+                                 _p(v), _p(d), _p(fix), _p(red), _p(ws), _s()), f"topt_blas1({op})")
+    return red

@@ -1,0 +1,208 @@
+"""Continuous-batching design server on one GPU.
+
+Requests (design problems of any size, boundary conditions and loads)
+arrive over time. Every admitted problem keeps its rho / u / PCG / OC state
+resident in its own device workspace. Each engine step advances all active
+problems by one SIMP iteration with shared launches (one kernel per PCG slot
+covers every active problem through the plan's ragged block table).
+Finished problems leave, and waiting ones are admitted as slots and the HBM
+budget allow, so new work overlaps work already on the GPU.
+
+Per-problem results do not depend on the batch: every reduction of a
+problem uses only that problem's blocks in a fixed order, so a design is
+bitwise the same alone and in the mix (tested).
+
+Stopping rule per design: ``it > 5 and change < tol`` or ``it == max_iter``
+(reference ``optimizer/simp.py:217``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .engine import DesignState, Plan, PRECISION_IDS
+from .problem import Problem, make_problem
+
+
+@dataclass
+class Request:
+    rid: int
+    problem: Problem
+    arrival: float                     # seconds since the stream started
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    iterations: int
+    compliance: float
+    volume: float
+    change: float
+    n_cg: int
+    rho: np.ndarray | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def footprint_bytes(problem: Problem, precision: str, preconditioner: str | None = None) -> int:
+    """Device bytes one resident problem needs (its workspace layout)."""
+    offs = (C.c_int64 * len(_lib.BUF))()
+    total = C.c_int64()
+    pc = _lib.PRECONDITIONER_IDS[preconditioner or problem.preconditioner]
+    _lib.check(_lib.load().topt_problem_layout(problem.nelx, problem.nely, PRECISION_IDS[precision], pc,
+                                               float(problem.rmin), offs, C.byref(total)),
+               "topt_problem_layout")
+    return int(total.value)
+
+
+@dataclass
+class _Active:
+    req: Request
+    state: DesignState
+    admitted: float
+    it: int = 0
+    n_cg: int = 0
+
+
+@dataclass
+class ServerStats:
+    results: list = field(default_factory=list)
+    makespan: float = 0.0
+    steps: int = 0
+    max_active: int = 0
+
+    def summary(self) -> dict:
+        lat = np.array([r.latency for r in self.results]) if self.results else np.zeros(1)
+        return {"designs": len(self.results), "makespan_s": self.makespan,
+                "p50_latency_s": float(np.percentile(lat, 50)),
+                "p99_latency_s": float(np.percentile(lat, 99)),
+                "designs_per_hour": 3600.0 * len(self.results) / max(self.makespan, 1e-9),
+                "engine_steps": self.steps, "max_active": self.max_active}
+
+
+class DesignServer:
+    """Admission by slot count and HBM budget; one SIMP iteration per step
+    for every active design."""
+
+    def __init__(self, max_active: int = 64, hbm_budget: int = 32 << 30, precision: str = "mixed",
+                 preconditioner: str | None = None, keep_rho: bool = False):
+        self.max_active = int(max_active)
+        self.hbm_budget = int(hbm_budget)
+        self.precision = precision
+        self.preconditioner = preconditioner
+        self.keep_rho = keep_rho
+
+    def run(self, requests: list[Request], clock: str = "wall") -> ServerStats:
+        """Serve the stream. clock='wall': arrivals are real time offsets from
+        the start; clock='instant': every request is available at t=0."""
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active: list[_Active] = []
+        stats = ServerStats()
+        plan: Plan | None = None
+        used = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while pending or active:
+            t = now()
+            changed = False
+            # admission: arrived requests, in order, while slots and memory allow
+            while pending and len(active) < self.max_active:
+                r = pending[0]
+                if clock == "wall" and r.arrival > t:
+                    break
+                need = footprint_bytes(r.problem, self.precision, self.preconditioner)
+                if active and used + need > self.hbm_budget:
+                    break
+                pending.pop(0)
+                st = DesignState(r.problem, precision=self.precision, preconditioner=self.preconditioner)
+                active.append(_Active(r, st, t))
+                used += need
+                changed = True
+            if not active:
+                if pending and clock == "wall":
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                continue
+            if changed or plan is None:
+                if plan is not None:
+                    plan.close()
+                plan = Plan([a.state for a in active])
+            stats.max_active = max(stats.max_active, len(active))
+            sens = [a.req.problem.filter_type == "sensitivity" for a in active]
+            if any(sens) and not all(sens):
+                raise NotImplementedError("a batch mixes density and sensitivity filtering")
+            out = plan.iteration(sensitivity_filter=sens[0])
+            stats.steps += 1
+            t = now()
+            keep = []
+            for a, s in zip(active, out):
+                a.it += 1
+                a.n_cg += s.n_cg
+                pr = a.req.problem
+                if a.it >= pr.max_iter or (a.it > 5 and s.change < pr.tol):
+                    stats.results.append(Result(
+                        a.req.rid, a.req.arrival, a.admitted, t, a.it, s.compliance, s.volume, s.change,
+                        a.n_cg, a.state.rho.cpu().numpy() if self.keep_rho else None))
+                    used -= footprint_bytes(pr, self.precision, self.preconditioner)
+                    changed = True
+                else:
+                    keep.append(a)
+            if len(keep) != len(active):
+                active = keep
+                if plan is not None:
+                    plan.close()
+                plan = None
+        if plan is not None:
+            plan.close()
+        stats.makespan = now()
+        stats.results.sort(key=lambda r: r.rid)
+        return stats
+
+
+def synthetic_stream(n: int = 512, rate: float = 20.0, seed: int = 0, nelx_range=(64, 768),
+                     max_iter: int = 100, tol: float = 0.01, **solver) -> list[Request]:
+    """Seeded request stream: Poisson arrivals at `rate`/s; nelx uniform over
+    multiples of 16 in nelx_range; nely = round(nelx * aspect), aspect in
+    {1/4, 1/3, 1/2} (round half to even); BC type uniform over cantilever /
+    MBB / bridge; seeded load node and angle; volfrac U[0.3, 0.5];
+    rmin = 0.008 nelx + 1."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    sizes = np.arange(nelx_range[0], nelx_range[1] + 1, 16)
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        nelx = int(rng.choice(sizes))
+        aspect = float(rng.choice([0.25, 1.0 / 3.0, 0.5]))
+        nely = max(1, int(round(nelx * aspect)))
+        kind = ["cantilever", "mbb", "bridge"][int(rng.integers(0, 3))]
+        th = float(rng.uniform(0.0, 2.0 * np.pi))
+        fx, fy = float(np.cos(th)), float(np.sin(th))
+        if kind == "cantilever":
+            fixed = [{"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x", "y"]}]
+            loads = [{"nodes": {"type": "point", "location": [nelx, int(rng.integers(0, nely + 1))]},
+                      "force": [fx, fy]}]
+        elif kind == "mbb":
+            fixed = [{"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x"]},
+                     {"nodes": {"type": "point", "corner": "bottom-right"}, "dofs": ["y"]}]
+            loads = [{"nodes": {"type": "point", "location": [int(rng.integers(0, nelx + 1)), nely]},
+                      "force": [0.0, -1.0]}]
+        else:
+            fixed = [{"nodes": {"type": "point", "corner": "bottom-left"}, "dofs": ["x", "y"]},
+                     {"nodes": {"type": "point", "corner": "bottom-right"}, "dofs": ["x", "y"]}]
+            loads = [{"nodes": {"type": "edge", "edge": "top"}, "force": [0.0, -1.0 / (nelx + 1)]}]
+        prob = make_problem(nelx, nely, fixed=fixed, loads=loads, name=f"{kind}-{i}",
+                            volume_fraction=float(rng.uniform(0.3, 0.5)),
+                            filter_radius=0.008 * nelx + 1.0, max_iterations=max_iter, tolerance=tol,
+                            **solver)
+        out.append(Request(i, prob, t))
+    return out
