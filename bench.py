@@ -151,6 +151,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="profiling driver: one OC iteration, 100 slot-A launches, op-level matvecs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a design problem is "
@@ -161,6 +163,17 @@ def main() -> int:
     import torch
     torch.cuda.set_device(0)
     nn, ne = (NELX + 1) * (NELY + 1), NELX * NELY
+    if args.probe:
+        from toptimization_amd.engine import DesignState, Plan
+        st = DesignState(bridge(NELX, NELY, "mixed"))
+        plan = Plan([st])
+        s = plan.iteration()[0]
+        log(f"[probe] OC iteration 1: c={s.compliance:.6e} n_cg={s.n_cg}")
+        kernel_roofline(plan, nn, ne, log, reps=100)
+        matvec_gdofs(NELX, NELY, log, reps=50)
+        matvec_gdofs(8192, 4096, log, reps=10)
+        plan.close()
+        return 0
 
     log(f"[bench] {NELX}x{NELY} bridge, warmup {args.warmup}, steps {args.steps}")
     mixed = run_path("mixed", args.warmup, args.steps, log)

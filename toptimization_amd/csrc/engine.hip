@@ -47,10 +47,10 @@ int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 struct Geom {
   int64_t nn, ne, words;
-  int nb[4];
+  int nb[TOPT_NGRID];
 };
 
-Geom geom(int nelx, int nely) {
+Geom geom(int nelx, int nely, int dtype = TOPT_F64) {
   Geom g;
   g.nn = int64_t(nelx + 1) * (nely + 1);
   g.ne = int64_t(nelx) * nely;
@@ -59,6 +59,8 @@ Geom geom(int nelx, int nely) {
   g.nb[G_ELEM] = (int)cdiv(g.ne, TOPT_BLOCK);
   g.nb[G_FILT] = (int)(cdiv(nelx, TOPT_FILT_TX) * cdiv(nely, TOPT_FILT_TY));
   g.nb[G_OC] = (int)cdiv(g.ne, int64_t(TOPT_BLOCK) * TOPT_OC_PER_THREAD);
+  const int npt = dtype == TOPT_F32 ? Npt<float>::v : Npt<double>::v;
+  g.nb[G_PCG] = (int)cdiv(g.nn, int64_t(TOPT_BLOCK) * npt);
   return g;
 }
 
@@ -104,7 +106,7 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   if (dtype != TOPT_F32 && dtype != TOPT_F64) return fail(TOPT_EINVAL, "dtype must be TOPT_F32 or TOPT_F64");
   if (precond < TOPT_PC_NONE || precond > TOPT_PC_BLOCK2) return fail(TOPT_EINVAL, "unknown preconditioner");
   if (!(rmin > 0.0)) return fail(TOPT_EINVAL, "rmin must be > 0");
-  const Geom g = geom(nelx, nely);
+  const Geom g = geom(nelx, nely, dtype);
   const int64_t ts = dtype == TOPT_F32 ? 4 : 8;
   const int64_t ntaps = (int64_t)build_taps(rmin).size();
   int64_t sz[TOPT_BUF_COUNT] = {0};
@@ -119,9 +121,9 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   sz[TOPT_BUF_S] = g.ne * ts;
   sz[TOPT_BUF_S64] = dtype == TOPT_F32 ? g.ne * 8 : 0;     // plain path: s64 aliases s
   sz[TOPT_BUF_TAPS] = std::max<int64_t>(ntaps, 1) * (int64_t)sizeof(Tap);
-  sz[TOPT_BUF_CG] = sizeof(CgState);
+  sz[TOPT_BUF_CG] = 2 * sizeof(CgState);
   sz[TOPT_BUF_OC] = sizeof(OcState);
-  sz[TOPT_BUF_PART] = tree_parts(g.nb[G_NODE]) * 2 * 8;
+  sz[TOPT_BUF_PART] = std::max<int64_t>(tree_parts(g.nb[G_NODE]) * 2, 4 * int64_t(g.nb[G_PCG])) * 8;
   sz[TOPT_BUF_OPART] = std::max<int64_t>(tree_parts(g.nb[G_ELEM]), tree_parts(g.nb[G_OC]) * 32) * 8;
   sz[TOPT_BUF_CTR] = ctr_words(g) * 4;
   int64_t o = 0;
@@ -191,10 +193,10 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
   int rc = make_layout(p.nelx, p.nely, dtype, p.precond, p.rmin, L);
   if (rc) return rc;
   if (p.ws == nullptr) return fail(TOPT_EINVAL, "problem workspace is NULL");
-  const Geom g = geom(p.nelx, p.nely);
+  const Geom g = geom(p.nelx, p.nely, dtype);
   P.nelx = p.nelx;
   P.nely = p.nely;
-  for (int k = 0; k < 4; ++k) P.nb[k] = g.nb[k];
+  for (int k = 0; k < TOPT_NGRID; ++k) P.nb[k] = g.nb[k];
   P.precond = p.precond;
   P.max_it = p.max_cg_iter;
   const std::vector<Tap> taps = build_taps(p.rmin);
@@ -241,10 +243,10 @@ size_t filter_shmem(int R) {
   return size_t(TOPT_FILT_TX + 2 * R) * size_t(TOPT_FILT_TY + 2 * R) * sizeof(double);
 }
 
-__global__ void k_gather_status(const ProbDev* tab, int n, int* out) {
+__global__ void k_gather_status(const ProbDev* tab, int n, int parity, int* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const CgState* st = tab[i].cg;
+  const CgState* st = tab[i].cg + parity;
   out[2 * i] = st->status;
   out[2 * i + 1] = st->it;
 }
@@ -274,16 +276,27 @@ __global__ void k_cg_force_run(const ProbDev* tab, int n) {
   if (i >= n) return;
   CgState* st = tab[i].cg;
   st->status = ST_RUN;
-  st->mode = MODE_CG;
+  st->op = OP_C;          // next slot A finalises a "C" and runs A_CG
+  st->cflag = CF_CG;
+  st->amode = OP_A_CG;
   st->pending = 0;
-  st->pad = 0;
   st->it = 0;
+  if (!(st->rz > 0.0)) st->rz = 1.0;
 }
 
 __global__ void k_cg_finish(const ProbDev* tab, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  tab[i].cg->status = ST_CONV;
+  tab[i].cg[0].status = ST_CONV;
+  tab[i].cg[1].status = ST_CONV;
+}
+
+// After a solve: make cg[0] hold the final state (the last slot wrote
+// cg[q & 1]).
+__global__ void k_cg_settle(const ProbDev* tab, int n, int parity) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || parity == 0) return;
+  tab[i].cg[0] = tab[i].cg[1];
 }
 
 __global__ void k_oc_out(const OcState* o, double* out) {
@@ -304,8 +317,9 @@ struct topt_plan {
   int max_cg = 0;
   std::vector<ProbDev> host;
   ProbDev* dtab = nullptr;
-  int* dmap[4] = {nullptr, nullptr, nullptr, nullptr};
-  int nblk[4] = {0, 0, 0, 0};
+  int* dmap[TOPT_NGRID] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int nblk[TOPT_NGRID] = {0, 0, 0, 0, 0};
+  int64_t q = 0;          // PCG slot sequence number of the next launch
   int* dstat = nullptr;
   int* hstat[2] = {nullptr, nullptr};
   topt_iter_stats* dstats = nullptr;
@@ -315,20 +329,25 @@ struct topt_plan {
 
 namespace {
 
+// A batched launch over grid kind g: its block count and the block->problem
+// map of that same kind (the map holds exactly nb entries).
+struct GridRef {
+  int nb;
+  const int* map;
+};
+GridRef grid_of(const topt_plan* pl, int g) { return GridRef{pl->nblk[g], pl->dmap[g]}; }
+
 template <typename T>
 int launch_slots(topt_plan* pl, int count, hipStream_t s) {
   const ProbDev dummy{};
-  const int nb = pl->nblk[G_NODE];
-  const int np = pl->n;
+  const GridRef gr = grid_of(pl, G_PCG);
+  const int nb = gr.nb;
+  const int* map = gr.map;
   for (int k = 0; k < count; ++k) {
-    hipLaunchKernelGGL((k_cg_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-    hipLaunchKernelGGL((k_cg_fin<T, true>), dim3(np), dim3(TOPT_BLOCK), 0, s, pl->dtab, np, dummy, 0);
-    hipLaunchKernelGGL((k_cg_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-    hipLaunchKernelGGL((k_cg_fin<T, true>), dim3(np), dim3(TOPT_BLOCK), 0, s, pl->dtab, np, dummy, 1);
-    if (pl->any_elemblock) {
-      hipLaunchKernelGGL((k_cg_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-      hipLaunchKernelGGL((k_cg_fin<T, true>), dim3(np), dim3(TOPT_BLOCK), 0, s, pl->dtab, np, dummy, 2);
-    }
+    hipLaunchKernelGGL((k_slot_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, (int)(pl->q++ & 1));
+    hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, (int)(pl->q++ & 1));
+    if (pl->any_elemblock)
+      hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, (int)(pl->q++ & 1));
   }
   LAUNCH_TRY();
   return TOPT_OK;
@@ -337,7 +356,8 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
 int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, s) : launch_slots<double>(pl, count, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstat);
+  hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n,
+                     (int)(pl->q & 1), pl->dstat);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstat[slot], pl->dstat, sizeof(int) * 2 * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipEventRecord(pl->ev[slot], s));
@@ -353,20 +373,20 @@ bool any_running(const topt_plan* pl, const int* st) {
 template <typename T>
 int launch_solve_setup(topt_plan* pl, int warm, int use_filtered, hipStream_t s) {
   const ProbDev dummy{};
-  hipLaunchKernelGGL((k_scale<T, true>), dim3(pl->nblk[G_ELEM]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                     pl->dmap[G_ELEM], dummy, use_filtered);
-  hipLaunchKernelGGL((k_precond<T, true>), dim3(pl->nblk[G_NODE]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                     pl->dmap[G_NODE], dummy, 0);
-  hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(pl->nblk[G_NODE]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                     pl->dmap[G_NODE], dummy, warm);
+  hipLaunchKernelGGL((k_scale<T, true>), dim3(grid_of(pl, G_ELEM).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     grid_of(pl, G_ELEM).map, dummy, use_filtered);
+  hipLaunchKernelGGL((k_precond<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     grid_of(pl, G_NODE).map, dummy, 0);
+  hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     grid_of(pl, G_NODE).map, dummy, warm);
   LAUNCH_TRY();
   return TOPT_OK;
 }
 
 int plan_filter(topt_plan* pl, int mode, hipStream_t s) {
   const ProbDev dummy{};
-  hipLaunchKernelGGL((k_filter<true>), dim3(pl->nblk[G_FILT]), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s,
-                     pl->dtab, pl->dmap[G_FILT], dummy, mode);
+  hipLaunchKernelGGL((k_filter<true>), dim3(grid_of(pl, G_FILT).nb), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s,
+                     pl->dtab, grid_of(pl, G_FILT).map, dummy, mode);
   LAUNCH_TRY();
   return TOPT_OK;
 }
@@ -376,18 +396,18 @@ int plan_oc(topt_plan* pl, hipStream_t s) {
   constexpr int D = 5;
   hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, dummy);
   for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
-    hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(pl->nblk[G_OC]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                       pl->dmap[G_OC], dummy);
-  hipLaunchKernelGGL((k_oc_apply<true>), dim3(pl->nblk[G_OC]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                     pl->dmap[G_OC], dummy);
+    hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(grid_of(pl, G_OC).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                       grid_of(pl, G_OC).map, dummy);
+  hipLaunchKernelGGL((k_oc_apply<true>), dim3(grid_of(pl, G_OC).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     grid_of(pl, G_OC).map, dummy);
   LAUNCH_TRY();
   return TOPT_OK;
 }
 
 int plan_sens(topt_plan* pl, int use_filtered, hipStream_t s) {
   const ProbDev dummy{};
-  hipLaunchKernelGGL((k_sens<true>), dim3(pl->nblk[G_ELEM]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                     pl->dmap[G_ELEM], dummy, use_filtered);
+  hipLaunchKernelGGL((k_sens<true>), dim3(grid_of(pl, G_ELEM).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     grid_of(pl, G_ELEM).map, dummy, use_filtered);
   LAUNCH_TRY();
   return TOPT_OK;
 }
@@ -396,6 +416,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   int rc = pl->dtype == TOPT_F32 ? launch_solve_setup<float>(pl, warm, use_filtered, s)
                                  : launch_solve_setup<double>(pl, warm, use_filtered, s);
   if (rc) return rc;
+  pl->q = 0;
   // slots issued so far; a solve needs at most max_cg CG slots plus refresh slots
   const int64_t slot_cap = 4 * int64_t(pl->max_cg) + 4096;
   int64_t issued = 0;
@@ -418,6 +439,8 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   }
   // drain the chunk still in flight and take a final snapshot
   HIP_TRY(hipEventSynchronize(pl->ev[wait ^ 1]));
+  hipLaunchKernelGGL(k_cg_settle, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, (int)(pl->q & 1));
+  pl->q = 0;
   hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
@@ -471,7 +494,7 @@ int op_prob(ProbDev& P, int nelx, int nely, void* ws, hipStream_t s) {
   const Geom g = geom(nelx, nely);
   P.nelx = nelx;
   P.nely = nely;
-  for (int k = 0; k < 4; ++k) P.nb[k] = g.nb[k];
+  for (int k = 0; k < TOPT_NGRID; ++k) P.nb[k] = g.nb[k];
   const OpScratch o = op_scratch(nelx, nely);
   char* w = (char*)ws;
   P.part = (double*)(w + o.part_off);
@@ -526,7 +549,7 @@ int topt_problem_init(const topt_problem* prob, int32_t dtype, void* stream) {
   HIP_TRY(hipMemcpyAsync((void*)P.taps, taps.data(), taps.size() * sizeof(Tap), hipMemcpyHostToDevice, s));
   const Geom g = geom(prob->nelx, prob->nely);
   HIP_TRY(hipMemsetAsync(P.u64, 0, g.nn * 16, s));
-  HIP_TRY(hipMemsetAsync(P.cg, 0, sizeof(CgState), s));
+  HIP_TRY(hipMemsetAsync(P.cg, 0, 2 * sizeof(CgState), s));
   HIP_TRY(hipMemsetAsync(P.oc, 0, sizeof(OcState), s));
   HIP_TRY(hipMemsetAsync(P.ctr[G_NODE], 0, ctr_words(g) * 4, s));
   P.b0[G_FILT] = 0;
@@ -543,7 +566,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   pl->n = nprob;
   pl->dtype = dtype;
   pl->host.resize(nprob);
-  std::vector<int> maps[4];
+  std::vector<int> maps[TOPT_NGRID];
   for (int i = 0; i < nprob; ++i) {
     ProbDev& P = pl->host[i];
     const int rc = fill_prob(P, probs[i], dtype);
@@ -551,15 +574,20 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
       delete pl;
       return rc;
     }
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < TOPT_NGRID; ++g) {
       P.b0[g] = (int)maps[g].size();
       maps[g].insert(maps[g].end(), P.nb[g], i);
     }
-    pl->any_elemblock |= (P.precond == PC_ELEMBLOCK);
+    if (i > 0 && (P.precond == PC_ELEMBLOCK) != pl->any_elemblock) {
+      delete pl;
+      return fail(TOPT_EINVAL, "a plan cannot mix the element-block preconditioner with others "
+                               "(its PCG iteration has three slots)");
+    }
+    pl->any_elemblock = (P.precond == PC_ELEMBLOCK);
     pl->maxR = std::max(pl->maxR, P.R);
     pl->max_cg = std::max(pl->max_cg, P.max_it);
   }
-  for (int g = 0; g < 4; ++g) pl->nblk[g] = (int)maps[g].size();
+  for (int g = 0; g < TOPT_NGRID; ++g) pl->nblk[g] = (int)maps[g].size();
   auto cleanup = [&](int code) {
     topt_plan_destroy(pl);
     return code;
@@ -567,7 +595,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   if (hipMalloc(&pl->dtab, sizeof(ProbDev) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc plan table"));
   if (hipMemcpy(pl->dtab, pl->host.data(), sizeof(ProbDev) * nprob, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(TOPT_EHIP, "upload plan table"));
-  for (int g = 0; g < 4; ++g) {
+  for (int g = 0; g < TOPT_NGRID; ++g) {
     if (hipMalloc(&pl->dmap[g], sizeof(int) * maps[g].size()) != hipSuccess)
       return cleanup(fail(TOPT_ENOMEM, "hipMalloc block map"));
     if (hipMemcpy(pl->dmap[g], maps[g].data(), sizeof(int) * maps[g].size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -590,7 +618,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
 int topt_plan_destroy(topt_plan* pl) {
   if (pl == nullptr) return TOPT_OK;
   (void)hipFree(pl->dtab);
-  for (int g = 0; g < 4; ++g) (void)hipFree(pl->dmap[g]);
+  for (int g = 0; g < TOPT_NGRID; ++g) (void)hipFree(pl->dmap[g]);
   (void)hipFree(pl->dstat);
   (void)hipFree(pl->dstats);
   for (int k = 0; k < 2; ++k) {
@@ -643,21 +671,20 @@ int topt_plan_time_slot(topt_plan* pl, int32_t which, int32_t reps, float* ms, v
     return fail(TOPT_EINVAL, "invalid slot timing call (only which = 0 is supported)");
   hipStream_t s = (hipStream_t)stream;
   const ProbDev dummy{};
-  const int nb = pl->nblk[G_NODE];
-  // max_it is never reached: the timed launches do not count iterations past max_cg
+  const GridRef gr = grid_of(pl, G_PCG);
+  const int nb = gr.nb;
+  const int* map = gr.map;
+  // every launch re-reads cg[0] (forced to "after slot C") and repeats one A_CG step
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   hipLaunchKernelGGL(k_cg_force_run, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
   HIP_TRY(hipEventRecord(e0, s));
   for (int k = 0; k < reps; ++k) {
-    if (pl->dtype == TOPT_F32) {
-      if (which == 0) hipLaunchKernelGGL((k_cg_a<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-      else hipLaunchKernelGGL((k_cg_b<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-    } else {
-      if (which == 0) hipLaunchKernelGGL((k_cg_a<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-      else hipLaunchKernelGGL((k_cg_b<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->dmap[G_NODE], dummy);
-    }
+    if (pl->dtype == TOPT_F32)
+      hipLaunchKernelGGL((k_slot_a<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0);
+    else
+      hipLaunchKernelGGL((k_slot_a<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0);
   }
   HIP_TRY(hipEventRecord(e1, s));
   hipLaunchKernelGGL(k_cg_finish, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
@@ -715,7 +742,7 @@ int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, cons
     P.nelx = g->nelx;
     P.nely = g->nely;
     const Geom gm = geom(g->nelx, g->nely);
-    for (int k = 0; k < 4; ++k) P.nb[k] = gm.nb[k];
+    for (int k = 0; k < TOPT_NGRID; ++k) P.nb[k] = gm.nb[k];
     if ((rc = fill_material(P, g->Ke, g->E_min, g->penal))) return rc;
   }
   P.s = (void*)s;
@@ -738,7 +765,7 @@ static int precond_op(const topt_grid* g, int32_t dtype, const void* s, const ui
   P.nelx = g->nelx;
   P.nely = g->nely;
   const Geom gm = geom(g->nelx, g->nely);
-  for (int k = 0; k < 4; ++k) P.nb[k] = gm.nb[k];
+  for (int k = 0; k < TOPT_NGRID; ++k) P.nb[k] = gm.nb[k];
   int rc = fill_material(P, g->Ke, g->E_min, g->penal);
   if (rc) return rc;
   P.precond = kind;

@@ -224,41 +224,189 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
     (gp((vec2<T>*)P.p1))[id.n] = zero;
   }
   if (lb == 0 && threadIdx.x == 0) {
-    CgState* st = P.cg;
-    st->rz = st->rz_prev = st->alpha = st->beta = st->pAp = st->rr = st->rr_true = 0.0;
-    st->rmax2 = 0.0;
-    st->it = 0;
-    st->status = zero_load ? ST_ZERO : ST_RUN;
-    st->mode = MODE_REFINE;
-    st->pending = 0;
-    st->pcur = 0;
-    st->first = 1;
-    st->refines = 0;
-    st->pad = 0;
+    CgState st{};
+    st.status = zero_load ? ST_ZERO : ST_RUN;
+    st.op = OP_START;
+    st.first = 1;
+    st.cflag = CF_CG;
+    st.amode = OP_A_CG;
+    P.cg[0] = st;
   }
 }
 
-// PCG slot kernels write one set of FP64 partials per block (plain stores,
-// no atomics, no state changes); the finalize kernel launched right after
-// each slot (one block per problem) sums them in a fixed order and updates
-// alpha / beta / convergence. Kernel boundaries order the hand-off, so there
-// is no cross-block synchronisation inside a kernel at all.
-template <int NV>
-__device__ __forceinline__ void store_partials(double (&v)[NV], PD& P, int lb, double* sh) {
-  block_sum<NV>(v, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) gp(P.part)[(long)lb * NV + k] = v[k];
-  }
-}
+// ---------------------------------------------------------------- PCG slots
+// The host launches slot kernels A, B[, C], A, B[, C], ... with sequence
+// numbers q = 0, 1, 2, ... Kernel q
+//   1. reads the state cg[q & 1] written by kernel q-1,
+//   2. sums kernel q-1's block partials part[(q-1) & 1] in a fixed order
+//      (every block computes the same totals),
+//   3. advances the state machine (pcg_advance: alpha, beta, convergence,
+//      residual refresh) -- every block reaches the same decision, block 0
+//      stores it in cg[(q+1) & 1],
+//   4. executes the operation that decision selects and writes its own block
+//      partials to part[q & 1].
+// No atomics, no grid barrier, one launch per slot.
 
-// Residual refresh r = f - A(u64 [+ x]) in FP64 (stored in T), partial
-// ||r||^2 in FP64. Starts every solve and, on the mixed path, is the
-// reliable update that replaces the recursive residual.
 template <typename T>
-__device__ void refine_a(PD& P, int lb) {
-  __shared__ double sh[16];
-  const NodeIdx id = node_of(P, lb);
+__device__ __forceinline__ NodeIdx slot_node(PD& P, int lb, int j) {
+  NodeIdx id;
+  const int nny = P.nely + 1;
+  const long nn = (long)(P.nelx + 1) * nny;
+  const long n = ((long)lb * Npt<T>::v + j) * TOPT_BLOCK + threadIdx.x;
+  id.act = n < nn;
+  id.n = id.act ? n : nn - 1;
+  id.ix = (int)(id.n / nny);
+  id.iy = (int)(id.n - (long)id.ix * nny);
+  return id;
+}
+
+__device__ __forceinline__ void refresh_done(CgState& s, double rzn) {
+  s.beta = s.first ? 0.0 : rzn / (s.rz_prev + 1e-60);
+  s.rz = rzn;
+  s.first = 0;
+  s.rmax2 = s.rr_true;
+  s.refines += 1;
+  s.op = OP_A_CG;
+}
+
+// State machine. In: s.op = what kernel q-1 did, t = its reduced partials.
+// Out: s.op = what kernel q does (or status != RUN). Follows fem/solver.py:
+// alpha = rz/pAp (:117), breakdown |pAp| < 1e-30 (:114), convergence on the
+// recursive |r|^2 < tol^2 |f|^2 (:147), beta = rz/(rz_old + 1e-60) (:151).
+// Mixed path: convergence, max_iter and breakdown first commit the FP32
+// correction and are confirmed on the FP64 true residual (A_REF/B_REF).
+template <typename T>
+__device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)[2]) {
+  constexpr bool MIXED = sizeof(T) == 4;
+  const bool eb = P.precond == PC_ELEMBLOCK;
+  switch (s.op) {
+    case OP_START:
+      s.op = OP_A_REF;
+      return;
+    case OP_A_REF:
+      s.rr_true = t[0];
+      if (!s.first && t[0] < P.tol2) s.pending = ST_CONV;
+      s.op = OP_B_REF;
+      return;
+    case OP_A_CG:
+      s.pcur ^= 1;
+      s.pAp = t[0];
+      if (fabs(t[0]) < 1e-30) {
+        if (MIXED) {
+          s.pending = ST_BREAK;
+          s.op = OP_B_REF;
+        } else {
+          s.status = ST_BREAK;
+        }
+      } else {
+        s.alpha = s.rz / t[0];
+        s.op = OP_B_CG;
+      }
+      return;
+    case OP_B_REF:
+      if (s.pending) {
+        s.status = s.pending;
+        return;
+      }
+      if (eb) {
+        s.cflag = CF_REFRESH;
+        s.op = OP_C;
+        return;
+      }
+      refresh_done(s, t[0]);
+      return;
+    case OP_B_CG: {
+      const double rr = t[1];
+      s.rr = rr;
+      s.it += 1;
+      int amode = OP_A_CG;
+      if (rr < P.tol2) {
+        if (!MIXED) { s.status = ST_CONV; return; }
+        amode = OP_A_REF;
+      } else if (s.it >= P.max_it) {
+        if (!MIXED) { s.status = ST_MAXIT; return; }
+        s.pending = ST_MAXIT;
+        amode = OP_A_REF;
+      } else if (MIXED && rr < P.delta2 * s.rmax2) {
+        amode = OP_A_REF;
+      }
+      if (eb) {
+        s.amode = amode;
+        s.cflag = CF_CG;
+        s.op = OP_C;
+        return;
+      }
+      s.beta = t[0] / (s.rz + 1e-60);
+      s.rz_prev = s.rz;
+      s.rz = t[0];
+      s.op = amode;
+      return;
+    }
+    default:   // OP_C
+      if (s.cflag == CF_REFRESH) {
+        refresh_done(s, t[0]);
+        return;
+      }
+      s.beta = t[0] / (s.rz + 1e-60);
+      s.rz_prev = s.rz;
+      s.rz = t[0];
+      s.op = s.amode;
+      return;
+  }
+}
+
+// Steps 1-3 of a slot kernel. Returns the advanced state (identical in every
+// block); block 0 publishes it for the next kernel.
+template <typename T>
+__device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, double* sh) {
+  const CST CgState* cgs = cp((const CgState*)P.cg);
+  CgState s;
+  {
+    const CST CgState& c = cgs[q & 1];
+    s.rz = c.rz; s.rz_prev = c.rz_prev; s.alpha = c.alpha; s.beta = c.beta;
+    s.pAp = c.pAp; s.rr = c.rr; s.rr_true = c.rr_true; s.rmax2 = c.rmax2;
+    s.it = c.it; s.status = c.status; s.op = c.op; s.pending = c.pending;
+    s.pcur = c.pcur; s.first = c.first; s.refines = c.refines; s.amode = c.amode;
+    s.cflag = c.cflag; s.pad0 = s.pad1 = s.pad2 = 0;
+  }
+  if (s.status == ST_RUN) {
+    double t[2] = {0.0, 0.0};
+    if (s.op != OP_START) {
+      const int nb = P.nb[G_PCG];
+      const GLB double* part = gp((const double*)P.part) + (long)((q - 1) & 1) * nb * 2;
+      for (int j = threadIdx.x; j < nb; j += TOPT_BLOCK) {
+        const d2v v = ((const GLB d2v*)part)[j];
+        t[0] += v.x;
+        t[1] += v.y;
+      }
+      block_sum<2>(t, sh);
+      if (threadIdx.x == 0) {
+        sh[0] = t[0];
+        sh[1] = t[1];
+      }
+      __syncthreads();
+      t[0] = sh[0];
+      t[1] = sh[1];
+    }
+    pcg_advance<T>(P, s, t);
+  }
+  if (lb == 0 && threadIdx.x == 0) P.cg[(q + 1) & 1] = s;
+  return s;
+}
+
+// Step 4 tail: this block's partials (two values) for kernel q + 1.
+__device__ __forceinline__ void slot_end(PD& P, int lb, int q, double v0, double v1, double* sh) {
+  double v[2] = {v0, v1};
+  block_sum<2>(v, sh);
+  if (threadIdx.x == 0) {
+    GLB d2v* part = (GLB d2v*)(gp(P.part) + (long)(q & 1) * P.nb[G_PCG] * 2);
+    part[lb] = d2v{v[0], v[1]};
+  }
+}
+
+// A_REF: r = f - A(u64 [+ x]) in FP64 (stored in T); partial |r|^2 (FP64).
+template <typename T>
+__device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id) {
   constexpr bool MIXED = sizeof(T) == 4;
   const GLB d2v* u = gp((const d2v*)P.u64);
   const GLB vec2<T>* x = gp((const vec2<T>*)P.x);
@@ -285,34 +433,17 @@ __device__ void refine_a(PD& P, int lb) {
   if (fb & 2u) ay = q[1][1].y;
   const d2v f = gp((const d2v*)P.f)[id.n];
   const double rx = f.x - ax, ry = f.y - ay;
-  double v[1] = {id.act ? rx * rx + ry * ry : 0.0};
-  if (id.act) {
-    vec2<T> r;
-    r.x = (T)rx;
-    r.y = (T)ry;
-    (gp((vec2<T>*)P.r))[id.n] = r;
-  }
-  store_partials<1>(v, P, lb, sh);
+  if (!id.act) return 0.0;
+  vec2<T> r;
+  r.x = (T)rx;
+  r.y = (T)ry;
+  gp((vec2<T>*)P.r)[id.n] = r;
+  return rx * rx + ry * ry;
 }
 
-// Slot A: p = z + beta p (p buffers ping-pong), Ap = K(s) p, partial p.Ap
-// (fem/fused_kernels.py:21-56; p update of fem/solver.py:151-154 fused in).
-template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_a(const ProbDev* tab, const int* bmap,
-                                                     ProbDev single) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  CgState* st = P.cg;
-  if (st->status != ST_RUN) return;
-  const int lb = b - P.b0[G_NODE];
-  if (st->mode == MODE_REFINE) {
-    refine_a<T>(P, lb);
-    return;
-  }
-  __shared__ double sh[16];
-  const NodeIdx id = node_of(P, lb);
-  const int pc = st->pcur;
-  const T beta = (T)st->beta;
+// A_CG at one node: p = z + beta p (ping-pong), Ap = K(s) p; returns p.Ap.
+template <typename T>
+__device__ __forceinline__ double slot_matvec(PD& P, const NodeIdx& id, int pc, T beta) {
   const GLB vec2<T>* pold = gp((const vec2<T>*)(pc ? P.p1 : P.p0));
   GLB vec2<T>* pnew = gp((vec2<T>*)(pc ? P.p0 : P.p1));
   const GLB vec2<T>* z = gp((const vec2<T>*)P.z);
@@ -337,43 +468,62 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_a(const ProbDev* tab, const i
   const unsigned fb = fixbits(P.fix, id.n);
   if (fb & 1u) ax = (double)q[1][1].x;
   if (fb & 2u) ay = (double)q[1][1].y;
-  double v[1] = {0.0};
-  if (id.act) {
-    pnew[id.n] = q[1][1];
-    vec2<T> o;
-    o.x = (T)ax;
-    o.y = (T)ay;
-    (gp((vec2<T>*)P.Ap))[id.n] = o;
-    v[0] = (double)q[1][1].x * (double)o.x + (double)q[1][1].y * (double)o.y;
-  }
-  store_partials<1>(v, P, lb, sh);
+  if (!id.act) return 0.0;
+  pnew[id.n] = q[1][1];
+  vec2<T> o;
+  o.x = (T)ax;
+  o.y = (T)ay;
+  gp((vec2<T>*)P.Ap)[id.n] = o;
+  return (double)q[1][1].x * (double)o.x + (double)q[1][1].y * (double)o.y;
 }
 
-// Slot B. CG mode: x += alpha p, r -= alpha Ap, z = M^-1 r, partial r.z and
-// r.r (fem/fused_kernels.py:110-166 update_and_precond_*; element-block
-// Jacobi defers z to slot C, as fem/fused_kernels.py:169-180 with
-// preconditioner.py do). Refine mode: u64 += x, x = 0 (mixed),
-// z = M^-1 r, partial r.z.
+// Slot A: A_REF (residual refresh) or A_CG (p update + matvec + p.Ap)
+// (fem/fused_kernels.py:21-56 with fem/solver.py:151-154 fused in).
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_b(const ProbDev* tab, const int* bmap,
-                                                     ProbDev single) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const int* bmap,
+                                                       ProbDev single, int q) {
+  __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   PD& P = get_prob<B>(tab, bmap, single, b);
-  CgState* st = P.cg;
-  if (st->status != ST_RUN) return;
-  constexpr bool MIXED = sizeof(T) == 4;
+  const int lb = b - P.b0[G_PCG];
+  const CgState s = slot_begin<T>(P, lb, q, sh);
+  if (s.status != ST_RUN) return;
+  double v = 0.0;
+  if (s.op == OP_A_REF) {
+#pragma unroll
+    for (int j = 0; j < Npt<T>::v; ++j) v += slot_refresh_residual<T>(P, slot_node<T>(P, lb, j));
+  } else {
+    const T beta = (T)s.beta;
+#pragma unroll
+    for (int j = 0; j < Npt<T>::v; ++j) v += slot_matvec<T>(P, slot_node<T>(P, lb, j), s.pcur, beta);
+  }
+  slot_end(P, lb, q, v, 0.0, sh);
+}
+
+// Slot B. B_CG: x += alpha p, r -= alpha Ap, z = M^-1 r, partials r.z, r.r
+// (fem/fused_kernels.py:110-166; element-block Jacobi leaves z to slot C as
+// fem/fused_kernels.py:169-180 + preconditioner.py do). B_REF: u64 += x,
+// x = 0 (mixed), z = M^-1 r, partial r.z.
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const int* bmap,
+                                                       ProbDev single, int q) {
   __shared__ double sh[16];
-  const int lb = b - P.b0[G_NODE];
-  const NodeIdx id = node_of(P, lb);
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  PD& P = get_prob<B>(tab, bmap, single, b);
+  const int lb = b - P.b0[G_PCG];
+  const CgState s = slot_begin<T>(P, lb, q, sh);
+  if (s.status != ST_RUN) return;
+  constexpr bool MIXED = sizeof(T) == 4;
+  const bool eb = P.precond == PC_ELEMBLOCK;
   GLB vec2<T>* r = gp((vec2<T>*)P.r);
   GLB vec2<T>* z = gp((vec2<T>*)P.z);
   GLB vec2<T>* x = gp((vec2<T>*)P.x);
-
-  if (st->mode == MODE_REFINE) {
-    double v[1] = {0.0};
-    vec2<T> zz;
-    if (P.precond == PC_ELEMBLOCK) zz = elemblock_node<T>(P, id, r);
-    if (id.act) {
+  double v0 = 0.0, v1 = 0.0;
+  if (s.op == OP_B_REF) {
+#pragma unroll
+    for (int j = 0; j < Npt<T>::v; ++j) {
+      const NodeIdx id = slot_node<T>(P, lb, j);
+      if (!id.act) continue;
       if (MIXED) {
         GLB d2v* u = gp((d2v*)P.u64);
         d2v uv = u[id.n];
@@ -386,159 +536,63 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_b(const ProbDev* tab, const i
         zero.y = T(0);
         x[id.n] = zero;
       }
-      const vec2<T> rv = r[id.n];
-      if (P.precond != PC_ELEMBLOCK) zz = precond_local<T>(P, id.n, rv);
-      z[id.n] = zz;
-      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+      if (!eb) {
+        const vec2<T> rv = r[id.n];
+        const vec2<T> zz = precond_local<T>(P, id.n, rv);
+        z[id.n] = zz;
+        v0 += (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+      }
     }
-    store_partials<1>(v, P, lb, sh);
-    return;
-  }
-
-  const T alpha = (T)st->alpha;
-  const int pc = st->pcur;
-  const GLB vec2<T>* p = gp((const vec2<T>*)(pc ? P.p1 : P.p0));
-  double v[2] = {0.0, 0.0};
-  if (id.act) {
-    const vec2<T> pv = p[id.n], ap = (gp((const vec2<T>*)P.Ap))[id.n];
-    vec2<T> xv = x[id.n], rv = r[id.n];
-    xv.x += alpha * pv.x;
-    xv.y += alpha * pv.y;
-    rv.x -= alpha * ap.x;
-    rv.y -= alpha * ap.y;
-    x[id.n] = xv;
-    r[id.n] = rv;
-    v[1] = (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
-    if (P.precond != PC_ELEMBLOCK) {
-      const vec2<T> zz = precond_local<T>(P, id.n, rv);
-      z[id.n] = zz;
-      v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+  } else {
+    const T alpha = (T)s.alpha;
+    const GLB vec2<T>* p = gp((const vec2<T>*)(s.pcur ? P.p1 : P.p0));
+    const GLB vec2<T>* Ap = gp((const vec2<T>*)P.Ap);
+#pragma unroll
+    for (int j = 0; j < Npt<T>::v; ++j) {
+      const NodeIdx id = slot_node<T>(P, lb, j);
+      if (!id.act) continue;
+      const vec2<T> pv = p[id.n], ap = Ap[id.n];
+      vec2<T> xv = x[id.n], rv = r[id.n];
+      xv.x += alpha * pv.x;
+      xv.y += alpha * pv.y;
+      rv.x -= alpha * ap.x;
+      rv.y -= alpha * ap.y;
+      x[id.n] = xv;
+      r[id.n] = rv;
+      v1 += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+      if (!eb) {
+        const vec2<T> zz = precond_local<T>(P, id.n, rv);
+        z[id.n] = zz;
+        v0 += (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+      }
     }
   }
-  store_partials<2>(v, P, lb, sh);
+  slot_end(P, lb, q, v0, v1, sh);
 }
 
 // Slot C (element-block Jacobi only): z = M^-1 r by gather, partial r.z
 // (fem/preconditioner.py:47-108 apply_and_rz_2d).
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_c(const ProbDev* tab, const int* bmap,
-                                                     ProbDev single) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int* bmap,
+                                                       ProbDev single, int q) {
+  __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   PD& P = get_prob<B>(tab, bmap, single, b);
-  CgState* st = P.cg;
-  if (st->status != ST_RUN || st->pad == 0) return;
-  __shared__ double sh[16];
-  const int lb = b - P.b0[G_NODE];
-  const NodeIdx id = node_of(P, lb);
+  const int lb = b - P.b0[G_PCG];
+  const CgState s = slot_begin<T>(P, lb, q, sh);
+  if (s.status != ST_RUN) return;
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
-  const vec2<T> zz = elemblock_node<T>(P, id, r);
-  double v[1] = {0.0};
-  if (id.act) {
-    (gp((vec2<T>*)P.z))[id.n] = zz;
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < Npt<T>::v; ++j) {
+    const NodeIdx id = slot_node<T>(P, lb, j);
+    const vec2<T> zz = elemblock_node<T>(P, id, r);
+    if (!id.act) continue;
+    gp((vec2<T>*)P.z)[id.n] = zz;
     const vec2<T> rv = r[id.n];
-    v[0] = (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+    v += (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
   }
-  store_partials<1>(v, P, lb, sh);
-}
-
-// Convergence bookkeeping after a CG update (fem/solver.py:146-156).
-// have_rz = false: r.z comes later from slot C (element-block Jacobi).
-template <typename T>
-__device__ __forceinline__ void cg_after_update(PD& P, CgState* st, double rr, double rzn,
-                                                bool have_rz) {
-  constexpr bool MIXED = sizeof(T) == 4;
-  st->rr = rr;
-  st->it += 1;
-  if (rr < P.tol2) {
-    if (!MIXED) { st->status = ST_CONV; return; }
-    st->mode = MODE_REFINE;     // confirm on the true FP64 residual
-  } else if (st->it >= P.max_it) {
-    if (!MIXED) { st->status = ST_MAXIT; return; }
-    st->pending = ST_MAXIT;
-    st->mode = MODE_REFINE;
-  } else if (MIXED && rr < P.delta2 * st->rmax2) {
-    st->mode = MODE_REFINE;     // reliable update
-  }
-  if (have_rz) {
-    st->beta = rzn / (st->rz + 1e-60);
-    st->rz_prev = st->rz;
-    st->rz = rzn;
-  } else {
-    st->pad = 1;                // slot C must run
-  }
-}
-
-// Finalize after slot `slot` (0 = A, 1 = B, 2 = C): one block per problem
-// sums that slot's block partials in a fixed order and applies the PCG
-// scalar logic (fem/solver.py:107-156):
-//   A/CG     alpha = rz / pAp, |pAp| < 1e-30 -> breakdown (solver.py:114-117)
-//   A/refine ||f - A u||^2 of the FP64 residual refresh
-//   B/CG     beta, convergence on the recursive residual (solver.py:146-151)
-//   B/refine beta after the refresh (first refresh: p = z)
-//   C        beta from the element-block r.z
-template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_fin(const ProbDev* tab, int nprob, ProbDev single,
-                                                       int slot) {
-  __shared__ double sh[16];
-  PD& P = B ? *(PD*)(tab + blockIdx.x) : *(PD*)&single;
-  CgState* st = P.cg;
-  if (st->status != ST_RUN) return;
-  if (slot == 2 && st->pad == 0) return;
-  const int mode = st->mode;
-  const int nv = (slot == 1 && mode == MODE_CG) ? 2 : 1;
-  const GLB double* part = gp((const double*)P.part);
-  const int nb = P.nb[G_NODE];
-  double acc[2] = {0.0, 0.0};
-  if (nv == 2) {
-    for (int j = threadIdx.x; j < nb; j += TOPT_BLOCK) {
-      acc[0] += part[2L * j];
-      acc[1] += part[2L * j + 1];
-    }
-  } else {
-    for (int j = threadIdx.x; j < nb; j += TOPT_BLOCK) acc[0] += part[j];
-  }
-  block_sum<2>(acc, sh);
-  if (threadIdx.x != 0) return;
-  constexpr bool MIXED = sizeof(T) == 4;
-  if (slot == 0) {
-    if (mode == MODE_REFINE) {
-      st->rr_true = acc[0];
-      if (!st->first && acc[0] < P.tol2) st->pending = ST_CONV;
-    } else {
-      st->pcur ^= 1;
-      st->pAp = acc[0];
-      if (fabs(acc[0]) < 1e-30) {
-        if (MIXED) {            // commit the correction before stopping
-          st->pending = ST_BREAK;
-          st->mode = MODE_REFINE;
-        } else {
-          st->status = ST_BREAK;
-        }
-      } else {
-        st->alpha = st->rz / acc[0];
-      }
-    }
-  } else if (slot == 1) {
-    if (mode == MODE_REFINE) {
-      if (st->pending) {
-        st->status = st->pending;
-      } else {
-        st->beta = st->first ? 0.0 : acc[0] / (st->rz_prev + 1e-60);
-        st->rz = acc[0];
-        st->first = 0;
-        st->mode = MODE_CG;
-        st->rmax2 = st->rr_true;
-        st->refines += 1;
-      }
-    } else {
-      cg_after_update<T>(P, st, acc[1], acc[0], P.precond != PC_ELEMBLOCK);
-    }
-  } else {
-    st->beta = acc[0] / (st->rz + 1e-60);
-    st->rz_prev = st->rz;
-    st->rz = acc[0];
-    st->pad = 0;
-  }
+  slot_end(P, lb, q, v, 0.0, sh);
 }
 
 // Compliance and sensitivities (fem/kernels.py:234-272), FP64:

@@ -28,11 +28,17 @@ enum FilterMode : int { F_DENSITY = 0, F_DCAVG = 1, F_SENS = 2, F_TRANSPOSE = 3,
 struct Tap { int dx, dy; double w; };  // filter stencil entry (reference table order)
 
 // PCG scalars of one problem (device resident, FP64).
+// Double-buffered: the PCG slot kernel with sequence number q reads
+// cg[q & 1] and its block 0 writes the advanced state to cg[(q + 1) & 1].
 struct CgState {
   double rz, rz_prev, alpha, beta, pAp, rr, rr_true, rmax2;
-  int it, status, mode, pending;
-  int pcur, first, refines, pad;
+  int it, status, op, pending;   // op: what the kernel that wrote this state executed
+  int pcur, first, refines, amode;
+  int cflag, pad0, pad1, pad2;
 };
+// PCG slot operations (the state machine of fem/solver.py:27-159 on the device).
+enum SlotOp : int { OP_START = 0, OP_A_CG = 1, OP_A_REF = 2, OP_B_CG = 3, OP_B_REF = 4, OP_C = 5 };
+enum CFlag : int { CF_CG = 0, CF_REFRESH = 1 };
 
 // OC / SIMP per-iteration scalars of one problem.
 struct OcState {
@@ -42,15 +48,18 @@ struct OcState {
 
 // Block grids a batch is launched over; each problem owns nb[g] consecutive
 // logical blocks starting at b0[g].
-enum Grid : int { G_NODE = 0, G_ELEM = 1, G_FILT = 2, G_OC = 3 };
+enum Grid : int { G_NODE = 0, G_ELEM = 1, G_FILT = 2, G_OC = 3, G_PCG = 4 };
+#define TOPT_NGRID 5
 #define TOPT_FILT_TY 64   // filter tile: 64 elements along y ...
 #define TOPT_FILT_TX 8    // ... by 8 along x, 2 per thread
 #define TOPT_OC_PER_THREAD 8
 #define TOPT_OC_MAXD 6    // bisection levels evaluated per OC pass (<= 6)
+// nodes per thread in the PCG slot kernels (FP32 / FP64 vectors)
+template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
 
 struct ProbDev {
   int nelx, nely;
-  int nb[4], b0[4];
+  int nb[TOPT_NGRID], b0[TOPT_NGRID];
   int precond, max_it, ntaps, R;
   double Emin, penal, tol2, delta2;   // tol2 = (tol |f|)^2; delta2 = reliable-update ratio^2
   double alpha_reg, volfrac, move, fnorm2;
@@ -70,7 +79,7 @@ struct ProbDev {
   // design fields (FP64, per element)
   double* rho; double* rhof; double* dc; double* dcf; const double* dv; double* Hs;
   const Tap* taps;
-  CgState* cg; double* part;   // node-grid tree partials: tree_parts(nb[G_NODE]) * 2
+  CgState* cg; double* part;   // cg[2]; partials: PCG slots 2 x nb[G_PCG] x 2, node-grid tree
   OcState* oc; double* opart;  // elem / OC-grid tree partials
   unsigned* ctr[4];            // tree counters per grid (G_NODE, G_ELEM, -, G_OC)
 };
