@@ -9,7 +9,8 @@ refresh, 2x2 block Jacobi. Inputs live in HBM before the timed region.
 Also reported (same JSON line): PCG iterations/s, matvec GDOF/s, the plain
 FP64 GPU path on the same iterations, the compliance agreement of the two
 paths on one density, the roofline of the dominant kernel (PCG slot A:
-p = z + beta p, Ap = K p, p.Ap) timed with HIP events, and the C
+p = z + beta p over the tile halo, Ap = K p, p.Ap) timed with HIP events,
+and the C
 restatement of the reference algorithm (FP32, Jacobi, as the reference) on
 the host cores as the CPU baseline.
 
@@ -28,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per slot-A launch at 1200x400 (FP32), measured with rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --probe`, each scaled
+# by the counter's reading for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
+# profiles/pcg_slots_bridge1200x400_mixed.md. Re-measure when the kernel changes.
+SLOT_A_TRAFFIC_BYTES = 17.93e6
 NELX, NELY = 1200, 400
 
 
@@ -82,9 +88,11 @@ def same_rho_compliance(rho, log):
 
 
 def kernel_roofline(plan, nn, ne, log, reps=200):
-    """Average duration of PCG slot A over back-to-back launches (HIP events)."""
+    """Average duration of PCG slot A (k_slot_a<float>) over back-to-back
+    launches, timed with HIP events on the launch stream."""
     ms = plan.time_slot(0, reps)
-    # FP32 slot A moves: read z, p_old (8 B/node each), s (4 B/elem); write p_new, Ap (8 B/node each)
+    # FP32 slot A per node: read z, p_old (8 B each), write p_new, Ap (8 B each);
+    # per element: read s_e (4 B)
     bytes_a = 32 * nn + 4 * ne
     gbs = bytes_a / (ms * 1e-3) / 1e9
     log(f"  slot A: {ms * 1e3:.2f} us/launch, {bytes_a / 1e6:.2f} MB -> {gbs:.0f} GB/s")
@@ -173,6 +181,14 @@ def main() -> int:
         matvec_gdofs(NELX, NELY, log, reps=50)
         matvec_gdofs(8192, 4096, log, reps=10)
         plan.close()
+        # calibration for HBM byte counters: a plain device copy of 256 MiB
+        # (reads and writes exactly 256 MiB each), three times
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[probe] copy calibration: 3 x 256 MiB device copy")
         return 0
 
     log(f"[bench] {NELX}x{NELY} bridge, warmup {args.warmup}, steps {args.steps}")
@@ -216,9 +232,9 @@ def main() -> int:
         "matvec_gbs_algorithmic": round(mv_gbs, 1),
         "matvec_8192x4096_gdofs": round(mvb_gdofs, 2),
         "matvec_8192x4096_gbs": round(mvb_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_cg_a<float> (PCG slot A)",
+        "roofline": {"bound": "hbm", "kernel": "k_slot_a<float> (PCG slot A)",
                      "achieved": round(gbs_a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs_a / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs_a / HBM_PEAK_GBS, 4), "traffic": SLOT_A_TRAFFIC_BYTES,
                      "bytes_per_launch": bytes_a, "us_per_launch": round(ms_a * 1e3, 3)},
         "cpu_baseline": cpu,
     }

@@ -126,9 +126,10 @@ int topt_plan_iteration(topt_plan* plan, int32_t sens_filter, topt_iter_stats* s
 int topt_plan_stats(topt_plan* plan, topt_iter_stats* stats, void* stream);
 
 /* Benchmark support: run `reps` back-to-back launches of PCG slot kernel A
- * (which must be 0: p = z + beta p, Ap = K p, p.Ap) on the plan's current
- * vectors and return the elapsed GPU time (hipEvents on the stream) in *ms.
- * Leaves every problem marked converged. No reference counterpart. Blocks. */
+ * (which must be 0: p = z + beta p over the tile halo, Ap = K p, p.Ap) on
+ * the plan's current vectors and return the elapsed GPU time (hipEvents on
+ * the stream) in *ms. Leaves every problem marked converged. No reference
+ * counterpart. Blocks. */
 int topt_plan_time_slot(topt_plan* plan, int32_t which, int32_t reps, float* ms, void* stream);
 
 /* ---- op-level kernels (single problem, explicit buffers) ---- */

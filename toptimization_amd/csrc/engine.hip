@@ -48,6 +48,7 @@ int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 struct Geom {
   int64_t nn, ne, words;
   int nb[TOPT_NGRID];
+  int ntree;   // blocks of the node-level reduction tree (node grid or tiles)
 };
 
 Geom geom(int nelx, int nely, int dtype = TOPT_F64) {
@@ -59,8 +60,9 @@ Geom geom(int nelx, int nely, int dtype = TOPT_F64) {
   g.nb[G_ELEM] = (int)cdiv(g.ne, TOPT_BLOCK);
   g.nb[G_FILT] = (int)(cdiv(nelx, TOPT_FILT_TX) * cdiv(nely, TOPT_FILT_TY));
   g.nb[G_OC] = (int)cdiv(g.ne, int64_t(TOPT_BLOCK) * TOPT_OC_PER_THREAD);
-  const int npt = dtype == TOPT_F32 ? Npt<float>::v : Npt<double>::v;
-  g.nb[G_PCG] = (int)cdiv(g.nn, int64_t(TOPT_BLOCK) * npt);
+  g.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float>(nelx, nely) : pcg_tiles<double>(nelx, nely));
+  g.ntree = (int)std::max<int64_t>(g.nb[G_NODE], std::max(pcg_tiles<float>(nelx, nely),
+                                                          pcg_tiles<double>(nelx, nely)));
   return g;
 }
 
@@ -86,12 +88,12 @@ int tap_radius(const std::vector<Tap>& taps) {
 
 // Counter words of the three reduction trees (node, element, OC grid).
 int64_t ctr_words(const Geom& g) {
-  return (tree_ctrs(g.nb[G_NODE]) + tree_ctrs(g.nb[G_ELEM]) + tree_ctrs(g.nb[G_OC]) + 1) * TOPT_CTR_STRIDE;
+  return (tree_ctrs(g.ntree) + tree_ctrs(g.nb[G_ELEM]) + tree_ctrs(g.nb[G_OC]) + 1) * TOPT_CTR_STRIDE;
 }
 
 void set_counters(ProbDev& P, unsigned* base, const Geom& g) {
   P.ctr[G_NODE] = base;
-  P.ctr[G_ELEM] = base + tree_ctrs(g.nb[G_NODE]) * TOPT_CTR_STRIDE;
+  P.ctr[G_ELEM] = base + tree_ctrs(g.ntree) * TOPT_CTR_STRIDE;
   P.ctr[G_OC] = P.ctr[G_ELEM] + tree_ctrs(g.nb[G_ELEM]) * TOPT_CTR_STRIDE;
   P.ctr[G_FILT] = nullptr;
 }
@@ -123,7 +125,8 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   sz[TOPT_BUF_TAPS] = std::max<int64_t>(ntaps, 1) * (int64_t)sizeof(Tap);
   sz[TOPT_BUF_CG] = 2 * sizeof(CgState);
   sz[TOPT_BUF_OC] = sizeof(OcState);
-  sz[TOPT_BUF_PART] = std::max<int64_t>(tree_parts(g.nb[G_NODE]) * 2, 4 * int64_t(g.nb[G_PCG])) * 8;
+  // PCG slot partials: two parities x nb x 2 values
+  sz[TOPT_BUF_PART] = std::max<int64_t>(tree_parts(g.ntree) * 2, 4 * int64_t(g.nb[G_PCG])) * 8;
   sz[TOPT_BUF_OPART] = std::max<int64_t>(tree_parts(g.nb[G_ELEM]), tree_parts(g.nb[G_OC]) * 32) * 8;
   sz[TOPT_BUF_CTR] = ctr_words(g) * 4;
   int64_t o = 0;
@@ -320,6 +323,7 @@ struct topt_plan {
   int* dmap[TOPT_NGRID] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int nblk[TOPT_NGRID] = {0, 0, 0, 0, 0};
   int64_t q = 0;          // PCG slot sequence number of the next launch
+  int64_t na = 0;         // slot-A launches so far in this solve (p buffer parity)
   int* dstat = nullptr;
   int* hstat[2] = {nullptr, nullptr};
   topt_iter_stats* dstats = nullptr;
@@ -344,10 +348,16 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
   const int nb = gr.nb;
   const int* map = gr.map;
   for (int k = 0; k < count; ++k) {
-    hipLaunchKernelGGL((k_slot_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, (int)(pl->q++ & 1));
-    hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, (int)(pl->q++ & 1));
+    // slot A number na reads p buffer (na & 1) and writes the other one
+    const int pa = (int)(pl->na & 1);
+    pl->na++;
+    hipLaunchKernelGGL((k_slot_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
+                       (int)(pl->q++ & 1), pa);
+    hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
+                       (int)(pl->q++ & 1), pa ^ 1);
     if (pl->any_elemblock)
-      hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, (int)(pl->q++ & 1));
+      hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
+                         (int)(pl->q++ & 1), pa ^ 1);
   }
   LAUNCH_TRY();
   return TOPT_OK;
@@ -417,6 +427,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
                                  : launch_solve_setup<double>(pl, warm, use_filtered, s);
   if (rc) return rc;
   pl->q = 0;
+  pl->na = 0;
   // slots issued so far; a solve needs at most max_cg CG slots plus refresh slots
   const int64_t slot_cap = 4 * int64_t(pl->max_cg) + 4096;
   int64_t issued = 0;
@@ -441,6 +452,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   HIP_TRY(hipEventSynchronize(pl->ev[wait ^ 1]));
   hipLaunchKernelGGL(k_cg_settle, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, (int)(pl->q & 1));
   pl->q = 0;
+  pl->na = 0;
   hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
@@ -472,7 +484,7 @@ OpScratch op_scratch(int nelx, int nely) {
   OpScratch o;
   int64_t off = 0;
   o.part_off = off;
-  off += align256(tree_parts(g.nb[G_NODE]) * 2 * 8);
+  off += align256(tree_parts(g.ntree) * 2 * 8);
   o.opart_off = off;
   off += align256(std::max<int64_t>(tree_parts(g.nb[G_ELEM]), tree_parts(g.nb[G_OC]) * 32) * 8);
   o.cg_off = off;
@@ -682,9 +694,9 @@ int topt_plan_time_slot(topt_plan* pl, int32_t which, int32_t reps, float* ms, v
   HIP_TRY(hipEventRecord(e0, s));
   for (int k = 0; k < reps; ++k) {
     if (pl->dtype == TOPT_F32)
-      hipLaunchKernelGGL((k_slot_a<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0);
+      hipLaunchKernelGGL((k_slot_a<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
     else
-      hipLaunchKernelGGL((k_slot_a<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0);
+      hipLaunchKernelGGL((k_slot_a<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
   }
   HIP_TRY(hipEventRecord(e1, s));
   hipLaunchKernelGGL(k_cg_finish, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
@@ -747,11 +759,13 @@ int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, cons
   }
   P.s = (void*)s;
   P.fix = fix;
+  // one block per tile; the tree over those blocks (sized by geom().ntree)
+  P.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float>(P.nelx, P.nely) : pcg_tiles<double>(P.nelx, P.nely));
   if (dtype == TOPT_F32)
-    hipLaunchKernelGGL((k_matvec<float>), dim3(P.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, P, (const float*)p,
+    hipLaunchKernelGGL((k_matvec<float>), dim3(P.nb[G_PCG]), dim3(TOPT_BLOCK), 0, st, P, (const float*)p,
                        (float*)Ap, pAp);
   else
-    hipLaunchKernelGGL((k_matvec<double>), dim3(P.nb[G_NODE]), dim3(TOPT_BLOCK), 0, st, P, (const double*)p,
+    hipLaunchKernelGGL((k_matvec<double>), dim3(P.nb[G_PCG]), dim3(TOPT_BLOCK), 0, st, P, (const double*)p,
                        (double*)Ap, pAp);
   LAUNCH_TRY();
   return TOPT_OK;

@@ -6,8 +6,8 @@
 // Reference counterparts (src/toptimization/):
 //   fem/kernels.py:143-176 compute_matvec_2d      -> q4 gather (no atomics)
 //   fem/kernels.py:76-101  compute_diagonal_2d    -> k_precond (Jacobi)
-//   fem/preconditioner.py:47-108 apply_and_rz_2d  -> k_cg_c / refine B (element-block)
-//   fem/fused_kernels.py:21-56,110-187            -> k_cg_a / k_cg_b
+//   fem/preconditioner.py:47-108 apply_and_rz_2d  -> k_slot_c (element-block)
+//   fem/fused_kernels.py:21-56,110-187            -> k_slot_a / k_slot_b
 //   fem/solver.py:27-167 solve                    -> slot sequence driven by engine.hip (host)
 //   fem/kernels.py:234-272 compute_sensitivity_2d -> k_sens
 #include "topt_dev.h"
@@ -32,16 +32,6 @@ __device__ __forceinline__ NodeIdx node_of(PD& P, int lb) {
   id.ix = (int)(id.n / nny);
   id.iy = (int)(id.n - (long)id.ix * nny);
   return id;
-}
-
-// Gather a T2 vector over the 3x3 node neighbourhood.
-template <typename T>
-__device__ __forceinline__ void gather9(const GLB vec2<T>* v, const NodeIdx& id,
-                                        PD& P, vec2<T> (&q)[3][3]) {
-#pragma unroll
-  for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy) q[dx][dy] = v[nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely)];
 }
 
 // Element-block Jacobi (reference fem/preconditioner.py:47-108) in gather
@@ -88,44 +78,6 @@ __device__ __forceinline__ vec2<T> elemblock_node(PD& P, const NodeIdx& id,
   z.x = (fb & 1u) ? T(0) : zx;
   z.y = (fb & 2u) ? T(0) : zy;
   return z;
-}
-
-// z = M^-1 r at one node for the elementwise preconditioners.
-template <typename T>
-__device__ __forceinline__ vec2<T> precond_local(PD& P, long n, vec2<T> r) {
-  vec2<T> z;
-  if (P.precond == PC_JACOBI) {
-    const vec2<T> d = (gp((const vec2<T>*)P.pre))[n];
-    z.x = r.x * d.x;
-    z.y = r.y * d.y;
-  } else if (P.precond == PC_BLOCK2) {
-    const vec2<T> d = (gp((const vec2<T>*)P.pre))[n];
-    const T o = (gp((const T*)P.pre2))[n];
-    z.x = d.x * r.x + o * r.y;
-    z.y = o * r.x + d.y * r.y;
-  } else {
-    z = r;
-  }
-  return z;
-}
-
-// K(s) q at one node with FP64 Ke and FP64 accumulation; q and s in T.
-template <typename T>
-__device__ __forceinline__ void stencil_fp64(PD& P, const NodeIdx& id,
-                                             const vec2<T> (&q)[3][3], double& ax, double& ay) {
-  T sc[2][2];
-  load_scales<T>((const T*)P.s, id.ix, id.iy, P.nelx, P.nely, sc);
-  double scd[2][2];
-  d2v qd[3][3];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) scd[a][b] = (double)sc[a][b];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) qd[a][b] = d2v{(double)q[a][b].x, (double)q[a][b].y};
-  q4_gather<double>(P.Ke64, scd, qd, ax, ay);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -245,19 +197,165 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
 //      stores it in cg[(q+1) & 1],
 //   4. executes the operation that decision selects and writes its own block
 //      partials to part[q & 1].
-// No atomics, no grid barrier, one launch per slot.
+// No atomics, no grid barrier, one launch per slot. Loads that do not
+// depend on the decision (the partials, z, r, Ap, x, preconditioner data,
+// element scales, fixed bits) are issued before step 1 completes, so a slot
+// pays about two memory latencies: one for those loads and the state, one
+// for the p buffer the state selects.
+//
+// Work decomposition: one block = one tile of TY = 64 node rows (one per
+// lane, contiguous in memory) by TX = 4 Npt node columns (wave w owns
+// columns w, w + 4, ...). The stencil kernels stage the tile's node halo
+// ((TX + 2) x 66 values) and element halo ((TX + 1) x 65 scales) in LDS:
+// every value is fetched from L2/HBM once per tile instead of once per
+// neighbour, and all of a thread's fetches are in flight together.
 
 template <typename T>
-__device__ __forceinline__ NodeIdx slot_node(PD& P, int lb, int j) {
+struct Tile {
+  static constexpr int NPT = Npt<T>::v;
+  static constexpr int NW = TOPT_BLOCK / TOPT_WAVE;        // waves per block
+  static constexpr int TY = TOPT_WAVE, TX = NW * NPT;      // tile: rows x columns
+  static constexpr int HY = TY + 2, HN = (TX + 2) * HY;    // node halo
+  static constexpr int EY = TY + 1, EN = (TX + 1) * EY;    // element halo
+  static constexpr int HL = (HN + TOPT_BLOCK - 1) / TOPT_BLOCK;
+  static constexpr int EL = (EN + TOPT_BLOCK - 1) / TOPT_BLOCK;
+};
+
+// Tiles of a grid (the slot kernels' block count per problem).
+template <typename T>
+__host__ __device__ inline long pcg_tiles(int nelx, int nely) {
+  return (long)((nely + TOPT_WAVE) / TOPT_WAVE) * ((nelx + Tile<T>::TX) / Tile<T>::TX);
+}
+
+struct TileOrg {
+  int x0, y0;
+};
+
+// Tiles are numbered along y first (the memory order of the nodes).
+template <typename T>
+__device__ __forceinline__ TileOrg tile_org(PD& P, int lb) {
+  const int nty = (P.nely + TOPT_WAVE) / TOPT_WAVE;
+  const int tx = lb / nty;
+  return TileOrg{tx * Tile<T>::TX, (lb - tx * nty) * TOPT_WAVE};
+}
+
+// Tile node j of this thread; nodes outside the grid are clamped onto it and
+// marked inactive (they store nothing and add nothing).
+template <typename T>
+__device__ __forceinline__ NodeIdx tile_node(PD& P, TileOrg o, int j) {
   NodeIdx id;
-  const int nny = P.nely + 1;
-  const long nn = (long)(P.nelx + 1) * nny;
-  const long n = ((long)lb * Npt<T>::v + j) * TOPT_BLOCK + threadIdx.x;
-  id.act = n < nn;
-  id.n = id.act ? n : nn - 1;
-  id.ix = (int)(id.n / nny);
-  id.iy = (int)(id.n - (long)id.ix * nny);
+  const int ix = o.x0 + (int)(threadIdx.x >> 6) + Tile<T>::NW * j;
+  const int iy = o.y0 + (int)(threadIdx.x & 63);
+  id.act = ix <= P.nelx && iy <= P.nely;
+  id.ix = ix <= P.nelx ? ix : P.nelx;
+  id.iy = iy <= P.nely ? iy : P.nely;
+  id.n = (long)id.ix * (P.nely + 1) + id.iy;
   return id;
+}
+
+// Halo entry i (i < HN) -> clamped global node (as nbr_node).
+template <typename T>
+__device__ __forceinline__ long halo_node(PD& P, TileOrg o, int i) {
+  const int cx = i / Tile<T>::HY, cy = i - cx * Tile<T>::HY;
+  int jx = o.x0 - 1 + cx, jy = o.y0 - 1 + cy;
+  jx = jx < 0 ? 0 : (jx > P.nelx ? P.nelx : jx);
+  jy = jy < 0 ? 0 : (jy > P.nely ? P.nely : jy);
+  return (long)jx * (P.nely + 1) + jy;
+}
+
+// Element halo entry i (i < EN) -> scale, zero outside the grid (as load_scales).
+template <typename T>
+__device__ __forceinline__ T halo_scale(PD& P, const GLB T* s, TileOrg o, int i) {
+  const int cx = i / Tile<T>::EY, cy = i - cx * Tile<T>::EY;
+  const int ex = o.x0 - 1 + cx, ey = o.y0 - 1 + cy;
+  const bool ok = (ex >= 0) & (ex < P.nelx) & (ey >= 0) & (ey < P.nely);
+  const int kx = ex < 0 ? 0 : (ex >= P.nelx ? P.nelx - 1 : ex);
+  const int ky = ey < 0 ? 0 : (ey >= P.nely ? P.nely - 1 : ey);
+  const T v = s[(long)kx * P.nely + ky];
+  return ok ? v : T(0);
+}
+
+// Element scales of the tile, fetched into registers (entry threadIdx + 256 k).
+template <typename T>
+__device__ __forceinline__ void tile_scales_fetch(PD& P, TileOrg o, T (&hs)[Tile<T>::EL]) {
+  const GLB T* s = gp((const T*)P.s);
+#pragma unroll
+  for (int k = 0; k < Tile<T>::EL; ++k) {
+    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
+    hs[k] = halo_scale<T>(P, s, o, i < Tile<T>::EN ? i : Tile<T>::EN - 1);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void tile_scales_store(const T (&hs)[Tile<T>::EL], double* esc) {
+#pragma unroll
+  for (int k = 0; k < Tile<T>::EL; ++k) {
+    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
+    if (k + 1 < Tile<T>::EL || i < Tile<T>::EN) esc[i] = (double)hs[k];
+  }
+}
+
+// K(s) q at the thread's NPT tile nodes from the LDS halo (values and
+// scales held in FP64): FP64 Ke, FP64 accumulation. Element-outer order: the
+// two Ke rows of our local node in element (a, b) (16 scalars in SGPRs) serve
+// all NPT nodes before the next element's rows are loaded.
+template <typename T>
+__device__ __forceinline__ void tile_stencil(PD& P, const d2v* hal, const double* esc,
+                                             double (&ax)[Tile<T>::NPT], double (&ay)[Tile<T>::NPT]) {
+  using Tl = Tile<T>;
+  constexpr int CX[4] = {0, 1, 1, 0};
+  constexpr int CY[4] = {0, 0, 1, 1};
+  const int w = (int)(threadIdx.x >> 6), row = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int j = 0; j < Tl::NPT; ++j) ax[j] = ay[j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      // local index of our node inside element (ix - 1 + a, iy - 1 + b)
+      const int me = (a == 1) ? ((b == 1) ? 0 : 3) : ((b == 1) ? 1 : 2);
+      const auto K0 = P.Ke64 + (2 * me) * 8;
+      const auto K1 = K0 + 8;
+#pragma unroll
+      for (int j = 0; j < Tl::NPT; ++j) {
+        const int c = w + Tl::NW * j;
+        double tx = 0.0, ty = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const d2v v = hal[(c + a + CX[k]) * Tl::HY + row + b + CY[k]];
+          // one FMA per product (a*x + b*y added to t would cost a mul, an fma and an add)
+          tx = fma(K0[2 * k], v.x, tx);
+          tx = fma(K0[2 * k + 1], v.y, tx);
+          ty = fma(K1[2 * k], v.x, ty);
+          ty = fma(K1[2 * k + 1], v.y, ty);
+        }
+        const double sc = esc[(c + a) * Tl::EY + row + b];
+        ax[j] = fma(sc, tx, ax[j]);
+        ay[j] = fma(sc, ty, ay[j]);
+      }
+    }
+}
+
+// The node's own halo value (centre of its 3x3 neighbourhood).
+template <typename T>
+__device__ __forceinline__ d2v tile_own(const d2v* hal, int j) {
+  using Tl = Tile<T>;
+  const int c = (int)(threadIdx.x >> 6) + Tl::NW * j, row = (int)(threadIdx.x & 63);
+  return hal[(c + 1) * Tl::HY + row + 1];
+}
+
+// Previous slot's partials, fetched into registers before the state is read.
+#define TOPT_PART_PT 8
+struct PrevParts {
+  d2v v[TOPT_PART_PT];
+};
+__device__ __forceinline__ void parts_fetch(PD& P, int q, PrevParts& pp) {
+  const int nb = P.nb[G_PCG];
+  const GLB d2v* part = (const GLB d2v*)(gp((const double*)P.part) + (long)((q - 1) & 1) * nb * 2);
+#pragma unroll
+  for (int k = 0; k < TOPT_PART_PT; ++k) {
+    const int j = (int)threadIdx.x + k * TOPT_BLOCK;
+    pp.v[k] = part[j < nb ? j : 0];   // entries j >= nb are masked when summed
+  }
 }
 
 __device__ __forceinline__ void refresh_done(CgState& s, double rzn) {
@@ -289,7 +387,6 @@ __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)
       s.op = OP_B_REF;
       return;
     case OP_A_CG:
-      s.pcur ^= 1;
       s.pAp = t[0];
       if (fabs(t[0]) < 1e-30) {
         if (MIXED) {
@@ -356,9 +453,13 @@ __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)
 }
 
 // Steps 1-3 of a slot kernel. Returns the advanced state (identical in every
-// block); block 0 publishes it for the next kernel.
+// block); block 0 publishes it for the next kernel. The summation order of
+// the partials is fixed: thread t adds j = t, t + 256, ... in order, then the
+// block sum in lane and wave order. The sum is formed unconditionally (it is
+// unused after OP_START or when the problem has stopped) so that the partial
+// loads stay ahead of the state load instead of being sunk behind it.
 template <typename T>
-__device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, double* sh) {
+__device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevParts& pp, double* sh) {
   const CST CgState* cgs = cp((const CgState*)P.cg);
   CgState s;
   {
@@ -366,30 +467,37 @@ __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, double* sh) 
     s.rz = c.rz; s.rz_prev = c.rz_prev; s.alpha = c.alpha; s.beta = c.beta;
     s.pAp = c.pAp; s.rr = c.rr; s.rr_true = c.rr_true; s.rmax2 = c.rmax2;
     s.it = c.it; s.status = c.status; s.op = c.op; s.pending = c.pending;
-    s.pcur = c.pcur; s.first = c.first; s.refines = c.refines; s.amode = c.amode;
+    s.pad3 = 0; s.first = c.first; s.refines = c.refines; s.amode = c.amode;
     s.cflag = c.cflag; s.pad0 = s.pad1 = s.pad2 = 0;
   }
-  if (s.status == ST_RUN) {
-    double t[2] = {0.0, 0.0};
-    if (s.op != OP_START) {
-      const int nb = P.nb[G_PCG];
-      const GLB double* part = gp((const double*)P.part) + (long)((q - 1) & 1) * nb * 2;
-      for (int j = threadIdx.x; j < nb; j += TOPT_BLOCK) {
-        const d2v v = ((const GLB d2v*)part)[j];
-        t[0] += v.x;
-        t[1] += v.y;
-      }
-      block_sum<2>(t, sh);
-      if (threadIdx.x == 0) {
-        sh[0] = t[0];
-        sh[1] = t[1];
-      }
-      __syncthreads();
-      t[0] = sh[0];
-      t[1] = sh[1];
-    }
-    pcg_advance<T>(P, s, t);
+  // keep every load issued so far (partials, state, the caller's
+  // state-independent data) ahead of the first wait
+  __builtin_amdgcn_sched_barrier(0);
+  const int nb = P.nb[G_PCG];
+  double t[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < TOPT_PART_PT; ++k) {
+    const bool in = (int)threadIdx.x + k * TOPT_BLOCK < nb;
+    t[0] += in ? pp.v[k].x : 0.0;
+    t[1] += in ? pp.v[k].y : 0.0;
   }
+  if (nb > TOPT_PART_PT * TOPT_BLOCK) {   // very large grids: the rest in a loop
+    const GLB d2v* part = (const GLB d2v*)(gp((const double*)P.part) + (long)((q - 1) & 1) * nb * 2);
+    for (int j = (int)threadIdx.x + TOPT_PART_PT * TOPT_BLOCK; j < nb; j += TOPT_BLOCK) {
+      const d2v v = part[j];
+      t[0] += v.x;
+      t[1] += v.y;
+    }
+  }
+  block_sum<2>(t, sh);
+  if (threadIdx.x == 0) {
+    sh[0] = t[0];
+    sh[1] = t[1];
+  }
+  __syncthreads();
+  t[0] = sh[0];
+  t[1] = sh[1];
+  if (s.status == ST_RUN) pcg_advance<T>(P, s, t);
   if (lb == 0 && threadIdx.x == 0) P.cg[(q + 1) & 1] = s;
   return s;
 }
@@ -405,6 +513,7 @@ __device__ __forceinline__ void slot_end(PD& P, int lb, int q, double v0, double
 }
 
 // A_REF: r = f - A(u64 [+ x]) in FP64 (stored in T); partial |r|^2 (FP64).
+// Rare (a few times per solve): direct loads.
 template <typename T>
 __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id) {
   constexpr bool MIXED = sizeof(T) == 4;
@@ -441,63 +550,106 @@ __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id
   return rx * rx + ry * ry;
 }
 
-// A_CG at one node: p = z + beta p (ping-pong), Ap = K(s) p; returns p.Ap.
-template <typename T>
-__device__ __forceinline__ double slot_matvec(PD& P, const NodeIdx& id, int pc, T beta) {
-  const GLB vec2<T>* pold = gp((const vec2<T>*)(pc ? P.p1 : P.p0));
-  GLB vec2<T>* pnew = gp((vec2<T>*)(pc ? P.p0 : P.p1));
-  const GLB vec2<T>* z = gp((const vec2<T>*)P.z);
-  vec2<T> q[3][3];
-#pragma unroll
-  for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
-      const vec2<T> zv = z[m], pv = pold[m];
-      vec2<T> v;
-      v.x = zv.x + beta * pv.x;
-      v.y = zv.y + beta * pv.y;
-      q[dx][dy] = v;
-    }
-  // FP32 storage, FP64 registers: the stencil of the stored p_new is summed
-  // with the FP64 Ke so the inner operator is K(s32) up to the final
-  // rounding of Ap (an FP32 Ke and FP32 sums break the rigid-body
-  // cancellation and stall the FP64 residual refresh).
-  double ax, ay;
-  stencil_fp64<T>(P, id, q, ax, ay);
-  const unsigned fb = fixbits(P.fix, id.n);
-  if (fb & 1u) ax = (double)q[1][1].x;
-  if (fb & 2u) ay = (double)q[1][1].y;
-  if (!id.act) return 0.0;
-  pnew[id.n] = q[1][1];
-  vec2<T> o;
-  o.x = (T)ax;
-  o.y = (T)ay;
-  gp((vec2<T>*)P.Ap)[id.n] = o;
-  return (double)q[1][1].x * (double)o.x + (double)q[1][1].y * (double)o.y;
-}
-
-// Slot A: A_REF (residual refresh) or A_CG (p update + matvec + p.Ap)
-// (fem/fused_kernels.py:21-56 with fem/solver.py:151-154 fused in).
+// Slot A: A_CG = p = z + beta p (ping-pong buffers, recomputed over the
+// halo), Ap = K(s) p, partial p.Ap (fem/fused_kernels.py:21-56 with
+// fem/solver.py:151-154 fused in); A_REF = residual refresh, and p carried
+// over unchanged into the other buffer. Every slot-A launch of a solve thus
+// swaps the p buffers, so the host passes the parity par (p_old = par ? p1 : p0)
+// and the p halo is fetched together with z, before the state is known.
+// FP32 storage, FP64 registers: the stencil of the stored p is summed with
+// the FP64 Ke, so the inner operator is K(s32) up to the final rounding of
+// Ap (an FP32 Ke and FP32 sums break the rigid-body cancellation and stall
+// the FP64 residual refresh).
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const int* bmap,
-                                                       ProbDev single, int q) {
+                                                       ProbDev single, int q, int par) {
+  using Tl = Tile<T>;
+  __shared__ d2v hal[Tl::HN];
+  __shared__ double esc[Tl::EN];
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_PCG];
-  const CgState s = slot_begin<T>(P, lb, q, sh);
-  if (s.status != ST_RUN) return;
+  const TileOrg o = tile_org<T>(P, lb);
+  const GLB vec2<T>* pold = gp((const vec2<T>*)(par ? P.p1 : P.p0));
+  GLB vec2<T>* pnew = gp((vec2<T>*)(par ? P.p0 : P.p1));
+  // state-independent loads first
+  PrevParts pp;
+  parts_fetch(P, q, pp);
+  const GLB vec2<T>* z = gp((const vec2<T>*)P.z);
+  vec2<T> hz[Tl::HL], hp[Tl::HL];
+#pragma unroll
+  for (int k = 0; k < Tl::HL; ++k) {
+    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
+    const long m = halo_node<T>(P, o, i < Tl::HN ? i : Tl::HN - 1);
+    hz[k] = z[m];
+    hp[k] = pold[m];
+  }
+  T hs[Tl::EL];
+  tile_scales_fetch<T>(P, o, hs);
+  NodeIdx ids[Tl::NPT];
+  unsigned fb[Tl::NPT];
+#pragma unroll
+  for (int j = 0; j < Tl::NPT; ++j) {
+    ids[j] = tile_node<T>(P, o, j);
+    fb[j] = fixbits(P.fix, ids[j].n);
+  }
+  const CgState st = slot_begin<T>(P, lb, q, pp, sh);
+  if (st.status != ST_RUN) return;
   double v = 0.0;
-  if (s.op == OP_A_REF) {
+  if (st.op == OP_A_REF) {
 #pragma unroll
-    for (int j = 0; j < Npt<T>::v; ++j) v += slot_refresh_residual<T>(P, slot_node<T>(P, lb, j));
+    for (int j = 0; j < Tl::NPT; ++j) {
+      v += slot_refresh_residual<T>(P, ids[j]);
+      if (ids[j].act) pnew[ids[j].n] = pold[ids[j].n];
+    }
   } else {
-    const T beta = (T)s.beta;
+    const T beta = (T)st.beta;
 #pragma unroll
-    for (int j = 0; j < Npt<T>::v; ++j) v += slot_matvec<T>(P, slot_node<T>(P, lb, j), s.pcur, beta);
+    for (int k = 0; k < Tl::HL; ++k) {
+      const int i = (int)threadIdx.x + k * TOPT_BLOCK;
+      vec2<T> w;
+      w.x = hz[k].x + beta * hp[k].x;
+      w.y = hz[k].y + beta * hp[k].y;
+      if (k + 1 < Tl::HL || i < Tl::HN) hal[i] = d2v{(double)w.x, (double)w.y};
+    }
+    tile_scales_store<T>(hs, esc);
+    __syncthreads();
+    double ax[Tl::NPT], ay[Tl::NPT];
+    tile_stencil<T>(P, hal, esc, ax, ay);
+    GLB vec2<T>* Ap = gp((vec2<T>*)P.Ap);
+#pragma unroll
+    for (int j = 0; j < Tl::NPT; ++j) {
+      const d2v own = tile_own<T>(hal, j);
+      if (fb[j] & 1u) ax[j] = own.x;
+      if (fb[j] & 2u) ay[j] = own.y;
+      if (ids[j].act) {
+        pnew[ids[j].n] = vec2<T>{(T)own.x, (T)own.y};
+        vec2<T> ov;
+        ov.x = (T)ax[j];
+        ov.y = (T)ay[j];
+        Ap[ids[j].n] = ov;
+        v += own.x * (double)ov.x + own.y * (double)ov.y;
+      }
+    }
   }
   slot_end(P, lb, q, v, 0.0, sh);
+}
+
+// z = M^-1 r at one node from preloaded preconditioner data.
+template <typename T>
+__device__ __forceinline__ vec2<T> precond_apply(int pc, vec2<T> d, T o, vec2<T> r) {
+  vec2<T> z;
+  if (pc == PC_JACOBI) {
+    z.x = r.x * d.x;
+    z.y = r.y * d.y;
+  } else if (pc == PC_BLOCK2) {
+    z.x = d.x * r.x + o * r.y;
+    z.y = o * r.x + d.y * r.y;
+  } else {
+    z = r;
+  }
+  return z;
 }
 
 // Slot B. B_CG: x += alpha p, r -= alpha Ap, z = M^-1 r, partials r.z, r.r
@@ -506,64 +658,81 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const
 // x = 0 (mixed), z = M^-1 r, partial r.z.
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const int* bmap,
-                                                       ProbDev single, int q) {
+                                                       ProbDev single, int q, int par) {
+  using Tl = Tile<T>;
   __shared__ double sh[16];
+  constexpr bool MIXED = sizeof(T) == 4;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_PCG];
-  const CgState s = slot_begin<T>(P, lb, q, sh);
-  if (s.status != ST_RUN) return;
-  constexpr bool MIXED = sizeof(T) == 4;
-  const bool eb = P.precond == PC_ELEMBLOCK;
+  const TileOrg o = tile_org<T>(P, lb);
+  const int pc = P.precond;
   GLB vec2<T>* r = gp((vec2<T>*)P.r);
   GLB vec2<T>* z = gp((vec2<T>*)P.z);
   GLB vec2<T>* x = gp((vec2<T>*)P.x);
-  double v0 = 0.0, v1 = 0.0;
-  if (s.op == OP_B_REF) {
+  const GLB vec2<T>* Ap = gp((const vec2<T>*)P.Ap);
+  // state-independent loads first
+  PrevParts pp;
+  parts_fetch(P, q, pp);
+  // p as written by the preceding slot A (par = parity of that buffer)
+  const GLB vec2<T>* p = gp((const vec2<T>*)(par ? P.p1 : P.p0));
+  NodeIdx ids[Tl::NPT];
+  vec2<T> rv[Tl::NPT], xv[Tl::NPT], av[Tl::NPT], dv[Tl::NPT], pv[Tl::NPT];
+  T ov[Tl::NPT];
 #pragma unroll
-    for (int j = 0; j < Npt<T>::v; ++j) {
-      const NodeIdx id = slot_node<T>(P, lb, j);
-      if (!id.act) continue;
+  for (int j = 0; j < Tl::NPT; ++j) {
+    ids[j] = tile_node<T>(P, o, j);
+    const long n = ids[j].n;
+    pv[j] = p[n];
+    rv[j] = r[n];
+    xv[j] = x[n];
+    av[j] = Ap[n];
+    dv[j] = vec2<T>{T(0), T(0)};
+    ov[j] = T(0);
+    if (pc == PC_JACOBI || pc == PC_BLOCK2) dv[j] = (gp((const vec2<T>*)P.pre))[n];
+    if (pc == PC_BLOCK2) ov[j] = (gp((const T*)P.pre2))[n];
+  }
+  const CgState st = slot_begin<T>(P, lb, q, pp, sh);
+  if (st.status != ST_RUN) return;
+  const bool eb = pc == PC_ELEMBLOCK;
+  double v0 = 0.0, v1 = 0.0;
+  if (st.op == OP_B_REF) {
+#pragma unroll
+    for (int j = 0; j < Tl::NPT; ++j) {
+      if (!ids[j].act) continue;
+      const long n = ids[j].n;
       if (MIXED) {
         GLB d2v* u = gp((d2v*)P.u64);
-        d2v uv = u[id.n];
-        const vec2<T> c = x[id.n];
-        uv.x += (double)c.x;
-        uv.y += (double)c.y;
-        u[id.n] = uv;
-        vec2<T> zero;
-        zero.x = T(0);
-        zero.y = T(0);
-        x[id.n] = zero;
+        d2v uv = u[n];
+        uv.x += (double)xv[j].x;
+        uv.y += (double)xv[j].y;
+        u[n] = uv;
+        x[n] = vec2<T>{T(0), T(0)};
       }
       if (!eb) {
-        const vec2<T> rv = r[id.n];
-        const vec2<T> zz = precond_local<T>(P, id.n, rv);
-        z[id.n] = zz;
-        v0 += (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+        const vec2<T> zz = precond_apply<T>(pc, dv[j], ov[j], rv[j]);
+        z[n] = zz;
+        v0 += (double)rv[j].x * (double)zz.x + (double)rv[j].y * (double)zz.y;
       }
     }
   } else {
-    const T alpha = (T)s.alpha;
-    const GLB vec2<T>* p = gp((const vec2<T>*)(s.pcur ? P.p1 : P.p0));
-    const GLB vec2<T>* Ap = gp((const vec2<T>*)P.Ap);
+    const T alpha = (T)st.alpha;
 #pragma unroll
-    for (int j = 0; j < Npt<T>::v; ++j) {
-      const NodeIdx id = slot_node<T>(P, lb, j);
-      if (!id.act) continue;
-      const vec2<T> pv = p[id.n], ap = Ap[id.n];
-      vec2<T> xv = x[id.n], rv = r[id.n];
-      xv.x += alpha * pv.x;
-      xv.y += alpha * pv.y;
-      rv.x -= alpha * ap.x;
-      rv.y -= alpha * ap.y;
-      x[id.n] = xv;
-      r[id.n] = rv;
-      v1 += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+    for (int j = 0; j < Tl::NPT; ++j) {
+      if (!ids[j].act) continue;
+      const long n = ids[j].n;
+      vec2<T> xn = xv[j], rn = rv[j];
+      xn.x += alpha * pv[j].x;
+      xn.y += alpha * pv[j].y;
+      rn.x -= alpha * av[j].x;
+      rn.y -= alpha * av[j].y;
+      x[n] = xn;
+      r[n] = rn;
+      v1 += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
       if (!eb) {
-        const vec2<T> zz = precond_local<T>(P, id.n, rv);
-        z[id.n] = zz;
-        v0 += (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
+        const vec2<T> zz = precond_apply<T>(pc, dv[j], ov[j], rn);
+        z[n] = zz;
+        v0 += (double)rn.x * (double)zz.x + (double)rn.y * (double)zz.y;
       }
     }
   }
@@ -574,18 +743,22 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
 // (fem/preconditioner.py:47-108 apply_and_rz_2d).
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int* bmap,
-                                                       ProbDev single, int q) {
+                                                       ProbDev single, int q, int par) {
+  using Tl = Tile<T>;
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_PCG];
-  const CgState s = slot_begin<T>(P, lb, q, sh);
-  if (s.status != ST_RUN) return;
+  const TileOrg o = tile_org<T>(P, lb);
+  PrevParts pp;
+  parts_fetch(P, q, pp);
+  const CgState st = slot_begin<T>(P, lb, q, pp, sh);
+  if (st.status != ST_RUN) return;
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
   double v = 0.0;
 #pragma unroll
-  for (int j = 0; j < Npt<T>::v; ++j) {
-    const NodeIdx id = slot_node<T>(P, lb, j);
+  for (int j = 0; j < Tl::NPT; ++j) {
+    const NodeIdx id = tile_node<T>(P, o, j);
     const vec2<T> zz = elemblock_node<T>(P, id, r);
     if (!id.act) continue;
     gp((vec2<T>*)P.z)[id.n] = zz;
@@ -642,32 +815,61 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
 
 // Op-level K(s) p with the reference BC rule (fixed rows -> identity), and
 // optionally p.Ap into out[0] (fem/kernels.py:143-176, fused_kernels.py:21-56).
+// Same tiles and LDS staging as PCG slot A; P.nb[G_PCG] = number of tiles.
 template <typename T>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* __restrict__ pin,
                                                        T* __restrict__ Ap, double* out) {
+  using Tl = Tile<T>;
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  __shared__ d2v hal[Tl::HN];
+  __shared__ double esc[Tl::EN];
   __shared__ double sh[16];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const NodeIdx id = node_of(P, lb);
-  vec2<T> q[3][3];
-  gather9<T>(gp((const vec2<T>*)pin), id, P, q);
-  double ax, ay;
-  stencil_fp64<T>(P, id, q, ax, ay);
-  const unsigned fb = fixbits(P.fix, id.n);
-  if (fb & 1u) ax = (double)q[1][1].x;
-  if (fb & 2u) ay = (double)q[1][1].y;
+  const TileOrg o = tile_org<T>(P, lb);
+  const GLB vec2<T>* p = gp((const vec2<T>*)pin);
+  vec2<T> hp[Tl::HL];
+#pragma unroll
+  for (int k = 0; k < Tl::HL; ++k) {
+    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
+    hp[k] = p[halo_node<T>(P, o, i < Tl::HN ? i : Tl::HN - 1)];
+  }
+  T hs[Tl::EL];
+  tile_scales_fetch<T>(P, o, hs);
+  NodeIdx ids[Tl::NPT];
+  unsigned fb[Tl::NPT];
+#pragma unroll
+  for (int j = 0; j < Tl::NPT; ++j) {
+    ids[j] = tile_node<T>(P, o, j);
+    fb[j] = fixbits(P.fix, ids[j].n);
+  }
+#pragma unroll
+  for (int k = 0; k < Tl::HL; ++k) {
+    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
+    if (k + 1 < Tl::HL || i < Tl::HN) hal[i] = d2v{(double)hp[k].x, (double)hp[k].y};
+  }
+  tile_scales_store<T>(hs, esc);
+  __syncthreads();
+  double ax[Tl::NPT], ay[Tl::NPT];
+  tile_stencil<T>(P, hal, esc, ax, ay);
   double v[1] = {0.0};
-  if (id.act) {
-    vec2<T> o;
-    o.x = (T)ax;
-    o.y = (T)ay;
-    gp((vec2<T>*)Ap)[id.n] = o;
-    v[0] = (double)q[1][1].x * (double)o.x + (double)q[1][1].y * (double)o.y;
+  GLB vec2<T>* ap = gp((vec2<T>*)Ap);
+#pragma unroll
+  for (int j = 0; j < Tl::NPT; ++j) {
+    const d2v pc = tile_own<T>(hal, j);
+    if (fb[j] & 1u) ax[j] = pc.x;
+    if (fb[j] & 2u) ay[j] = pc.y;
+    if (ids[j].act) {
+      vec2<T> ov;
+      ov.x = (T)ax[j];
+      ov.y = (T)ay[j];
+      ap[ids[j].n] = ov;
+      v[0] += pc.x * (double)ov.x + pc.y * (double)ov.y;
+    }
   }
   if (out == nullptr) return;
   block_sum<1>(v, sh);
   double tot[1];
-  if (tree_reduce<1>(v, P.part, P.ctr[G_NODE], lb, P.nb[G_NODE], tot)) {
+  if (tree_reduce<1>(v, P.part, P.ctr[G_NODE], lb, P.nb[G_PCG], tot)) {
     if (threadIdx.x == 0) out[0] = tot[0];
   }
 }

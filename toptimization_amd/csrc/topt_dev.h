@@ -33,7 +33,7 @@ struct Tap { int dx, dy; double w; };  // filter stencil entry (reference table 
 struct CgState {
   double rz, rz_prev, alpha, beta, pAp, rr, rr_true, rmax2;
   int it, status, op, pending;   // op: what the kernel that wrote this state executed
-  int pcur, first, refines, amode;
+  int pad3, first, refines, amode;
   int cflag, pad0, pad1, pad2;
 };
 // PCG slot operations (the state machine of fem/solver.py:27-159 on the device).
@@ -172,7 +172,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* sh /* >= 4*NV
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
+    constexpr int nw = TOPT_BLOCK / TOPT_WAVE;   // every kernel runs TOPT_BLOCK threads
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double t = sh[k];
@@ -247,7 +247,8 @@ __device__ bool tree_reduce(double (&v)[NV], double* part, unsigned* ctr, int lb
                             double (&tot)[NV], unsigned maxmask = 0u) {
   __shared__ double tsh[NV];
   __shared__ int s_go;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int nw = TOPT_BLOCK / TOPT_WAVE;
   long idx = lb, n = nb;
   double* lvl = part;
   unsigned* cl = ctr;
@@ -323,11 +324,13 @@ __device__ __forceinline__ void q4_gather(KP K, const T (&sc)[2][2],
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const vec2<T> v = q[a + CX[k]][b + CY[k]];
-        tx += K[(2 * me) * 8 + 2 * k] * v.x + K[(2 * me) * 8 + 2 * k + 1] * v.y;
-        ty += K[(2 * me + 1) * 8 + 2 * k] * v.x + K[(2 * me + 1) * 8 + 2 * k + 1] * v.y;
+        tx = fma(K[(2 * me) * 8 + 2 * k], v.x, tx);
+        tx = fma(K[(2 * me) * 8 + 2 * k + 1], v.y, tx);
+        ty = fma(K[(2 * me + 1) * 8 + 2 * k], v.x, ty);
+        ty = fma(K[(2 * me + 1) * 8 + 2 * k + 1], v.y, ty);
       }
-      ax += sc[a][b] * tx;
-      ay += sc[a][b] * ty;
+      ax = fma(sc[a][b], tx, ax);
+      ay = fma(sc[a][b], ty, ay);
     }
   }
   ox = ax;
