@@ -187,6 +187,18 @@ int fill_material(ProbDev& P, const double* Ke, double E_min, double penal) {
     P.Ki64[i] = Ki[i];
     P.Ki32[i] = (float)Ki[i];
   }
+  // structured stencil when Ke follows the unit-square Q4 pattern (to rounding)
+  const double kc[6] = {Ke[0], Ke[1], -Ke[2], -Ke[3], -Ke[4], Ke[6]};
+  double kmax = 0.0, dev = 0.0;
+  for (int i = 0; i < 64; ++i) kmax = std::max(kmax, std::fabs(Ke[i]));
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) {
+      const int t = KS[i][j];
+      const double v = t < 0 ? -kc[-t] : kc[t];
+      dev = std::max(dev, std::fabs(v - Ke[i * 8 + j]));
+    }
+  P.kstruct = (kmax > 0.0 && dev <= 1e-14 * kmax) ? 1 : 0;
+  for (int t = 0; t < 6; ++t) P.kc[t] = kc[t];
   return TOPT_OK;
 }
 
@@ -316,6 +328,7 @@ struct topt_plan {
   int n = 0;
   int dtype = TOPT_F64;
   bool any_elemblock = false;
+  bool kstruct = true;    // every problem's Ke follows the unit-square Q4 pattern
   int maxR = 0;
   int max_cg = 0;
   std::vector<ProbDev> host;
@@ -351,8 +364,13 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
     // slot A number na reads p buffer (na & 1) and writes the other one
     const int pa = (int)(pl->na & 1);
     pl->na++;
-    hipLaunchKernelGGL((k_slot_a<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
-                       (int)(pl->q++ & 1), pa);
+    const int qa = (int)(pl->q++ & 1);
+    if (pl->kstruct)
+      hipLaunchKernelGGL((k_slot_a<T, true, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, qa,
+                         pa);
+    else
+      hipLaunchKernelGGL((k_slot_a<T, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, qa,
+                         pa);
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                        (int)(pl->q++ & 1), pa ^ 1);
     if (pl->any_elemblock)
@@ -596,6 +614,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
                                "(its PCG iteration has three slots)");
     }
     pl->any_elemblock = (P.precond == PC_ELEMBLOCK);
+    pl->kstruct = pl->kstruct && P.kstruct;
     pl->maxR = std::max(pl->maxR, P.R);
     pl->max_cg = std::max(pl->max_cg, P.max_it);
   }
@@ -693,10 +712,17 @@ int topt_plan_time_slot(topt_plan* pl, int32_t which, int32_t reps, float* ms, v
   hipLaunchKernelGGL(k_cg_force_run, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
   HIP_TRY(hipEventRecord(e0, s));
   for (int k = 0; k < reps; ++k) {
-    if (pl->dtype == TOPT_F32)
-      hipLaunchKernelGGL((k_slot_a<float, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
-    else
-      hipLaunchKernelGGL((k_slot_a<double, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
+    if (pl->dtype == TOPT_F32) {
+      if (pl->kstruct)
+        hipLaunchKernelGGL((k_slot_a<float, true, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
+      else
+        hipLaunchKernelGGL((k_slot_a<float, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
+    } else {
+      if (pl->kstruct)
+        hipLaunchKernelGGL((k_slot_a<double, true, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
+      else
+        hipLaunchKernelGGL((k_slot_a<double, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, 0, 0);
+    }
   }
   HIP_TRY(hipEventRecord(e1, s));
   hipLaunchKernelGGL(k_cg_finish, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n);
@@ -761,12 +787,18 @@ int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, cons
   P.fix = fix;
   // one block per tile; the tree over those blocks (sized by geom().ntree)
   P.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float>(P.nelx, P.nely) : pcg_tiles<double>(P.nelx, P.nely));
-  if (dtype == TOPT_F32)
-    hipLaunchKernelGGL((k_matvec<float>), dim3(P.nb[G_PCG]), dim3(TOPT_BLOCK), 0, st, P, (const float*)p,
-                       (float*)Ap, pAp);
-  else
-    hipLaunchKernelGGL((k_matvec<double>), dim3(P.nb[G_PCG]), dim3(TOPT_BLOCK), 0, st, P, (const double*)p,
-                       (double*)Ap, pAp);
+  const dim3 grid(P.nb[G_PCG]), blk(TOPT_BLOCK);
+  if (dtype == TOPT_F32) {
+    if (P.kstruct)
+      hipLaunchKernelGGL((k_matvec<float, true>), grid, blk, 0, st, P, (const float*)p, (float*)Ap, pAp);
+    else
+      hipLaunchKernelGGL((k_matvec<float, false>), grid, blk, 0, st, P, (const float*)p, (float*)Ap, pAp);
+  } else {
+    if (P.kstruct)
+      hipLaunchKernelGGL((k_matvec<double, true>), grid, blk, 0, st, P, (const double*)p, (double*)Ap, pAp);
+    else
+      hipLaunchKernelGGL((k_matvec<double, false>), grid, blk, 0, st, P, (const double*)p, (double*)Ap, pAp);
+  }
   LAUNCH_TRY();
   return TOPT_OK;
 }

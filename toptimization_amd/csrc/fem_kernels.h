@@ -253,20 +253,46 @@ __device__ __forceinline__ NodeIdx tile_node(PD& P, TileOrg o, int j) {
   return id;
 }
 
-// Halo entry i (i < HN) -> clamped global node (as nbr_node).
+// Staging map of a tile: thread (wave w, lane) owns the NPT nodes of tile
+// column w + 4j, row lane, and loads them as the interior entries of the
+// node halo (position (w + 4j + 1, lane + 1)); the halo ring (two columns and
+// two rows, Ring::NN entries) is spread over the first threads. Element
+// scales likewise: own entries at (w + 4j + 1, lane + 1) and a ring of
+// Ring::NE entries (column 0 and row 0). Every load is issued before the
+// first wait, with simple index arithmetic.
 template <typename T>
-__device__ __forceinline__ long halo_node(PD& P, TileOrg o, int i) {
-  const int cx = i / Tile<T>::HY, cy = i - cx * Tile<T>::HY;
+struct Ring {
+  static constexpr int NN = 2 * Tile<T>::HY + 2 * Tile<T>::TX;   // node ring entries
+  static constexpr int NE = Tile<T>::EY + Tile<T>::TX;           // element ring entries
+};
+
+// Ring node entry t (< Ring::NN) -> halo position (cx, cy).
+template <typename T>
+__device__ __forceinline__ void ring_node_pos(int t, int& cx, int& cy) {
+  constexpr int HY = Tile<T>::HY, TX = Tile<T>::TX;
+  if (t < HY) { cx = 0; cy = t; }
+  else if (t < 2 * HY) { cx = TX + 1; cy = t - HY; }
+  else if (t < 2 * HY + TX) { cx = t - 2 * HY + 1; cy = 0; }
+  else { cx = t - 2 * HY - TX + 1; cy = HY - 1; }
+}
+// Ring element entry t (< Ring::NE) -> element-tile position (cx, cy).
+template <typename T>
+__device__ __forceinline__ void ring_elem_pos(int t, int& cx, int& cy) {
+  constexpr int EY = Tile<T>::EY;
+  if (t < EY) { cx = 0; cy = t; }
+  else { cx = t - EY + 1; cy = 0; }
+}
+// Clamped global node at halo position (cx, cy) (as nbr_node).
+__device__ __forceinline__ long halo_node_at(PD& P, TileOrg o, int cx, int cy) {
   int jx = o.x0 - 1 + cx, jy = o.y0 - 1 + cy;
   jx = jx < 0 ? 0 : (jx > P.nelx ? P.nelx : jx);
   jy = jy < 0 ? 0 : (jy > P.nely ? P.nely : jy);
   return (long)jx * (P.nely + 1) + jy;
 }
-
-// Element halo entry i (i < EN) -> scale, zero outside the grid (as load_scales).
+// Scale of the element at element-tile position (cx, cy); zero outside the
+// grid (as load_scales).
 template <typename T>
-__device__ __forceinline__ T halo_scale(PD& P, const GLB T* s, TileOrg o, int i) {
-  const int cx = i / Tile<T>::EY, cy = i - cx * Tile<T>::EY;
+__device__ __forceinline__ T halo_scale_at(PD& P, const GLB T* s, TileOrg o, int cx, int cy) {
   const int ex = o.x0 - 1 + cx, ey = o.y0 - 1 + cy;
   const bool ok = (ex >= 0) & (ex < P.nelx) & (ey >= 0) & (ey < P.nely);
   const int kx = ex < 0 ? 0 : (ex >= P.nelx ? P.nelx - 1 : ex);
@@ -275,23 +301,79 @@ __device__ __forceinline__ T halo_scale(PD& P, const GLB T* s, TileOrg o, int i)
   return ok ? v : T(0);
 }
 
-// Element scales of the tile, fetched into registers (entry threadIdx + 256 k).
+// State-independent geometry of a thread's share of a tile: its nodes, their
+// fixed bits, the ring node, and the element scales (own + ring), fetched.
 template <typename T>
-__device__ __forceinline__ void tile_scales_fetch(PD& P, TileOrg o, T (&hs)[Tile<T>::EL]) {
+struct TileShare {
+  NodeIdx ids[Tile<T>::NPT];
+  unsigned fb[Tile<T>::NPT];
+  T se[Tile<T>::NPT];
+  long rn;          // ring node (clamped; valid for every thread)
+  int rcx, rcy;     // its halo position
+  int ecx, ecy;     // ring element position
+  T rse;            // ring element scale
+  bool hasr, hase;
+};
+// INNER: the tile and its halo lie inside the grid (no clamping, every
+// node active); see tile_inner.
+template <typename T, bool INNER>
+__device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T>& S) {
+  using Tl = Tile<T>;
   const GLB T* s = gp((const T*)P.s);
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int nny = P.nely + 1;
 #pragma unroll
-  for (int k = 0; k < Tile<T>::EL; ++k) {
-    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
-    hs[k] = halo_scale<T>(P, s, o, i < Tile<T>::EN ? i : Tile<T>::EN - 1);
+  for (int j = 0; j < Tl::NPT; ++j) {
+    if constexpr (INNER) {
+      const int ix = o.x0 + w + Tl::NW * j, iy = o.y0 + lane;
+      S.ids[j].ix = ix;
+      S.ids[j].iy = iy;
+      S.ids[j].act = true;
+      S.ids[j].n = (long)ix * nny + iy;
+      S.se[j] = s[(long)ix * P.nely + iy];
+    } else {
+      S.ids[j] = tile_node<T>(P, o, j);
+      S.se[j] = halo_scale_at<T>(P, s, o, w + Tl::NW * j + 1, lane + 1);
+    }
+    S.fb[j] = fixbits(P.fix, S.ids[j].n);
+  }
+  S.hasr = (int)threadIdx.x < Ring<T>::NN;
+  ring_node_pos<T>(S.hasr ? (int)threadIdx.x : 0, S.rcx, S.rcy);
+  S.hase = (int)threadIdx.x < Ring<T>::NE;
+  ring_elem_pos<T>(S.hase ? (int)threadIdx.x : 0, S.ecx, S.ecy);
+  if constexpr (INNER) {
+    S.rn = (long)(o.x0 - 1 + S.rcx) * nny + (o.y0 - 1 + S.rcy);
+    S.rse = s[(long)(o.x0 - 1 + S.ecx) * P.nely + (o.y0 - 1 + S.ecy)];
+  } else {
+    S.rn = halo_node_at(P, o, S.rcx, S.rcy);
+    S.rse = halo_scale_at<T>(P, s, o, S.ecx, S.ecy);
   }
 }
+// True when every halo node and element of the tile lies inside the grid.
 template <typename T>
-__device__ __forceinline__ void tile_scales_store(const T (&hs)[Tile<T>::EL], double* esc) {
+__device__ __forceinline__ bool tile_inner(PD& P, TileOrg o) {
+  return o.x0 >= 1 && o.y0 >= 1 && o.x0 + Tile<T>::TX <= P.nelx && o.y0 + Tile<T>::TY <= P.nely;
+}
+template <typename T>
+__device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T>& S) {
+  if (tile_inner<T>(P, o)) tile_share<T, true>(P, o, S);
+  else tile_share<T, false>(P, o, S);
+}
+
+// Write the tile's halo values (own + ring, FP64) and element scales to LDS.
+template <typename T>
+__device__ __forceinline__ void tile_store(const TileShare<T>& S, const d2v (&own)[Tile<T>::NPT], d2v ring,
+                                           d2v* hal, double* esc) {
+  using Tl = Tile<T>;
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
 #pragma unroll
-  for (int k = 0; k < Tile<T>::EL; ++k) {
-    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
-    if (k + 1 < Tile<T>::EL || i < Tile<T>::EN) esc[i] = (double)hs[k];
+  for (int j = 0; j < Tl::NPT; ++j) {
+    const int c = w + Tl::NW * j;
+    hal[(c + 1) * Tl::HY + lane + 1] = own[j];
+    esc[(c + 1) * Tl::EY + lane + 1] = (double)S.se[j];
   }
+  if (S.hasr) hal[S.rcx * Tl::HY + S.rcy] = ring;
+  if (S.hase) esc[S.ecx * Tl::EY + S.ecy] = (double)S.rse;
 }
 
 // K(s) q at the thread's NPT tile nodes from the LDS halo (values and
@@ -335,12 +417,38 @@ __device__ __forceinline__ void tile_stencil(PD& P, const d2v* hal, const double
     }
 }
 
-// The node's own halo value (centre of its 3x3 neighbourhood).
+// K(s) q at the thread's NPT tile nodes, structured Ke (six values in
+// scalar registers), node-outer: each neighbour value is read from LDS once.
 template <typename T>
-__device__ __forceinline__ d2v tile_own(const d2v* hal, int j) {
+__device__ __forceinline__ void tile_stencil_s(PD& P, const d2v* hal, const double* esc,
+                                               double (&ax)[Tile<T>::NPT], double (&ay)[Tile<T>::NPT]) {
   using Tl = Tile<T>;
-  const int c = (int)(threadIdx.x >> 6) + Tl::NW * j, row = (int)(threadIdx.x & 63);
-  return hal[(c + 1) * Tl::HY + row + 1];
+  const int w = (int)(threadIdx.x >> 6), row = (int)(threadIdx.x & 63);
+  double k[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) k[t] = P.kc[t];
+#pragma unroll
+  for (int j = 0; j < Tl::NPT; ++j) {
+    const int c = w + Tl::NW * j;
+    d2v q[3][3];
+    double sc[2][2];
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) q[dx][dy] = hal[(c + dx) * Tl::HY + row + dy];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) sc[a][b] = esc[(c + a) * Tl::EY + row + b];
+    q4_gather_s(k, sc, q, ax[j], ay[j]);
+  }
+}
+
+template <typename T, bool KST>
+__device__ __forceinline__ void tile_apply(PD& P, const d2v* hal, const double* esc,
+                                           double (&ax)[Tile<T>::NPT], double (&ay)[Tile<T>::NPT]) {
+  if constexpr (KST) tile_stencil_s<T>(P, hal, esc, ax, ay);
+  else tile_stencil<T>(P, hal, esc, ax, ay);
 }
 
 // Previous slot's partials, fetched into registers before the state is read.
@@ -560,10 +668,11 @@ __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id
 // the FP64 Ke, so the inner operator is K(s32) up to the final rounding of
 // Ap (an FP32 Ke and FP32 sums break the rigid-body cancellation and stall
 // the FP64 residual refresh).
-template <typename T, bool B>
+template <typename T, bool B, bool KST>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const int* bmap,
                                                        ProbDev single, int q, int par) {
   using Tl = Tile<T>;
+  constexpr int NPT = Tl::NPT;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   __shared__ double sh[16];
@@ -573,63 +682,57 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* pold = gp((const vec2<T>*)(par ? P.p1 : P.p0));
   GLB vec2<T>* pnew = gp((vec2<T>*)(par ? P.p0 : P.p1));
+  const GLB vec2<T>* z = gp((const vec2<T>*)P.z);
   // state-independent loads first
   PrevParts pp;
   parts_fetch(P, q, pp);
-  const GLB vec2<T>* z = gp((const vec2<T>*)P.z);
-  vec2<T> hz[Tl::HL], hp[Tl::HL];
+  TileShare<T> S;
+  tile_share<T>(P, o, S);
+  vec2<T> oz[NPT], op[NPT];
 #pragma unroll
-  for (int k = 0; k < Tl::HL; ++k) {
-    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
-    const long m = halo_node<T>(P, o, i < Tl::HN ? i : Tl::HN - 1);
-    hz[k] = z[m];
-    hp[k] = pold[m];
+  for (int j = 0; j < NPT; ++j) {
+    oz[j] = z[S.ids[j].n];
+    op[j] = pold[S.ids[j].n];
   }
-  T hs[Tl::EL];
-  tile_scales_fetch<T>(P, o, hs);
-  NodeIdx ids[Tl::NPT];
-  unsigned fb[Tl::NPT];
-#pragma unroll
-  for (int j = 0; j < Tl::NPT; ++j) {
-    ids[j] = tile_node<T>(P, o, j);
-    fb[j] = fixbits(P.fix, ids[j].n);
-  }
+  const vec2<T> rz = z[S.rn], rp = pold[S.rn];
   const CgState st = slot_begin<T>(P, lb, q, pp, sh);
   if (st.status != ST_RUN) return;
   double v = 0.0;
   if (st.op == OP_A_REF) {
 #pragma unroll
-    for (int j = 0; j < Tl::NPT; ++j) {
-      v += slot_refresh_residual<T>(P, ids[j]);
-      if (ids[j].act) pnew[ids[j].n] = pold[ids[j].n];
+    for (int j = 0; j < NPT; ++j) {
+      v += slot_refresh_residual<T>(P, S.ids[j]);
+      if (S.ids[j].act) pnew[S.ids[j].n] = op[j];
     }
   } else {
     const T beta = (T)st.beta;
+    vec2<T> pn[NPT];
+    d2v hv[NPT];
 #pragma unroll
-    for (int k = 0; k < Tl::HL; ++k) {
-      const int i = (int)threadIdx.x + k * TOPT_BLOCK;
-      vec2<T> w;
-      w.x = hz[k].x + beta * hp[k].x;
-      w.y = hz[k].y + beta * hp[k].y;
-      if (k + 1 < Tl::HL || i < Tl::HN) hal[i] = d2v{(double)w.x, (double)w.y};
+    for (int j = 0; j < NPT; ++j) {
+      pn[j].x = fma(beta, op[j].x, oz[j].x);
+      pn[j].y = fma(beta, op[j].y, oz[j].y);
+      hv[j] = d2v{(double)pn[j].x, (double)pn[j].y};
     }
-    tile_scales_store<T>(hs, esc);
+    vec2<T> rw;
+    rw.x = fma(beta, rp.x, rz.x);
+    rw.y = fma(beta, rp.y, rz.y);
+    tile_store<T>(S, hv, d2v{(double)rw.x, (double)rw.y}, hal, esc);
     __syncthreads();
-    double ax[Tl::NPT], ay[Tl::NPT];
-    tile_stencil<T>(P, hal, esc, ax, ay);
+    double ax[NPT], ay[NPT];
+    tile_apply<T, KST>(P, hal, esc, ax, ay);
     GLB vec2<T>* Ap = gp((vec2<T>*)P.Ap);
 #pragma unroll
-    for (int j = 0; j < Tl::NPT; ++j) {
-      const d2v own = tile_own<T>(hal, j);
-      if (fb[j] & 1u) ax[j] = own.x;
-      if (fb[j] & 2u) ay[j] = own.y;
-      if (ids[j].act) {
-        pnew[ids[j].n] = vec2<T>{(T)own.x, (T)own.y};
+    for (int j = 0; j < NPT; ++j) {
+      if (S.fb[j] & 1u) ax[j] = (double)pn[j].x;
+      if (S.fb[j] & 2u) ay[j] = (double)pn[j].y;
+      if (S.ids[j].act) {
+        pnew[S.ids[j].n] = pn[j];
         vec2<T> ov;
         ov.x = (T)ax[j];
         ov.y = (T)ay[j];
-        Ap[ids[j].n] = ov;
-        v += own.x * (double)ov.x + own.y * (double)ov.y;
+        Ap[S.ids[j].n] = ov;
+        v += (double)pn[j].x * (double)ov.x + (double)pn[j].y * (double)ov.y;
       }
     }
   }
@@ -816,10 +919,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
 // Op-level K(s) p with the reference BC rule (fixed rows -> identity), and
 // optionally p.Ap into out[0] (fem/kernels.py:143-176, fused_kernels.py:21-56).
 // Same tiles and LDS staging as PCG slot A; P.nb[G_PCG] = number of tiles.
-template <typename T>
+template <typename T, bool KST>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* __restrict__ pin,
                                                        T* __restrict__ Ap, double* out) {
   using Tl = Tile<T>;
+  constexpr int NPT = Tl::NPT;
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
@@ -827,46 +931,34 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* 
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* p = gp((const vec2<T>*)pin);
-  vec2<T> hp[Tl::HL];
+  TileShare<T> S;
+  tile_share<T>(P, o, S);
+  vec2<T> op[NPT];
 #pragma unroll
-  for (int k = 0; k < Tl::HL; ++k) {
-    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
-    hp[k] = p[halo_node<T>(P, o, i < Tl::HN ? i : Tl::HN - 1)];
-  }
-  T hs[Tl::EL];
-  tile_scales_fetch<T>(P, o, hs);
-  NodeIdx ids[Tl::NPT];
-  unsigned fb[Tl::NPT];
+  for (int j = 0; j < NPT; ++j) op[j] = p[S.ids[j].n];
+  const vec2<T> rp = p[S.rn];
+  d2v hv[NPT];
 #pragma unroll
-  for (int j = 0; j < Tl::NPT; ++j) {
-    ids[j] = tile_node<T>(P, o, j);
-    fb[j] = fixbits(P.fix, ids[j].n);
-  }
-#pragma unroll
-  for (int k = 0; k < Tl::HL; ++k) {
-    const int i = (int)threadIdx.x + k * TOPT_BLOCK;
-    if (k + 1 < Tl::HL || i < Tl::HN) hal[i] = d2v{(double)hp[k].x, (double)hp[k].y};
-  }
-  tile_scales_store<T>(hs, esc);
+  for (int j = 0; j < NPT; ++j) hv[j] = d2v{(double)op[j].x, (double)op[j].y};
+  tile_store<T>(S, hv, d2v{(double)rp.x, (double)rp.y}, hal, esc);
   __syncthreads();
-  double ax[Tl::NPT], ay[Tl::NPT];
-  tile_stencil<T>(P, hal, esc, ax, ay);
+  double ax[NPT], ay[NPT];
+  tile_apply<T, KST>(P, hal, esc, ax, ay);
   double v[1] = {0.0};
   GLB vec2<T>* ap = gp((vec2<T>*)Ap);
+  vec2<T> ov[NPT];
 #pragma unroll
-  for (int j = 0; j < Tl::NPT; ++j) {
-    const d2v pc = tile_own<T>(hal, j);
-    if (fb[j] & 1u) ax[j] = pc.x;
-    if (fb[j] & 2u) ay[j] = pc.y;
-    if (ids[j].act) {
-      vec2<T> ov;
-      ov.x = (T)ax[j];
-      ov.y = (T)ay[j];
-      ap[ids[j].n] = ov;
-      v[0] += pc.x * (double)ov.x + pc.y * (double)ov.y;
-    }
+  for (int j = 0; j < NPT; ++j) {
+    if (S.fb[j] & 1u) ax[j] = (double)op[j].x;
+    if (S.fb[j] & 2u) ay[j] = (double)op[j].y;
+    ov[j].x = (T)ax[j];
+    ov[j].y = (T)ay[j];
+    if (S.ids[j].act) ap[S.ids[j].n] = ov[j];
   }
   if (out == nullptr) return;
+#pragma unroll
+  for (int j = 0; j < NPT; ++j)
+    if (S.ids[j].act) v[0] += (double)op[j].x * (double)ov[j].x + (double)op[j].y * (double)ov[j].y;
   block_sum<1>(v, sh);
   double tot[1];
   if (tree_reduce<1>(v, P.part, P.ctr[G_NODE], lb, P.nb[G_PCG], tot)) {

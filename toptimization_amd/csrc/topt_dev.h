@@ -67,6 +67,8 @@ struct ProbDev {
   double Ke64[64];
   float Ki32[64];        // regularised Ke inverse (element-block Jacobi)
   double Ki64[64];
+  double kc[6];          // Ke = KS pattern x kc when kstruct (see q4_gather_s)
+  int kstruct, pad_k;
   // FEM vectors (node interleaved). T = float (mixed) or double (plain).
   void* x;   // CG iterate: plain = the solution; mixed = FP32 correction
   void* r; void* z; void* p0; void* p1; void* Ap;
@@ -328,6 +330,46 @@ __device__ __forceinline__ void q4_gather(KP K, const T (&sc)[2][2],
         tx = fma(K[(2 * me) * 8 + 2 * k + 1], v.y, tx);
         ty = fma(K[(2 * me + 1) * 8 + 2 * k], v.x, ty);
         ty = fma(K[(2 * me + 1) * 8 + 2 * k + 1], v.y, ty);
+      }
+      ax = fma(sc[a][b], tx, ax);
+      ay = fma(sc[a][b], ty, ay);
+    }
+  }
+  ox = ax;
+  oy = ay;
+}
+
+// The unit-square Q4 plane-stress stiffness of material.py:20-86 has six
+// distinct magnitudes whatever E and nu: Ke[i][j] = sign(KS[i][j]) *
+// kc[|KS[i][j]|] (kc[0] is the diagonal). The host checks a given Ke against
+// this pattern (kstruct); the structured stencil then keeps the six values
+// in scalar registers for the whole kernel instead of reloading 64.
+constexpr int KS[8][8] = {{0, 1, -2, -3, -4, -1, 5, 3},  {1, 0, 3, 5, -1, -4, -3, -2},
+                          {-2, 3, 0, -1, 5, -3, -4, 1},  {-3, 5, -1, 0, 3, -2, 1, -4},
+                          {-4, -1, 5, 3, 0, 1, -2, -3},  {-1, -4, -3, -2, 1, 0, 3, 5},
+                          {5, -3, -4, 1, -2, 3, 0, -1},  {3, -2, 1, -4, -3, 5, -1, 0}};
+
+// q4_gather with Ke from the structured pattern (FP64).
+__device__ __forceinline__ void q4_gather_s(const double (&k)[6], const double (&sc)[2][2],
+                                            const d2v (&q)[3][3], double& ox, double& oy) {
+  constexpr int CX[4] = {0, 1, 1, 0};
+  constexpr int CY[4] = {0, 0, 1, 1};
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int me = (a == 1) ? ((b == 1) ? 0 : 3) : ((b == 1) ? 1 : 2);
+      double tx = 0.0, ty = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const d2v v = q[a + CX[kk]][b + CY[kk]];
+        const int i0 = KS[2 * me][2 * kk], i1 = KS[2 * me][2 * kk + 1];
+        const int j0 = KS[2 * me + 1][2 * kk], j1 = KS[2 * me + 1][2 * kk + 1];
+        tx = fma(i0 < 0 ? -k[-i0] : k[i0], v.x, tx);
+        tx = fma(i1 < 0 ? -k[-i1] : k[i1], v.y, tx);
+        ty = fma(j0 < 0 ? -k[-j0] : k[j0], v.x, ty);
+        ty = fma(j1 < 0 ? -k[-j1] : k[j1], v.y, ty);
       }
       ax = fma(sc[a][b], tx, ax);
       ay = fma(sc[a][b], ty, ay);
