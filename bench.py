@@ -15,6 +15,8 @@ restatement of the reference algorithm (FP32, Jacobi, as the reference) on
 the host cores as the CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --batch      (64 x 128x64 cantilevers, batched vs one at a time)
+       python bench.py --serving    (512-request stream: designs/hour, p50/p99 latency)
 """
 
 from __future__ import annotations
@@ -152,6 +154,89 @@ def cpu_baseline(log, budget_s=25.0):
                       f"restatement of the reference algorithm"}
 
 
+def batch_bench(log, n=64, iters=100, seq_sample=None):
+    """64 independent 128x64 cantilevers (seeded loads, volfrac U[0.3, 0.6],
+    rmin 2.5, mixed FP32/FP64 PCG rtol 1e-6, 2x2 block Jacobi) advanced
+    together by one plan (one launch per kernel for all of them) against the
+    same problems solved one at a time. Every problem's iterates are the same
+    either way (per-problem reductions in a fixed order)."""
+    import torch
+    from toptimization_amd.engine import DesignState, Plan
+    from toptimization_amd.serving import batch_cantilevers
+    probs = batch_cantilevers(n, max_iter=iters, cg_tolerance=1e-6, max_cg_iterations=50000,
+                              preconditioner="block_jacobi_2x2", precision="mixed")
+    states = [DesignState(p) for p in probs]
+    plan = Plan(states)
+    plan.iteration()   # warm-up: first iteration (setup, cold caches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ncg = 0
+    for k in range(iters - 1):
+        ncg += sum(s.n_cg for s in plan.iteration())
+        if k % 20 == 19:
+            log(f"  batch: {k + 2} OC iterations done")
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    plan.close()
+    batched = n * (iters - 1) / tb
+    log(f"  batch {n} x 128x64: {iters - 1} OC it each in {tb:.2f} s -> {batched:.1f} OC it/s "
+        f"({ncg} CG iterations)")
+    m = n if seq_sample is None else min(n, seq_sample)
+    ts = 0.0
+    for i, p in enumerate(probs[:m]):
+        if i % 8 == 0:
+            log(f"  sequential: problem {i}")
+        st = DesignState(p)
+        pl = Plan([st])
+        pl.iteration()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters - 1):
+            pl.iteration()
+        torch.cuda.synchronize()
+        ts += time.perf_counter() - t0
+        pl.close()
+    seq = m * (iters - 1) / ts
+    log(f"  sequential ({m} problems): {seq:.1f} OC it/s")
+    return {"metric": f"batched OC it/s, {n} x 128x64 cantilevers", "value": round(batched, 2),
+            "unit": "OC it/s", "sequential_oc_its_per_s": round(seq, 2),
+            "sequential_sample": m, "speedup": round(batched / seq, 2), "oc_iterations_each": iters - 1,
+            "cg_iterations_batched": ncg, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
+
+
+def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32):
+    """Serving stream: n seeded requests, Poisson arrivals at `rate`/s, nelx
+    uniform over multiples of 16 in [64, 768], aspect 1/4, 1/3 or 1/2, BC
+    cantilever / MBB / bridge, volfrac U[0.3, 0.5], rmin 0.008 nelx + 1, run to
+    100 OC iterations or max change < 0.01; continuous batching with HBM
+    admission. Sequential baseline: the first seq_sample requests of the same
+    stream served one at a time (designs/hour of that sample)."""
+    from toptimization_amd.serving import DesignServer, synthetic_stream
+    solver = dict(cg_tolerance=1e-6, max_cg_iterations=50000, preconditioner="block_jacobi_2x2",
+                  precision="mixed")
+    reqs = synthetic_stream(n=n, rate=rate, seed=0, nelx_range=(64, 768), max_iter=100, tol=0.01, **solver)
+    server = DesignServer(max_active=max_active, hbm_budget=64 << 30, precision="mixed")
+    st = server.run(reqs, clock="wall", progress=log)
+    s = st.summary()
+    log(f"  serving {n} requests at {rate}/s: makespan {s['makespan_s']:.1f} s, "
+        f"{s['designs_per_hour']:.0f} designs/h, p50 {s['p50_latency_s']:.2f} s, p99 {s['p99_latency_s']:.2f} s, "
+        f"max active {s['max_active']}")
+    seq = None
+    if seq_sample:
+        sub = synthetic_stream(n=seq_sample, rate=rate, seed=0, nelx_range=(64, 768), max_iter=100,
+                               tol=0.01, **solver)
+        one = DesignServer(max_active=1, hbm_budget=64 << 30, precision="mixed")
+        ss = one.run(sub, clock="instant", progress=log).summary()
+        seq = ss["designs_per_hour"]
+        log(f"  sequential, first {seq_sample} requests back to back: {seq:.0f} designs/h")
+    return {"metric": "serving designs/hour (512-request Poisson stream)", "value": round(s["designs_per_hour"], 1),
+            "unit": "designs/hour", "p50_latency_s": round(s["p50_latency_s"], 3),
+            "p99_latency_s": round(s["p99_latency_s"], 3), "makespan_s": round(s["makespan_s"], 2),
+            "requests": n, "arrival_rate_per_s": rate, "max_active": s["max_active"],
+            "engine_steps": s["engine_steps"], "sequential_designs_per_hour": None if seq is None else round(seq, 1),
+            "sequential_sample": seq_sample, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +246,12 @@ def main() -> int:
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="profiling driver: one OC iteration, 100 slot-A launches, op-level matvecs")
+    ap.add_argument("--batch", action="store_true",
+                    help="64 x 128x64 cantilevers: one batched plan vs one at a time (own JSON line)")
+    ap.add_argument("--serving", action="store_true",
+                    help="512-request serving stream: designs/hour, p50/p99 latency (own JSON line)")
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--seq-sample", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a design problem is "
@@ -171,6 +262,12 @@ def main() -> int:
     import torch
     torch.cuda.set_device(0)
     nn, ne = (NELX + 1) * (NELY + 1), NELX * NELY
+    if args.batch:
+        print(json.dumps(batch_bench(log)), flush=True)
+        return 0
+    if args.serving:
+        print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample)), flush=True)
+        return 0
     if args.probe:
         from toptimization_amd.engine import DesignState, Plan
         st = DesignState(bridge(NELX, NELY, "mixed"))

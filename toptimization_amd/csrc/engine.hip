@@ -116,7 +116,7 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   sz[TOPT_BUF_F] = g.nn * 16;
   sz[TOPT_BUF_FIX] = g.words * 4;
   sz[TOPT_BUF_RHO] = sz[TOPT_BUF_RHOF] = sz[TOPT_BUF_DC] = sz[TOPT_BUF_DCF] = sz[TOPT_BUF_HS] = g.ne * 8;
-  sz[TOPT_BUF_X] = dtype == TOPT_F32 ? g.nn * 2 * ts : 0;   // plain path: x aliases u
+  sz[TOPT_BUF_X] = 0;   // the PCG iterate is the FP64 solution u on both paths
   sz[TOPT_BUF_R] = sz[TOPT_BUF_Z] = sz[TOPT_BUF_P0] = sz[TOPT_BUF_P1] = sz[TOPT_BUF_AP] = g.nn * 2 * ts;
   sz[TOPT_BUF_PRE] = (precond == TOPT_PC_JACOBI || precond == TOPT_PC_BLOCK2) ? g.nn * 2 * ts : 0;
   sz[TOPT_BUF_PRE2] = precond == TOPT_PC_BLOCK2 ? g.nn * ts : 0;

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, cons
   }
 }
 
-// Start of a solve: u[fixed] = 0 (or u = 0 cold / zero load), x = 0 (mixed),
+// Start of a solve: u[fixed] = 0 (or u = 0 cold / zero load),
 // p buffers = 0, CG state reset. warm: 0 cold, 1 warm, 2 warm iff the
 // problem has finished at least one OC iteration (fem/solver.py:77-83;
 // optimizer/simp.py:160 passes warm_start = it > 1).
@@ -479,8 +479,9 @@ __device__ __forceinline__ void refresh_done(CgState& s, double rzn) {
 // Out: s.op = what kernel q does (or status != RUN). Follows fem/solver.py:
 // alpha = rz/pAp (:117), breakdown |pAp| < 1e-30 (:114), convergence on the
 // recursive |r|^2 < tol^2 |f|^2 (:147), beta = rz/(rz_old + 1e-60) (:151).
-// Mixed path: convergence, max_iter and breakdown first commit the FP32
-// correction and are confirmed on the FP64 true residual (A_REF/B_REF).
+// Mixed path: convergence, max_iter and breakdown are confirmed on the FP64
+// true residual (A_REF/B_REF), which also replaces the FP32 recursive one
+// whenever that has fallen by delta (reliable update).
 template <typename T>
 __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)[2]) {
   constexpr bool MIXED = sizeof(T) == 4;
@@ -620,27 +621,16 @@ __device__ __forceinline__ void slot_end(PD& P, int lb, int q, double v0, double
   }
 }
 
-// A_REF: r = f - A(u64 [+ x]) in FP64 (stored in T); partial |r|^2 (FP64).
+// A_REF: r = f - A u in FP64 (stored in T); partial |r|^2 (FP64).
 // Rare (a few times per solve): direct loads.
 template <typename T>
 __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id) {
-  constexpr bool MIXED = sizeof(T) == 4;
   const GLB d2v* u = gp((const d2v*)P.u64);
-  const GLB vec2<T>* x = gp((const vec2<T>*)P.x);
   d2v q[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
-      d2v v = u[m];
-      if (MIXED) {
-        const vec2<T> c = x[m];
-        v.x += (double)c.x;
-        v.y += (double)c.y;
-      }
-      q[dx][dy] = v;
-    }
+    for (int dy = 0; dy < 3; ++dy) q[dx][dy] = u[nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely)];
   double sc[2][2];
   load_scales<double>(P.s64, id.ix, id.iy, P.nelx, P.nely, sc);
   double ax, ay;
@@ -757,14 +747,17 @@ __device__ __forceinline__ vec2<T> precond_apply(int pc, vec2<T> d, T o, vec2<T>
 
 // Slot B. B_CG: x += alpha p, r -= alpha Ap, z = M^-1 r, partials r.z, r.r
 // (fem/fused_kernels.py:110-166; element-block Jacobi leaves z to slot C as
-// fem/fused_kernels.py:169-180 + preconditioner.py do). B_REF: u64 += x,
-// x = 0 (mixed), z = M^-1 r, partial r.z.
+// fem/fused_kernels.py:169-180 + preconditioner.py do). B_REF: z = M^-1 r,
+// partial r.z. The iterate x is the FP64 solution u on both paths: with an
+// FP32 iterate the small late updates a p are rounded away against large
+// intermediate values and the true residual stops falling (measured on the
+// CPU restatement: a 608x152 MBB stalls at 9e-6 with an FP32 iterate and
+// converges in the FP64 iteration count with an FP64 one).
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const int* bmap,
                                                        ProbDev single, int q, int par) {
   using Tl = Tile<T>;
   __shared__ double sh[16];
-  constexpr bool MIXED = sizeof(T) == 4;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   PD& P = get_prob<B>(tab, bmap, single, b);
   const int lb = b - P.b0[G_PCG];
@@ -772,7 +765,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
   const int pc = P.precond;
   GLB vec2<T>* r = gp((vec2<T>*)P.r);
   GLB vec2<T>* z = gp((vec2<T>*)P.z);
-  GLB vec2<T>* x = gp((vec2<T>*)P.x);
+  GLB d2v* x = gp((d2v*)P.u64);
   const GLB vec2<T>* Ap = gp((const vec2<T>*)P.Ap);
   // state-independent loads first
   PrevParts pp;
@@ -780,7 +773,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
   // p as written by the preceding slot A (par = parity of that buffer)
   const GLB vec2<T>* p = gp((const vec2<T>*)(par ? P.p1 : P.p0));
   NodeIdx ids[Tl::NPT];
-  vec2<T> rv[Tl::NPT], xv[Tl::NPT], av[Tl::NPT], dv[Tl::NPT], pv[Tl::NPT];
+  vec2<T> rv[Tl::NPT], av[Tl::NPT], dv[Tl::NPT], pv[Tl::NPT];
+  d2v xv[Tl::NPT];
   T ov[Tl::NPT];
 #pragma unroll
   for (int j = 0; j < Tl::NPT; ++j) {
@@ -804,14 +798,6 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
     for (int j = 0; j < Tl::NPT; ++j) {
       if (!ids[j].act) continue;
       const long n = ids[j].n;
-      if (MIXED) {
-        GLB d2v* u = gp((d2v*)P.u64);
-        d2v uv = u[n];
-        uv.x += (double)xv[j].x;
-        uv.y += (double)xv[j].y;
-        u[n] = uv;
-        x[n] = vec2<T>{T(0), T(0)};
-      }
       if (!eb) {
         const vec2<T> zz = precond_apply<T>(pc, dv[j], ov[j], rv[j]);
         z[n] = zz;
@@ -824,9 +810,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
     for (int j = 0; j < Tl::NPT; ++j) {
       if (!ids[j].act) continue;
       const long n = ids[j].n;
-      vec2<T> xn = xv[j], rn = rv[j];
-      xn.x += alpha * pv[j].x;
-      xn.y += alpha * pv[j].y;
+      d2v xn = xv[j];
+      vec2<T> rn = rv[j];
+      xn.x = fma((double)alpha, (double)pv[j].x, xn.x);
+      xn.y = fma((double)alpha, (double)pv[j].y, xn.y);
       rn.x -= alpha * av[j].x;
       rn.y -= alpha * av[j].y;
       x[n] = xn;

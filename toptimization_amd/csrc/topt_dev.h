@@ -70,7 +70,7 @@ struct ProbDev {
   double kc[6];          // Ke = KS pattern x kc when kstruct (see q4_gather_s)
   int kstruct, pad_k;
   // FEM vectors (node interleaved). T = float (mixed) or double (plain).
-  void* x;   // CG iterate: plain = the solution; mixed = FP32 correction
+  void* x;   // CG iterate: aliases the FP64 solution u64 (both paths)
   void* r; void* z; void* p0; void* p1; void* Ap;
   void* pre; void* pre2;   // preconditioner data
   void* s;                 // element scale in T

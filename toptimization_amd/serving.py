@@ -102,9 +102,10 @@ class DesignServer:
         self.preconditioner = preconditioner
         self.keep_rho = keep_rho
 
-    def run(self, requests: list[Request], clock: str = "wall") -> ServerStats:
+    def run(self, requests: list[Request], clock: str = "wall", progress=None) -> ServerStats:
         """Serve the stream. clock='wall': arrivals are real time offsets from
-        the start; clock='instant': every request is available at t=0."""
+        the start; clock='instant': every request is available at t=0.
+        progress(msg) is called whenever a design finishes."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active: list[_Active] = []
         stats = ServerStats()
@@ -143,6 +144,9 @@ class DesignServer:
             out = plan.iteration(sensitivity_filter=sens[0])
             stats.steps += 1
             t = now()
+            if progress is not None and stats.steps % 25 == 1:
+                progress(f"step {stats.steps} at {t:.1f} s: {len(active)} active, max CG "
+                         f"{max(o.n_cg for o in out)}, {len(stats.results)} finished, {len(pending)} waiting")
             keep = []
             for a, s in zip(active, out):
                 a.it += 1
@@ -154,6 +158,11 @@ class DesignServer:
                         a.n_cg, a.state.rho.cpu().numpy() if self.keep_rho else None))
                     used -= footprint_bytes(pr, self.precision, self.preconditioner)
                     changed = True
+                    if progress is not None:
+                        r = stats.results[-1]
+                        progress(f"design {r.rid} ({pr.nelx}x{pr.nely}) done after {r.iterations} it, "
+                                 f"latency {r.latency:.2f} s, {len(stats.results)} finished, "
+                                 f"{len(pending)} waiting")
                 else:
                     keep.append(a)
             if len(keep) != len(active):
@@ -206,3 +215,25 @@ def synthetic_stream(n: int = 512, rate: float = 20.0, seed: int = 0, nelx_range
                             **solver)
         out.append(Request(i, prob, t))
     return out
+
+
+def batch_cantilevers(n: int = 64, nelx: int = 128, nely: int = 64, seed: int = 0, max_iter: int = 100,
+                      **solver) -> list[Problem]:
+    """Independent cantilevers for one ragged batch: left edge clamped, unit
+    point load on a seeded right-edge node with angle U[0, 2 pi), volfrac
+    U[0.3, 0.6], rmin 2.5, p 3 (problem i uses default_rng(seed + i))."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed + i)
+        iy = int(rng.integers(0, nely + 1))
+        th = float(rng.uniform(0.0, 2.0 * np.pi))
+        vf = float(rng.uniform(0.3, 0.6))
+        out.append(make_problem(
+            nelx, nely,
+            fixed=[{"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x", "y"]}],
+            loads=[{"nodes": {"type": "point", "location": [nelx, iy]},
+                    "force": [float(np.cos(th)), float(np.sin(th))]}],
+            name=f"cantilever-{i}", volume_fraction=vf, filter_radius=2.5, penalty=3.0,
+            max_iterations=max_iter, tolerance=0.0, **solver))
+    return out
+
