@@ -107,6 +107,13 @@ int topt_problem_init(const topt_problem* prob, int32_t dtype, void* stream);
 /* ---- batched engine: one launch per kernel advances every problem ---- */
 int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, topt_plan** out);
 int topt_plan_destroy(topt_plan* plan);
+
+/* Replay the PCG slot launches of every full chunk (32 CG iterations) from a
+ * HIP graph captured once per plan (on a private stream, kernels only),
+ * instead of enqueueing them one by one. Results are identical either way.
+ * Replaces the per-iteration host loop of fem/solver.py:107-156 further; no
+ * direct reference counterpart. Default off. */
+int topt_plan_set_graph(topt_plan* plan, int32_t enable);
 /* Density / sensitivity filter of every problem (filter.py:106-171). */
 int topt_plan_filter(topt_plan* plan, int32_t mode, void* stream);
 /* PCG solve K(rho_phys) u = f for every problem (fem/solver.py:27-159).

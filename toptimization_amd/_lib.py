@@ -50,6 +50,7 @@ _SIGS = {
     "topt_problem_init": (C.c_int, [C.POINTER(TProblem), _i32, _vp]),
     "topt_plan_create": (C.c_int, [C.POINTER(TProblem), _i32, _i32, C.POINTER(_vp)]),
     "topt_plan_destroy": (C.c_int, [_vp]),
+    "topt_plan_set_graph": (C.c_int, [_vp, C.c_int32]),
     "topt_plan_filter": (C.c_int, [_vp, _i32, _vp]),
     "topt_plan_solve": (C.c_int, [_vp, _i32, _i32, C.POINTER(_i32), C.POINTER(_i32), _vp]),
     "topt_plan_sensitivity": (C.c_int, [_vp, _i32, _vp]),

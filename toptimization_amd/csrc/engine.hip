@@ -342,6 +342,13 @@ struct topt_plan {
   topt_iter_stats* dstats = nullptr;
   topt_iter_stats* hstats = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // HIP graph of one full chunk of PCG slots (TOPT_GRAPH_CHUNK iterations),
+  // captured on a private stream the first time it is needed, replayed with
+  // hipGraphLaunch on the caller's stream
+  bool use_graph = false;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -381,9 +388,45 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
   return TOPT_OK;
 }
 
+#define TOPT_GRAPH_CHUNK 32
+
+// The slot launches of one full chunk, captured into a graph. Every full chunk
+// starts at an even sequence number and an even slot-A count (the ramp-up
+// chunks 4 + 4 + 8 + 16 = 32 iterations leave both even), so its kernel
+// arguments are the same each time and one graph serves the whole solve.
+int chunk_graph(topt_plan* pl) {
+  if (pl->gexec) return TOPT_OK;
+  if (!pl->cap) HIP_TRY(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
+  const int64_t q0 = pl->q, na0 = pl->na;
+  HIP_TRY(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
+  const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, TOPT_GRAPH_CHUNK, pl->cap)
+                                       : launch_slots<double>(pl, TOPT_GRAPH_CHUNK, pl->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(pl->cap, &g);
+  pl->q = q0;     // capturing launched nothing
+  pl->na = na0;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(TOPT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  pl->graph = g;
+  HIP_TRY(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0));
+  return TOPT_OK;
+}
+
 int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
-  const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, s) : launch_slots<double>(pl, count, s);
-  if (rc) return rc;
+  int rc;
+  if (pl->use_graph && count == TOPT_GRAPH_CHUNK && (pl->q & 1) == 0 && (pl->na & 1) == 0) {
+    if ((rc = chunk_graph(pl))) return rc;
+    HIP_TRY(hipGraphLaunch(pl->gexec, s));
+    const int per_it = pl->any_elemblock ? 3 : 2;
+    pl->q += (int64_t)per_it * TOPT_GRAPH_CHUNK;
+    pl->na += TOPT_GRAPH_CHUNK;
+  } else {
+    rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, s) : launch_slots<double>(pl, count, s);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n,
                      (int)(pl->q & 1), pl->dstat);
   LAUNCH_TRY();
@@ -657,7 +700,16 @@ int topt_plan_destroy(topt_plan* pl) {
     if (pl->ev[k]) (void)hipEventDestroy(pl->ev[k]);
   }
   if (pl->hstats) (void)hipHostFree(pl->hstats);
+  if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->cap) (void)hipStreamDestroy(pl->cap);
   delete pl;
+  return TOPT_OK;
+}
+
+int topt_plan_set_graph(topt_plan* pl, int32_t enable) {
+  if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
+  pl->use_graph = enable != 0;
   return TOPT_OK;
 }
 

@@ -146,7 +146,9 @@ class DesignState:
 class Plan:
     """A batch of DesignStates advanced by shared launches (``topt_plan_*``)."""
 
-    def __init__(self, states: list[DesignState]):
+    def __init__(self, states: list[DesignState], graph: bool = False):
+        """graph=True replays every full 32-iteration chunk of PCG launches
+        from a HIP graph (topt_plan_set_graph); results are identical."""
         if not states:
             raise ValueError("empty plan")
         dt = {s.dtype_id for s in states}
@@ -159,6 +161,8 @@ class Plan:
         check(_lib.load().topt_plan_create(self._descs, len(states), self.dtype_id, C.byref(handle)),
               "topt_plan_create")
         self.handle = handle
+        if graph:
+            check(_lib.load().topt_plan_set_graph(self.handle, 1), "topt_plan_set_graph")
 
     def close(self) -> None:
         if getattr(self, "handle", None):
