@@ -229,7 +229,7 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32):
         ss = one.run(sub, clock="instant", progress=log).summary()
         seq = ss["designs_per_hour"]
         log(f"  sequential, first {seq_sample} requests back to back: {seq:.0f} designs/h")
-    return {"metric": "serving designs/hour (512-request Poisson stream)", "value": round(s["designs_per_hour"], 1),
+    return {"metric": f"serving designs/hour ({n}-request Poisson stream)", "value": round(s["designs_per_hour"], 1),
             "unit": "designs/hour", "p50_latency_s": round(s["p50_latency_s"], 3),
             "p99_latency_s": round(s["p99_latency_s"], 3), "makespan_s": round(s["makespan_s"], 2),
             "requests": n, "arrival_rate_per_s": rate, "max_active": s["max_active"],
