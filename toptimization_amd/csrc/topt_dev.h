@@ -149,17 +149,45 @@ template <> struct KeSel<double> {
 };
 
 // ------------------------------------------------------------ reductions
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = TOPT_WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TOPT_WAVE);
-  return v;
+// Wavefront reductions of doubles with DPP lane moves (VALU, no LDS
+// round trip): pairs (quad_perm 1,0,3,2), quads (2,3,0,1), half rows
+// (row_half_mirror), rows (row_mirror), then row_bcast:15 into rows 1 and 3
+// and row_bcast:31 into rows 2 and 3, which leaves the total in lane 63.
+// Fixed combination order: bitwise reproducible. Every lane must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v, double ident) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(ident);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int o = TOPT_WAVE / 2; o > 0; o >>= 1) { T w = __shfl_xor(v, o, TOPT_WAVE); v = v > w ? v : w; }
-  return v;
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// Sum over the wavefront, returned to every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0xB1, 0xF>(v, 0.0);    // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xF>(v, 0.0);    // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xF>(v, 0.0);   // row_half_mirror
+  v += dpp_f64<0x140, 0xF>(v, 0.0);   // row_mirror
+  v += dpp_f64<0x142, 0xA>(v, 0.0);   // row_bcast:15 -> rows 1, 3
+  v += dpp_f64<0x143, 0xC>(v, 0.0);   // row_bcast:31 -> rows 2, 3
+  return lane63(v);
+}
+// Maximum over the wavefront, returned to every lane.
+__device__ __forceinline__ double wave_max(double v) {
+  const double lo = -__builtin_huge_val();
+  double w;
+  w = dpp_f64<0xB1, 0xF>(v, lo);  v = v > w ? v : w;
+  w = dpp_f64<0x4E, 0xF>(v, lo);  v = v > w ? v : w;
+  w = dpp_f64<0x141, 0xF>(v, lo); v = v > w ? v : w;
+  w = dpp_f64<0x140, 0xF>(v, lo); v = v > w ? v : w;
+  w = dpp_f64<0x142, 0xA>(v, lo); v = v > w ? v : w;
+  w = dpp_f64<0x143, 0xC>(v, lo); v = v > w ? v : w;
+  return lane63(v);
 }
 
 // Sum NV values over the block; result valid in thread 0. Fixed order.
