@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per slot-A launch at 1200x400 (FP32), measured with rocprofv3
+# HBM bytes per slot-A launch at 1200x400 (FP32, CG step: the replay applies
+# no deferred solution update), measured with rocprofv3
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --probe`, each scaled
 # by the counter's reading for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
 # profiles/pcg_slots_bridge1200x400_mixed.md. Re-measure when the kernel changes.

@@ -295,6 +295,7 @@ __global__ void k_cg_force_run(const ProbDev* tab, int n) {
   st->cflag = CF_CG;
   st->amode = OP_A_CG;
   st->pending = 0;
+  st->upend = 0;
   st->it = 0;
   if (!(st->rz > 0.0)) st->rz = 1.0;
 }

@@ -567,8 +567,16 @@ __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)
 // block sum in lane and wave order. The sum is formed unconditionally (it is
 // unused after OP_START or when the problem has stopped) so that the partial
 // loads stay ahead of the state load instead of being sunk behind it.
-template <typename T>
-__device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevParts& pp, double* sh) {
+// The solution update u += alpha p of a B_CG step is deferred to the next
+// slot A, which holds p at its nodes anyway (slot B then streams only r, Ap
+// and M). The flag upend marks it. Slot A applies it in A_CG mode and when
+// the solve has stopped; in A_REF mode neighbours read u, so the refresh
+// uses u + alpha p on the fly and B_REF applies the update. *upd (out): this
+// kernel must apply u += alpha p at its own nodes (alpha from the state).
+enum SlotKind : int { SK_A = 0, SK_B = 1, SK_C = 2 };
+template <typename T, int K>
+__device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevParts& pp, double* sh,
+                                              bool* upd) {
   const CST CgState* cgs = cp((const CgState*)P.cg);
   CgState s;
   {
@@ -577,8 +585,9 @@ __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevPa
     s.pAp = c.pAp; s.rr = c.rr; s.rr_true = c.rr_true; s.rmax2 = c.rmax2;
     s.it = c.it; s.status = c.status; s.op = c.op; s.pending = c.pending;
     s.pad3 = 0; s.first = c.first; s.refines = c.refines; s.amode = c.amode;
-    s.cflag = c.cflag; s.pad0 = s.pad1 = s.pad2 = 0;
+    s.cflag = c.cflag; s.upend = c.upend; s.pad1 = s.pad2 = 0;
   }
+  const bool was_running = s.status == ST_RUN;
   // keep every load issued so far (partials, state, the caller's
   // state-independent data) ahead of the first wait
   __builtin_amdgcn_sched_barrier(0);
@@ -606,7 +615,18 @@ __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevPa
   __syncthreads();
   t[0] = sh[0];
   t[1] = sh[1];
-  if (s.status == ST_RUN) pcg_advance<T>(P, s, t);
+  if (was_running) pcg_advance<T>(P, s, t);
+  const bool pend = was_running && s.upend;
+  const bool run = s.status == ST_RUN;
+  if (K == SK_A) {
+    *upd = pend && !(run && s.op == OP_A_REF);
+    s.upend = (pend && !*upd) ? 1 : 0;
+  } else if (K == SK_B) {
+    *upd = pend && !(run && s.op == OP_B_CG);
+    s.upend = (run && s.op == OP_B_CG) ? 1 : 0;
+  } else {
+    *upd = false;
+  }
   if (lb == 0 && threadIdx.x == 0) P.cg[(q + 1) & 1] = s;
   return s;
 }
@@ -621,16 +641,27 @@ __device__ __forceinline__ void slot_end(PD& P, int lb, int q, double v0, double
   }
 }
 
-// A_REF: r = f - A u in FP64 (stored in T); partial |r|^2 (FP64).
-// Rare (a few times per solve): direct loads.
+// A_REF: r = f - A (u [+ alpha p]) in FP64 (stored in T); partial |r|^2
+// (FP64). With a deferred update (upd) the neighbours' u + alpha p is formed
+// on the fly. Rare (a few times per solve): direct loads.
 template <typename T>
-__device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id) {
+__device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id, bool upd, double alpha,
+                                                        const GLB vec2<T>* pold) {
   const GLB d2v* u = gp((const d2v*)P.u64);
   d2v q[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy) q[dx][dy] = u[nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely)];
+    for (int dy = 0; dy < 3; ++dy) {
+      const long m = nbr_node(id.ix, id.iy, dx - 1, dy - 1, P.nelx, P.nely);
+      d2v v = u[m];
+      if (upd) {
+        const vec2<T> c = pold[m];
+        v.x = fma(alpha, (double)c.x, v.x);
+        v.y = fma(alpha, (double)c.y, v.y);
+      }
+      q[dx][dy] = v;
+    }
   double sc[2][2];
   load_scales<double>(P.s64, id.ix, id.iy, P.nelx, P.nely, sc);
   double ax, ay;
@@ -678,20 +709,36 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const
   parts_fetch(P, q, pp);
   TileShare<T> S;
   tile_share<T>(P, o, S);
+  GLB d2v* u = gp((d2v*)P.u64);
   vec2<T> oz[NPT], op[NPT];
+  d2v ou[NPT];
 #pragma unroll
   for (int j = 0; j < NPT; ++j) {
     oz[j] = z[S.ids[j].n];
     op[j] = pold[S.ids[j].n];
+    ou[j] = u[S.ids[j].n];
   }
   const vec2<T> rz = z[S.rn], rp = pold[S.rn];
-  const CgState st = slot_begin<T>(P, lb, q, pp, sh);
+  bool upd;
+  const CgState st = slot_begin<T, SK_A>(P, lb, q, pp, sh, &upd);
+  if (upd) {
+    // deferred u += alpha p of the last B_CG (p_old is that step's p)
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+      if (!S.ids[j].act) continue;
+      d2v un = ou[j];
+      un.x = fma(st.alpha, (double)op[j].x, un.x);
+      un.y = fma(st.alpha, (double)op[j].y, un.y);
+      u[S.ids[j].n] = un;
+    }
+  }
   if (st.status != ST_RUN) return;
   double v = 0.0;
   if (st.op == OP_A_REF) {
+    const bool later = st.upend != 0;   // B_REF applies it; form it on the fly
 #pragma unroll
     for (int j = 0; j < NPT; ++j) {
-      v += slot_refresh_residual<T>(P, S.ids[j]);
+      v += slot_refresh_residual<T>(P, S.ids[j], later, st.alpha, pold);
       if (S.ids[j].act) pnew[S.ids[j].n] = op[j];
     }
   } else {
@@ -765,31 +812,42 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
   const int pc = P.precond;
   GLB vec2<T>* r = gp((vec2<T>*)P.r);
   GLB vec2<T>* z = gp((vec2<T>*)P.z);
-  GLB d2v* x = gp((d2v*)P.u64);
   const GLB vec2<T>* Ap = gp((const vec2<T>*)P.Ap);
   // state-independent loads first
   PrevParts pp;
   parts_fetch(P, q, pp);
-  // p as written by the preceding slot A (par = parity of that buffer)
-  const GLB vec2<T>* p = gp((const vec2<T>*)(par ? P.p1 : P.p0));
   NodeIdx ids[Tl::NPT];
-  vec2<T> rv[Tl::NPT], av[Tl::NPT], dv[Tl::NPT], pv[Tl::NPT];
-  d2v xv[Tl::NPT];
+  vec2<T> rv[Tl::NPT], av[Tl::NPT], dv[Tl::NPT];
   T ov[Tl::NPT];
 #pragma unroll
   for (int j = 0; j < Tl::NPT; ++j) {
     ids[j] = tile_node<T>(P, o, j);
     const long n = ids[j].n;
-    pv[j] = p[n];
     rv[j] = r[n];
-    xv[j] = x[n];
     av[j] = Ap[n];
     dv[j] = vec2<T>{T(0), T(0)};
     ov[j] = T(0);
     if (pc == PC_JACOBI || pc == PC_BLOCK2) dv[j] = (gp((const vec2<T>*)P.pre))[n];
     if (pc == PC_BLOCK2) ov[j] = (gp((const T*)P.pre2))[n];
   }
-  const CgState st = slot_begin<T>(P, lb, q, pp, sh);
+  bool upd;
+  const CgState st = slot_begin<T, SK_B>(P, lb, q, pp, sh, &upd);
+  if (upd) {
+    // deferred u += alpha p of the last B_CG (after a refresh, or at the stop);
+    // p is the buffer the preceding slot A wrote (par)
+    const GLB vec2<T>* p = gp((const vec2<T>*)(par ? P.p1 : P.p0));
+    GLB d2v* u = gp((d2v*)P.u64);
+#pragma unroll
+    for (int j = 0; j < Tl::NPT; ++j) {
+      if (!ids[j].act) continue;
+      const long n = ids[j].n;
+      const vec2<T> pj = p[n];
+      d2v un = u[n];
+      un.x = fma(st.alpha, (double)pj.x, un.x);
+      un.y = fma(st.alpha, (double)pj.y, un.y);
+      u[n] = un;
+    }
+  }
   if (st.status != ST_RUN) return;
   const bool eb = pc == PC_ELEMBLOCK;
   double v0 = 0.0, v1 = 0.0;
@@ -810,13 +868,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
     for (int j = 0; j < Tl::NPT; ++j) {
       if (!ids[j].act) continue;
       const long n = ids[j].n;
-      d2v xn = xv[j];
       vec2<T> rn = rv[j];
-      xn.x = fma((double)alpha, (double)pv[j].x, xn.x);
-      xn.y = fma((double)alpha, (double)pv[j].y, xn.y);
       rn.x -= alpha * av[j].x;
       rn.y -= alpha * av[j].y;
-      x[n] = xn;
       r[n] = rn;
       v1 += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
       if (!eb) {
@@ -842,7 +896,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
   const TileOrg o = tile_org<T>(P, lb);
   PrevParts pp;
   parts_fetch(P, q, pp);
-  const CgState st = slot_begin<T>(P, lb, q, pp, sh);
+  bool upd;
+  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
   if (st.status != ST_RUN) return;
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
   double v = 0.0;

@@ -34,7 +34,7 @@ struct CgState {
   double rz, rz_prev, alpha, beta, pAp, rr, rr_true, rmax2;
   int it, status, op, pending;   // op: what the kernel that wrote this state executed
   int pad3, first, refines, amode;
-  int cflag, pad0, pad1, pad2;
+  int cflag, upend, pad1, pad2;   // upend: u += alpha p of the last B_CG not applied yet
 };
 // PCG slot operations (the state machine of fem/solver.py:27-159 on the device).
 enum SlotOp : int { OP_START = 0, OP_A_CG = 1, OP_A_REF = 2, OP_B_CG = 3, OP_B_REF = 4, OP_C = 5 };
