@@ -43,3 +43,33 @@ def test_admission_respects_memory_budget():
     stats = server.run(reqs, clock="instant")
     assert stats.summary()["designs"] == 4
     assert stats.max_active == 2
+
+
+def test_cli_serve_jsonl(tmp_path):
+    """`toptimization serve requests.jsonl`: JSON request format in, one
+    density file per design and a summary out."""
+    import json
+
+    from toptimization_amd.cli import main
+    reqs = []
+    for i, (nelx, nely) in enumerate([(24, 8), (32, 12)]):
+        reqs.append({"id": 100 + i, "arrival": 0.0,
+                     "problem": {"problem": {"name": f"c{i}"}, "domain": {"nelx": nelx, "nely": nely},
+                                 "boundary_conditions": {"fixed": [
+                                     {"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x", "y"]}]},
+                                 "loads": [{"nodes": {"type": "point", "location": [nelx, 0]},
+                                            "force": [0.0, -1.0]}],
+                                 "optimization": {"volume_fraction": 0.5, "filter_radius": 1.5,
+                                                  "max_iterations": 6, "tolerance": 0.0}}})
+    f = tmp_path / "reqs.jsonl"
+    f.write_text("\n".join(json.dumps(r) for r in reqs) + "\n")
+    out = tmp_path / "out"
+    assert main(["serve", str(f), "--output", str(out), "--max-active", "2"]) == 0
+    summary = json.loads((out / "summary.json").read_text())
+    assert summary["summary"]["designs"] == 2
+    assert sorted(d["id"] for d in summary["designs"]) == [100, 101]
+    for d in summary["designs"]:
+        rho = np.load(out / f"{d['id']}_density.npy")
+        assert rho.shape == (d["nelx"], d["nely"])
+        assert d["iterations"] == 6
+        assert abs(rho.mean() - 0.5) < 1e-3

@@ -24,9 +24,19 @@ def main(argv=None) -> int:
     run_p.add_argument("--precision", choices=["fp64", "mixed"])
     val_p = sub.add_parser("validate", help="Validate a problem file without running")
     val_p.add_argument("problem")
+    srv_p = sub.add_parser("serve", help="Serve a JSON-lines file of design requests (continuous batching)")
+    srv_p.add_argument("requests", help="one JSON request per line: {id, arrival, problem: {...}}")
+    srv_p.add_argument("--output", metavar="DIR", default="serve_results")
+    srv_p.add_argument("--max-active", type=int, default=64)
+    srv_p.add_argument("--hbm-budget-gb", type=float, default=64.0)
+    srv_p.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    srv_p.add_argument("--clock", choices=["wall", "instant"], default="instant",
+                       help="wall: honour arrival times; instant: all requests available at t=0")
     args = ap.parse_args(argv)
     if args.command == "run":
         return _cmd_run(args)
+    if args.command == "serve":
+        return _cmd_serve(args)
     if args.command == "validate":
         return _cmd_validate(args)
     ap.print_help()
@@ -89,6 +99,38 @@ def _cmd_run(args) -> int:
         except Exception as e:
             print(f"[warn] Could not show result: {e}")
     print(f"\nResults saved to: {problem.output_dir}/")
+    return 0
+
+
+def _cmd_serve(args) -> int:
+    import json
+
+    import numpy as np
+
+    from .serving import DesignServer, load_requests
+    try:
+        reqs = load_requests(args.requests)
+    except Exception as e:
+        print(f"Error reading requests: {e}", file=sys.stderr)
+        return 1
+    out = Path(args.output)
+    out.mkdir(parents=True, exist_ok=True)
+    server = DesignServer(max_active=args.max_active, hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
+                          precision=args.precision, keep_rho=True)
+    stats = server.run(reqs, clock=args.clock, progress=lambda m: print(m, flush=True))
+    by_id = {r.rid: r for r in reqs}
+    rows = []
+    for res in stats.results:
+        p = by_id[res.rid].problem
+        np.save(out / f"{res.rid}_density.npy", res.rho.astype(np.float32).reshape(p.nelx, p.nely))
+        rows.append({"id": res.rid, "nelx": p.nelx, "nely": p.nely, "iterations": res.iterations,
+                     "compliance": res.compliance, "volume": res.volume, "change": res.change,
+                     "cg_iterations": res.n_cg, "arrival_s": res.arrival, "finished_s": res.finished,
+                     "latency_s": res.latency})
+    summary = {"summary": stats.summary(), "designs": rows}
+    (out / "summary.json").write_text(json.dumps(summary, indent=1))
+    print(json.dumps(stats.summary()))
+    print(f"Results saved to: {out}/")
     return 0
 
 

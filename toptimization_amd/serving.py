@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _lib
 from .engine import DesignState, Plan, PRECISION_IDS
-from .problem import Problem, make_problem
+from .problem import Problem, make_problem, problem_from_dict
 
 
 @dataclass
@@ -52,6 +52,35 @@ class Result:
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
+
+
+def request_from_json(obj, default_id: int = 0) -> Request:
+    """One serving request from its JSON form (a dict or a JSON string):
+
+        {"id": 7, "arrival": 0.35,
+         "problem": {<the YAML problem schema: problem, domain, material,
+                     boundary_conditions, loads, optimization, solver>}}
+
+    ``arrival`` (seconds from the stream start) defaults to 0 and ``id`` to
+    ``default_id``; the problem mapping is validated exactly like a YAML file
+    (reference ``problem.py:87-164``)."""
+    import json
+    if isinstance(obj, (str, bytes)):
+        obj = json.loads(obj)
+    if not isinstance(obj, dict) or "problem" not in obj:
+        raise ValueError("a request is a JSON object with a 'problem' mapping")
+    return Request(int(obj.get("id", default_id)), problem_from_dict(obj["problem"]),
+                   float(obj.get("arrival", 0.0)))
+
+
+def load_requests(path) -> list[Request]:
+    """Requests from a JSON-lines file (one request object per line)."""
+    out = []
+    with open(path) as fh:
+        for k, line in enumerate(fh):
+            if line.strip():
+                out.append(request_from_json(line, default_id=k))
+    return out
 
 
 def footprint_bytes(problem: Problem, precision: str, preconditioner: str | None = None) -> int:
