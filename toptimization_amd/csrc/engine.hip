@@ -307,11 +307,11 @@ __global__ void k_cg_finish(const ProbDev* tab, int n) {
   tab[i].cg[1].status = ST_CONV;
 }
 
-// After a solve: make cg[0] hold the final state (the last slot wrote
-// cg[q & 1]).
-__global__ void k_cg_settle(const ProbDev* tab, int n, int parity) {
+// After a solve: make cg[0] hold the final state (the last slot that covered
+// problem i wrote cg[parity[i]]).
+__global__ void k_cg_settle(const ProbDev* tab, int n, const int* parity) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || parity == 0) return;
+  if (i >= n || parity[i] == 0) return;
   tab[i].cg[0] = tab[i].cg[1];
 }
 
@@ -350,6 +350,19 @@ struct topt_plan {
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // PCG grid map of (problem, local tile) pairs: the full map, and two
+  // compacted copies (alternating, each filled from its own pinned staging
+  // buffer) that leave out problems whose solve has ended
+  int2* dpfull = nullptr;
+  int2* dpmap[2] = {nullptr, nullptr};
+  int2* hpmap[2] = {nullptr, nullptr};
+  const int2* pcur = nullptr;   // map of the next launches
+  int npcur = 0;                // its block count
+  int pslot = 0;
+  std::vector<int2> pfull;      // host copy of the full map
+  std::vector<char> live;       // problem still in the PCG map
+  int* hpar = nullptr;          // pinned: parity of each problem's final state
+  int* dpar = nullptr;
 };
 
 namespace {
@@ -365,9 +378,13 @@ GridRef grid_of(const topt_plan* pl, int g) { return GridRef{pl->nblk[g], pl->dm
 template <typename T>
 int launch_slots(topt_plan* pl, int count, hipStream_t s) {
   const ProbDev dummy{};
-  const GridRef gr = grid_of(pl, G_PCG);
-  const int nb = gr.nb;
-  const int* map = gr.map;
+  const int nb = pl->npcur;
+  const int2* map = pl->pcur;
+  if (nb == 0) {   // every solve has ended: the sequence numbers still advance
+    pl->q += (int64_t)(pl->any_elemblock ? 3 : 2) * count;
+    pl->na += count;
+    return TOPT_OK;
+  }
   for (int k = 0; k < count; ++k) {
     // slot A number na reads p buffer (na & 1) and writes the other one
     const int pa = (int)(pl->na & 1);
@@ -418,7 +435,8 @@ int chunk_graph(topt_plan* pl) {
 
 int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   int rc;
-  if (pl->use_graph && count == TOPT_GRAPH_CHUNK && (pl->q & 1) == 0 && (pl->na & 1) == 0) {
+  if (pl->use_graph && count == TOPT_GRAPH_CHUNK && (pl->q & 1) == 0 && (pl->na & 1) == 0 &&
+      pl->pcur == pl->dpfull) {
     if ((rc = chunk_graph(pl))) return rc;
     HIP_TRY(hipGraphLaunch(pl->gexec, s));
     const int per_it = pl->any_elemblock ? 3 : 2;
@@ -434,12 +452,6 @@ int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   HIP_TRY(hipMemcpyAsync(pl->hstat[slot], pl->dstat, sizeof(int) * 2 * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipEventRecord(pl->ev[slot], s));
   return TOPT_OK;
-}
-
-bool any_running(const topt_plan* pl, const int* st) {
-  for (int i = 0; i < pl->n; ++i)
-    if (st[2 * i] == ST_RUN) return true;
-  return false;
 }
 
 template <typename T>
@@ -490,6 +502,9 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   if (rc) return rc;
   pl->q = 0;
   pl->na = 0;
+  pl->pcur = pl->dpfull;
+  pl->npcur = pl->nblk[G_PCG];
+  std::fill(pl->live.begin(), pl->live.end(), 1);
   // slots issued so far; a solve needs at most max_cg CG slots plus refresh slots
   const int64_t slot_cap = 4 * int64_t(pl->max_cg) + 4096;
   int64_t issued = 0;
@@ -502,17 +517,44 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   int wait = 0;
   for (;;) {
     HIP_TRY(hipEventSynchronize(pl->ev[wait]));
-    if (!any_running(pl, pl->hstat[wait])) break;
+    // the snapshot is one chunk old: a problem that had stopped then was
+    // still covered by the chunk in flight, so its final state is in
+    // cg[q & 1] for the current q; from the next chunk on it is left out
+    bool running = false, dropped = false;
+    for (int i = 0; i < pl->n; ++i) {
+      if (!pl->live[i]) continue;
+      if (pl->hstat[wait][2 * i] == ST_RUN) {
+        running = true;
+      } else if (!pl->use_graph) {
+        pl->live[i] = 0;
+        pl->hpar[i] = (int)(pl->q & 1);
+        dropped = true;
+      }
+    }
+    if (!running) break;
     if (issued > slot_cap) return fail(TOPT_EINVAL, "PCG slot loop did not terminate");
+    if (dropped) {
+      const int k = pl->pslot ^= 1;
+      int m = 0;
+      for (const int2& e : pl->pfull)
+        if (pl->live[e.x]) pl->hpmap[k][m++] = e;
+      if (m > 0)
+        HIP_TRY(hipMemcpyAsync(pl->dpmap[k], pl->hpmap[k], sizeof(int2) * m, hipMemcpyHostToDevice, s));
+      pl->pcur = pl->dpmap[k];
+      pl->npcur = m;
+    }
     chunk = std::min(chunk * 2, 32);
     rc = enqueue_chunk(pl, chunk, wait, s);
     if (rc) return rc;
     issued += chunk;
     wait ^= 1;
   }
-  // drain the chunk still in flight and take a final snapshot
+  // drain the chunk still in flight; problems still in the map end at q & 1
   HIP_TRY(hipEventSynchronize(pl->ev[wait ^ 1]));
-  hipLaunchKernelGGL(k_cg_settle, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, (int)(pl->q & 1));
+  for (int i = 0; i < pl->n; ++i)
+    if (pl->live[i]) pl->hpar[i] = (int)(pl->q & 1);
+  HIP_TRY(hipMemcpyAsync(pl->dpar, pl->hpar, sizeof(int) * pl->n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_cg_settle, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dpar);
   pl->q = 0;
   pl->na = 0;
   hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
@@ -686,6 +728,26 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   }
   if (hipHostMalloc(&pl->hstats, sizeof(topt_iter_stats) * nprob, hipHostMallocDefault) != hipSuccess)
     return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc stats"));
+  // PCG grid: (problem, local tile) pairs in the order of the block map
+  const int npcg = pl->nblk[G_PCG];
+  pl->pfull.resize(npcg);
+  for (int i = 0, b = 0; i < nprob; ++i)
+    for (int t = 0; t < pl->host[i].nb[G_PCG]; ++t) pl->pfull[b++] = make_int2(i, t);
+  pl->live.assign(nprob, 1);
+  if (hipMalloc(&pl->dpfull, sizeof(int2) * npcg) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc PCG map"));
+  if (hipMemcpy(pl->dpfull, pl->pfull.data(), sizeof(int2) * npcg, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(TOPT_EHIP, "upload PCG map"));
+  for (int k = 0; k < 2; ++k) {
+    if (hipMalloc(&pl->dpmap[k], sizeof(int2) * npcg) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipMalloc PCG map"));
+    if (hipHostMalloc(&pl->hpmap[k], sizeof(int2) * npcg, hipHostMallocDefault) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc PCG map"));
+  }
+  if (hipHostMalloc(&pl->hpar, sizeof(int) * nprob, hipHostMallocDefault) != hipSuccess)
+    return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc parity"));
+  if (hipMalloc(&pl->dpar, sizeof(int) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc parity"));
+  pl->pcur = pl->dpfull;
+  pl->npcur = npcg;
   *out = pl;
   return TOPT_OK;
 }
@@ -701,6 +763,13 @@ int topt_plan_destroy(topt_plan* pl) {
     if (pl->ev[k]) (void)hipEventDestroy(pl->ev[k]);
   }
   if (pl->hstats) (void)hipHostFree(pl->hstats);
+  (void)hipFree(pl->dpfull);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(pl->dpmap[k]);
+    if (pl->hpmap[k]) (void)hipHostFree(pl->hpmap[k]);
+  }
+  if (pl->hpar) (void)hipHostFree(pl->hpar);
+  (void)hipFree(pl->dpar);
   if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
   if (pl->cap) (void)hipStreamDestroy(pl->cap);
@@ -755,9 +824,8 @@ int topt_plan_time_slot(topt_plan* pl, int32_t which, int32_t reps, float* ms, v
     return fail(TOPT_EINVAL, "invalid slot timing call (only which = 0 is supported)");
   hipStream_t s = (hipStream_t)stream;
   const ProbDev dummy{};
-  const GridRef gr = grid_of(pl, G_PCG);
-  const int nb = gr.nb;
-  const int* map = gr.map;
+  const int nb = pl->nblk[G_PCG];
+  const int2* map = pl->dpfull;
   // every launch re-reads cg[0] (forced to "after slot C") and repeats one A_CG step
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));

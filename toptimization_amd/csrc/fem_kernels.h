@@ -187,6 +187,21 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
 }
 
 // ---------------------------------------------------------------- PCG slots
+// The PCG grid's block map holds (problem, local tile) pairs, so the host can
+// drop the tiles of problems whose solve has ended between chunks.
+template <bool B>
+__device__ __forceinline__ PD& pcg_prob(const ProbDev* tab, const int2* pmap, const ProbDev& single, int b,
+                                        int& lb) {
+  if constexpr (B) {
+    const int2 e = pmap[b];
+    lb = e.y;
+    return *(PD*)(tab + e.x);
+  } else {
+    lb = b;
+    return *(PD*)&single;
+  }
+}
+
 // The host launches slot kernels A, B[, C], A, B[, C], ... with sequence
 // numbers q = 0, 1, 2, ... Kernel q
 //   1. reads the state cg[q & 1] written by kernel q-1,
@@ -690,7 +705,7 @@ __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id
 // Ap (an FP32 Ke and FP32 sums break the rigid-body cancellation and stall
 // the FP64 residual refresh).
 template <typename T, bool B, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const int2* pmap,
                                                        ProbDev single, int q, int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
@@ -698,8 +713,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const
   __shared__ double esc[Tl::EN];
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  const int lb = b - P.b0[G_PCG];
+  int lb;
+  PD& P = pcg_prob<B>(tab, pmap, single, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* pold = gp((const vec2<T>*)(par ? P.p1 : P.p0));
   GLB vec2<T>* pnew = gp((vec2<T>*)(par ? P.p0 : P.p1));
@@ -801,13 +816,13 @@ __device__ __forceinline__ vec2<T> precond_apply(int pc, vec2<T> d, T o, vec2<T>
 // CPU restatement: a 608x152 MBB stalls at 9e-6 with an FP32 iterate and
 // converges in the FP64 iteration count with an FP64 one).
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const int2* pmap,
                                                        ProbDev single, int q, int par) {
   using Tl = Tile<T>;
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  const int lb = b - P.b0[G_PCG];
+  int lb;
+  PD& P = pcg_prob<B>(tab, pmap, single, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   const int pc = P.precond;
   GLB vec2<T>* r = gp((vec2<T>*)P.r);
@@ -886,13 +901,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
 // Slot C (element-block Jacobi only): z = M^-1 r by gather, partial r.z
 // (fem/preconditioner.py:47-108 apply_and_rz_2d).
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int2* pmap,
                                                        ProbDev single, int q, int par) {
   using Tl = Tile<T>;
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  const int lb = b - P.b0[G_PCG];
+  int lb;
+  PD& P = pcg_prob<B>(tab, pmap, single, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   PrevParts pp;
   parts_fetch(P, q, pp);
