@@ -31,12 +31,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per slot-A launch at 1200x400 (FP32, CG step: the replay applies
-# no deferred solution update), measured with rocprofv3
-# --pmc FETCH_SIZE and --pmc WRITE_SIZE over `bench.py --probe`, each scaled
-# by the counter's reading for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
-# profiles/pcg_slots_bridge1200x400_mixed.md. Re-measure when the kernel changes.
-SLOT_A_TRAFFIC_BYTES = 17.93e6
+# HBM bytes per slot-A launch at 1200x400 (FP32 CG step with the deferred
+# solution update), measured with rocprofv3 --pmc FETCH_SIZE and --pmc
+# WRITE_SIZE over `bench.py --probe`, each scaled by the counter's reading for
+# a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
+# profiles/pcg_slots_bridge1200x400_mixed.md. None until measured for the
+# current kernel.
+SLOT_A_TRAFFIC_BYTES = None
 NELX, NELY = 1200, 400
 
 
@@ -94,9 +95,10 @@ def kernel_roofline(plan, nn, ne, log, reps=200):
     """Average duration of PCG slot A (k_slot_a<float>) over back-to-back
     launches, timed with HIP events on the launch stream."""
     ms = plan.time_slot(0, reps)
-    # FP32 slot A per node: read z, p_old (8 B each), write p_new, Ap (8 B each);
+    # FP32 slot A per node: read z, p_old (8 B each), write p_new, Ap (8 B each),
+    # read and write the FP64 solution u (deferred u += alpha p: 16 B each);
     # per element: read s_e (4 B)
-    bytes_a = 32 * nn + 4 * ne
+    bytes_a = 64 * nn + 4 * ne
     gbs = bytes_a / (ms * 1e-3) / 1e9
     log(f"  slot A: {ms * 1e3:.2f} us/launch, {bytes_a / 1e6:.2f} MB -> {gbs:.0f} GB/s")
     return ms, bytes_a, gbs
@@ -205,18 +207,20 @@ def batch_bench(log, n=64, iters=100, seq_sample=None):
             "cg_iterations_batched": ncg, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
 
 
-def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32):
+def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="async"):
     """Serving stream: n seeded requests, Poisson arrivals at `rate`/s, nelx
     uniform over multiples of 16 in [64, 768], aspect 1/4, 1/3 or 1/2, BC
     cantilever / MBB / bridge, volfrac U[0.3, 0.5], rmin 0.008 nelx + 1, run to
     100 OC iterations or max change < 0.01; continuous batching with HBM
-    admission. Sequential baseline: the first seq_sample requests of the same
-    stream served one at a time (designs/hour of that sample)."""
+    admission (mode 'async': every design runs its own SIMP iterations;
+    'lockstep': one iteration of every active design per engine step).
+    Sequential baseline: the first seq_sample requests of the same stream
+    served one at a time (designs/hour of that sample)."""
     from toptimization_amd.serving import DesignServer, synthetic_stream
     solver = dict(cg_tolerance=1e-6, max_cg_iterations=50000, preconditioner="block_jacobi_2x2",
                   precision="mixed")
     reqs = synthetic_stream(n=n, rate=rate, seed=0, nelx_range=(64, 768), max_iter=100, tol=0.01, **solver)
-    server = DesignServer(max_active=max_active, hbm_budget=64 << 30, precision="mixed")
+    server = DesignServer(max_active=max_active, hbm_budget=64 << 30, precision="mixed", mode=mode)
     st = server.run(reqs, clock="wall", progress=log)
     s = st.summary()
     log(f"  serving {n} requests at {rate}/s: makespan {s['makespan_s']:.1f} s, "
@@ -233,7 +237,7 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32):
     return {"metric": f"serving designs/hour ({n}-request Poisson stream)", "value": round(s["designs_per_hour"], 1),
             "unit": "designs/hour", "p50_latency_s": round(s["p50_latency_s"], 3),
             "p99_latency_s": round(s["p99_latency_s"], 3), "makespan_s": round(s["makespan_s"], 2),
-            "requests": n, "arrival_rate_per_s": rate, "max_active": s["max_active"],
+            "requests": n, "arrival_rate_per_s": rate, "max_active": s["max_active"], "mode": mode,
             "engine_steps": s["engine_steps"], "sequential_designs_per_hour": None if seq is None else round(seq, 1),
             "sequential_sample": seq_sample, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
 
@@ -253,6 +257,7 @@ def main() -> int:
                     help="512-request serving stream: designs/hour, p50/p99 latency (own JSON line)")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--serving-mode", choices=["async", "lockstep"], default="async")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a design problem is "
@@ -267,7 +272,7 @@ def main() -> int:
         print(json.dumps(batch_bench(log)), flush=True)
         return 0
     if args.serving:
-        print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample)), flush=True)
+        print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample, mode=args.serving_mode)), flush=True)
         return 0
     if args.probe:
         from toptimization_amd.engine import DesignState, Plan

@@ -42,6 +42,7 @@ extern "C" {
 #define TOPT_ST_BREAK 2    /* |pAp| < 1e-30 (solver.py:114-115): reports max_iter */
 #define TOPT_ST_MAXIT 3
 #define TOPT_ST_ZERO 4     /* |f| < 1e-30 (solver.py:60-63): u = 0, 0 iterations */
+#define TOPT_ST_DONE 5     /* topt_plan_advance: the design's OC loop has stopped */
 
 /* error codes */
 #define TOPT_OK 0
@@ -77,6 +78,9 @@ typedef struct topt_problem {
   double delta;          /* mixed path: refresh residual when |r| drops by this factor (0.1) */
   double Ke[64];         /* element stiffness, row-major, E folded in (material.py:20-86) */
   void* ws;              /* device workspace of topt_problem_layout() bytes */
+  double oc_tol;         /* OC stop: it > 5 and change < oc_tol, or it == oc_max_iter */
+  int32_t oc_max_iter;   /* (optimizer/simp.py:217; used by topt_plan_advance) */
+  int32_t pad;
 } topt_problem;
 
 typedef struct topt_plan topt_plan;
@@ -91,6 +95,8 @@ typedef struct topt_iter_stats {
   int32_t cg_status;   /* TOPT_ST_* */
   int32_t oc_steps;    /* bisection steps taken (<= 50)      (oc.py:71) */
   int32_t refines;     /* FP64 residual refreshes (mixed path) */
+  int32_t iter;        /* OC iterations completed */
+  int32_t done;        /* the OC stopping rule has been met (simp.py:217) */
 } topt_iter_stats;
 
 const char* topt_last_error(void);
@@ -107,6 +113,30 @@ int topt_problem_init(const topt_problem* prob, int32_t dtype, void* stream);
 /* ---- batched engine: one launch per kernel advances every problem ---- */
 int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, topt_plan** out);
 int topt_plan_destroy(topt_plan* plan);
+
+/* Asynchronous SIMP driver (continuous batching at the iteration level).
+ * Every problem cycles through its own OC iterations — [density filter ->
+ * PCG setup] -> PCG -> [sensitivity -> dc filter -> OC] — independently of
+ * the others: PCG launches cover the problems currently solving, and a
+ * problem whose solve ends gets its OC step and its next solve set up at
+ * the next chunk boundary while the others keep solving. Per problem:
+ *   phase[i] (in/out): 0 = about to start an OC iteration, 1 = solving,
+ *                      2 = solve ended, OC step pending, 3 = done
+ *                      (stopping rule met);
+ *   stats[i] (out): statistics of its latest completed OC iteration;
+ *   done_now[i] (out): OC iterations completed during this call;
+ *   cg_now[i] (out): PCG iterations of those OC iterations.
+ * A problem's OC loop stops on the device (simp.py:217 with its oc_tol and
+ * oc_max_iter). Returns when a problem becomes done, after max_chunks
+ * chunks of 16 CG iterations, or when all are done; every problem is then at
+ * a chunk boundary, so the caller may rebuild the plan with other problems
+ * and call again (a solving problem resumes where it stopped; phase is the
+ * only state the caller carries). Results are bitwise those of
+ * topt_plan_iteration; the filtered density of a finished design is the
+ * filter of its final rho. Replaces the host loop of simp.py:141-219 for a
+ * batch; no reference counterpart. Blocks. Graph replay must be off. */
+int topt_plan_advance(topt_plan* plan, int32_t sens_filter, int32_t max_chunks, int32_t* phase,
+                      topt_iter_stats* stats, int32_t* done_now, int64_t* cg_now, void* stream);
 
 /* Replay the PCG slot launches of every full chunk (32 CG iterations) from a
  * HIP graph captured once per plan (on a private stream, kernels only),
@@ -133,10 +163,11 @@ int topt_plan_iteration(topt_plan* plan, int32_t sens_filter, topt_iter_stats* s
 int topt_plan_stats(topt_plan* plan, topt_iter_stats* stats, void* stream);
 
 /* Benchmark support: run `reps` back-to-back launches of PCG slot kernel A
- * (which must be 0: p = z + beta p over the tile halo, Ap = K p, p.Ap) on
- * the plan's current vectors and return the elapsed GPU time (hipEvents on
- * the stream) in *ms. Leaves every problem marked converged. No reference
- * counterpart. Blocks. */
+ * (which must be 0) as it runs inside a solve: the deferred u += alpha p of
+ * the previous step (here with alpha = 0: u keeps its values), p = z + beta p
+ * over the tile halo, Ap = K p, p.Ap; on the plan's current vectors. Returns
+ * the elapsed GPU time (hipEvents on the stream) in *ms. Leaves every
+ * problem marked converged. No reference counterpart. Blocks. */
 int topt_plan_time_slot(topt_plan* plan, int32_t which, int32_t reps, float* ms, void* stream);
 
 /* ---- op-level kernels (single problem, explicit buffers) ---- */

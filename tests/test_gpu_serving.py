@@ -8,13 +8,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_stream_results_equal_alone_runs():
+@pytest.mark.parametrize("mode", ["async", "lockstep"])
+def test_stream_results_equal_alone_runs(mode):
     from toptimization_amd.engine import DesignState, Plan
     from toptimization_amd.serving import DesignServer, synthetic_stream
     reqs = synthetic_stream(n=7, rate=50.0, seed=3, nelx_range=(32, 96), max_iter=12, tol=0.01,
                             cg_tolerance=1e-6, max_cg_iterations=20000,
                             preconditioner="block_jacobi_2x2", precision="mixed")
-    server = DesignServer(max_active=3, hbm_budget=1 << 30, precision="mixed", keep_rho=True)
+    server = DesignServer(max_active=3, hbm_budget=1 << 30, precision="mixed", keep_rho=True, mode=mode)
     stats = server.run(reqs, clock="wall")
     s = stats.summary()
     assert s["designs"] == 7 and s["max_active"] == 3
@@ -31,6 +32,7 @@ def test_stream_results_equal_alone_runs():
             out = plan.iteration()[0]
         plan.close()
         assert out.compliance == res.compliance, res.rid
+        assert (out.volume, out.change) == (res.volume, res.change), res.rid
         np.testing.assert_array_equal(st.rho.cpu().numpy(), res.rho)
 
 

@@ -16,7 +16,7 @@ PC_NONE, PC_JACOBI, PC_ELEMBLOCK, PC_BLOCK2 = 0, 1, 2, 3
 PRECONDITIONER_IDS = {"none": PC_NONE, "jacobi": PC_JACOBI, "block_jacobi": PC_ELEMBLOCK,
                       "block_jacobi_2x2": PC_BLOCK2}
 FILTER_DENSITY, FILTER_DCAVG, FILTER_SENS, FILTER_TRANSPOSE = 0, 1, 2, 3
-ST_RUN, ST_CONV, ST_BREAK, ST_MAXIT, ST_ZERO = 0, 1, 2, 3, 4
+ST_RUN, ST_CONV, ST_BREAK, ST_MAXIT, ST_ZERO, ST_DONE = 0, 1, 2, 3, 4, 5
 
 BUF_NAMES = ["u", "f", "fix", "rho", "rhof", "dc", "dcf", "Hs", "x", "r", "z", "p0", "p1", "Ap",
              "pre", "pre2", "s", "s64", "taps", "cg", "oc", "part", "opart", "ctr"]
@@ -28,7 +28,8 @@ class TProblem(C.Structure):
                 ("max_cg_iter", C.c_int32), ("E_min", C.c_double), ("penal", C.c_double),
                 ("cg_tol", C.c_double), ("volfrac", C.c_double), ("move", C.c_double),
                 ("rmin", C.c_double), ("fnorm2", C.c_double), ("delta", C.c_double),
-                ("Ke", C.c_double * 64), ("ws", C.c_void_p)]
+                ("Ke", C.c_double * 64), ("ws", C.c_void_p), ("oc_tol", C.c_double),
+                ("oc_max_iter", C.c_int32), ("pad", C.c_int32)]
 
 
 class TGrid(C.Structure):
@@ -39,7 +40,7 @@ class TGrid(C.Structure):
 class TIterStats(C.Structure):
     _fields_ = [("compliance", C.c_double), ("volume", C.c_double), ("change", C.c_double),
                 ("lam", C.c_double), ("n_cg", C.c_int32), ("cg_status", C.c_int32),
-                ("oc_steps", C.c_int32), ("refines", C.c_int32)]
+                ("oc_steps", C.c_int32), ("refines", C.c_int32), ("iter", C.c_int32), ("done", C.c_int32)]
 
 
 _vp, _i32, _i64, _d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
@@ -51,6 +52,8 @@ _SIGS = {
     "topt_plan_create": (C.c_int, [C.POINTER(TProblem), _i32, _i32, C.POINTER(_vp)]),
     "topt_plan_destroy": (C.c_int, [_vp]),
     "topt_plan_set_graph": (C.c_int, [_vp, C.c_int32]),
+    "topt_plan_advance": (C.c_int, [_vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32), _vp,
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_int64), _vp]),
     "topt_plan_filter": (C.c_int, [_vp, _i32, _vp]),
     "topt_plan_solve": (C.c_int, [_vp, _i32, _i32, C.POINTER(_i32), C.POINTER(_i32), _vp]),
     "topt_plan_sensitivity": (C.c_int, [_vp, _i32, _vp]),

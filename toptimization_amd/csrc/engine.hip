@@ -222,6 +222,8 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
   P.tol2 = (p.cg_tol * std::sqrt(p.fnorm2)) * (p.cg_tol * std::sqrt(p.fnorm2));
   P.delta2 = p.delta * p.delta;
   P.volfrac = p.volfrac;
+  P.oc_tol = p.oc_tol;
+  P.oc_max_iter = p.oc_max_iter;
   P.move = p.move;
   P.fnorm2 = p.fnorm2;
   char* w = (char*)p.ws;
@@ -266,9 +268,10 @@ __global__ void k_gather_status(const ProbDev* tab, int n, int parity, int* out)
   out[2 * i + 1] = st->it;
 }
 
-__global__ void k_gather_stats(const ProbDev* tab, int n, topt_iter_stats* out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void k_gather_stats(const ProbDev* tab, const int* idx, int n, topt_iter_stats* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = idx ? idx[t] : t;
   const CgState* st = tab[i].cg;
   const OcState* o = tab[i].oc;
   topt_iter_stats s;
@@ -281,6 +284,8 @@ __global__ void k_gather_stats(const ProbDev* tab, int n, topt_iter_stats* out) 
            : (st->status == ST_ZERO ? 0 : st->it);
   s.oc_steps = o->steps;
   s.refines = st->refines;
+  s.iter = o->iter;
+  s.done = o->done;
   out[i] = s;
 }
 
@@ -295,7 +300,8 @@ __global__ void k_cg_force_run(const ProbDev* tab, int n) {
   st->cflag = CF_CG;
   st->amode = OP_A_CG;
   st->pending = 0;
-  st->upend = 0;
+  st->upend = 1;          // ... including the deferred u += alpha p, with alpha = 0
+  st->alpha = 0.0;        // (the bytes move, u keeps its values)
   st->it = 0;
   if (!(st->rz > 0.0)) st->rz = 1.0;
 }
@@ -325,6 +331,7 @@ __global__ void k_oc_out(const OcState* o, double* out) {
 }  // namespace
 
 // ============================================================== plan
+#define TOPT_SUBSLOTS 4
 struct topt_plan {
   int n = 0;
   int dtype = TOPT_F64;
@@ -334,7 +341,7 @@ struct topt_plan {
   int max_cg = 0;
   std::vector<ProbDev> host;
   ProbDev* dtab = nullptr;
-  int* dmap[TOPT_NGRID] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int2* dmap[TOPT_NGRID] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // (problem, block)
   int nblk[TOPT_NGRID] = {0, 0, 0, 0, 0};
   int64_t q = 0;          // PCG slot sequence number of the next launch
   int64_t na = 0;         // slot-A launches so far in this solve (p buffer parity)
@@ -353,14 +360,22 @@ struct topt_plan {
   // PCG grid map of (problem, local tile) pairs: the full map, and two
   // compacted copies (alternating, each filled from its own pinned staging
   // buffer) that leave out problems whose solve has ended
-  int2* dpfull = nullptr;
+  int2* dpfull = nullptr;       // = dmap[G_PCG]
   int2* dpmap[2] = {nullptr, nullptr};
   int2* hpmap[2] = {nullptr, nullptr};
   const int2* pcur = nullptr;   // map of the next launches
   int npcur = 0;                // its block count
   int pslot = 0;
-  std::vector<int2> pfull;      // host copy of the full map
+  std::vector<int2> hmap[TOPT_NGRID];   // host copies of the full maps
   std::vector<char> live;       // problem still in the PCG map
+  // subset launches (asynchronous driver): rotating map / index / stats
+  // staging slots; a slot used at one chunk boundary is reused two
+  // boundaries later, when the chunk enqueued after it has completed
+  int2* sub_d[TOPT_SUBSLOTS][TOPT_NGRID] = {};
+  int2* sub_h[TOPT_SUBSLOTS][TOPT_NGRID] = {};
+  int* sub_di[TOPT_SUBSLOTS] = {};
+  int* sub_hi[TOPT_SUBSLOTS] = {};
+  topt_iter_stats* sub_hs[TOPT_SUBSLOTS] = {};
   int* hpar = nullptr;          // pinned: parity of each problem's final state
   int* dpar = nullptr;
 };
@@ -371,7 +386,7 @@ namespace {
 // map of that same kind (the map holds exactly nb entries).
 struct GridRef {
   int nb;
-  const int* map;
+  const int2* map;
 };
 GridRef grid_of(const topt_plan* pl, int g) { return GridRef{pl->nblk[g], pl->dmap[g]}; }
 
@@ -462,7 +477,7 @@ int launch_solve_setup(topt_plan* pl, int warm, int use_filtered, hipStream_t s)
   hipLaunchKernelGGL((k_precond<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, 0);
   hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                     grid_of(pl, G_NODE).map, dummy, warm);
+                     grid_of(pl, G_NODE).map, dummy, warm, 0);
   LAUNCH_TRY();
   return TOPT_OK;
 }
@@ -478,7 +493,8 @@ int plan_filter(topt_plan* pl, int mode, hipStream_t s) {
 int plan_oc(topt_plan* pl, hipStream_t s) {
   const ProbDev dummy{};
   constexpr int D = 5;
-  hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, dummy);
+  hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n,
+                     dummy);
   for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
     hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(grid_of(pl, G_OC).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                        grid_of(pl, G_OC).map, dummy);
@@ -536,7 +552,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
     if (dropped) {
       const int k = pl->pslot ^= 1;
       int m = 0;
-      for (const int2& e : pl->pfull)
+      for (const int2& e : pl->hmap[G_PCG])
         if (pl->live[e.x]) pl->hpmap[k][m++] = e;
       if (m > 0)
         HIP_TRY(hipMemcpyAsync(pl->dpmap[k], pl->hpmap[k], sizeof(int2) * m, hipMemcpyHostToDevice, s));
@@ -557,7 +573,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   hipLaunchKernelGGL(k_cg_settle, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dpar);
   pl->q = 0;
   pl->na = 0;
-  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -569,11 +585,224 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
 }
 
 int gather_stats(topt_plan* pl, topt_iter_stats* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dstats);
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   std::memcpy(out, pl->hstats, sizeof(topt_iter_stats) * pl->n);
+  return TOPT_OK;
+}
+
+// ---------------------------------------------------------------- asynchronous driver
+// Subset launches: the (problem, block) pairs of grid g for the problems in
+// `ids`, staged in slot `slot` and copied to the device in stream order.
+int subset_grid(topt_plan* pl, int slot, int g, const std::vector<int>& ids, hipStream_t s, GridRef& out) {
+  int m = 0;
+  for (int i : ids) {
+    const ProbDev& P = pl->host[i];
+    std::memcpy(pl->sub_h[slot][g] + m, pl->hmap[g].data() + P.b0[g], sizeof(int2) * P.nb[g]);
+    m += P.nb[g];
+  }
+  if (m > 0) HIP_TRY(hipMemcpyAsync(pl->sub_d[slot][g], pl->sub_h[slot][g], sizeof(int2) * m, hipMemcpyHostToDevice, s));
+  out = GridRef{m, pl->sub_d[slot][g]};
+  return TOPT_OK;
+}
+
+int subset_index(topt_plan* pl, int slot, const std::vector<int>& ids, hipStream_t s) {
+  std::memcpy(pl->sub_hi[slot], ids.data(), sizeof(int) * ids.size());
+  HIP_TRY(hipMemcpyAsync(pl->sub_di[slot], pl->sub_hi[slot], sizeof(int) * ids.size(), hipMemcpyHostToDevice, s));
+  return TOPT_OK;
+}
+
+// Start of an OC iteration for `ids`: density filter (density mode), element
+// scales, preconditioner, PCG state (auto_stop: a design whose stopping rule
+// has been met gets status ST_DONE instead of a solve).
+template <typename T>
+int subset_setup(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_filter, hipStream_t s) {
+  const ProbDev dummy{};
+  GridRef gf, ge, gn;
+  int rc;
+  if ((rc = subset_grid(pl, slot, G_FILT, ids, s, gf)) || (rc = subset_grid(pl, slot, G_ELEM, ids, s, ge)) ||
+      (rc = subset_grid(pl, slot, G_NODE, ids, s, gn)))
+    return rc;
+  if (!sens_filter)
+    hipLaunchKernelGGL((k_filter<true>), dim3(gf.nb), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s, pl->dtab, gf.map,
+                       dummy, (int)F_DENSITY);
+  hipLaunchKernelGGL((k_scale<T, true>), dim3(ge.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, ge.map, dummy,
+                     sens_filter ? 0 : 1);
+  hipLaunchKernelGGL((k_precond<T, true>), dim3(gn.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, gn.map, dummy, 0);
+  hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(gn.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, gn.map, dummy, 2, 1);
+  LAUNCH_TRY();
+  return TOPT_OK;
+}
+
+// End of an OC iteration for `ids` (their solves have ended and cg[0] holds
+// the final state): sensitivity, dc filter, OC bisection and commit, then
+// the statistics, copied to the slot's pinned buffer.
+int subset_post(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_filter, hipStream_t s) {
+  const ProbDev dummy{};
+  constexpr int D = 5;
+  GridRef ge, gf, go;
+  int rc;
+  if ((rc = subset_grid(pl, slot, G_ELEM, ids, s, ge)) || (rc = subset_grid(pl, slot, G_FILT, ids, s, gf)) ||
+      (rc = subset_grid(pl, slot, G_OC, ids, s, go)) || (rc = subset_index(pl, slot, ids, s)))
+    return rc;
+  const int n = (int)ids.size();
+  hipLaunchKernelGGL((k_sens<true>), dim3(ge.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, ge.map, dummy, sens_filter ? 0 : 1);
+  hipLaunchKernelGGL((k_filter<true>), dim3(gf.nb), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s, pl->dtab, gf.map,
+                     dummy, (int)(sens_filter ? F_SENS : F_DCAVG));
+  hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, dummy);
+  for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
+    hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
+  hipLaunchKernelGGL((k_oc_apply<true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, pl->dstats);
+  LAUNCH_TRY();
+  HIP_TRY(hipMemcpyAsync(pl->sub_hs[slot], pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
+  return TOPT_OK;
+}
+
+// Rebuild the compacted PCG map from pl->live (alternating device buffers).
+int rebuild_pcg_map(topt_plan* pl, hipStream_t s) {
+  const int k = pl->pslot ^= 1;
+  int m = 0;
+  for (const int2& e : pl->hmap[G_PCG])
+    if (pl->live[e.x]) pl->hpmap[k][m++] = e;
+  if (m > 0) HIP_TRY(hipMemcpyAsync(pl->dpmap[k], pl->hpmap[k], sizeof(int2) * m, hipMemcpyHostToDevice, s));
+  pl->pcur = pl->dpmap[k];
+  pl->npcur = m;
+  return TOPT_OK;
+}
+
+#define TOPT_ASYNC_CHUNK 16   // CG iterations per chunk (even: parities repeat)
+
+// Every chunk is TOPT_ASYNC_CHUNK CG iterations, so q and na are even at each
+// boundary: the state of every problem is in cg[0] there and a new solve
+// starts on p buffer 0 whichever chunk it joins. The snapshot read after
+// chunk c is one chunk old; a solve seen ended in it is still covered by
+// chunk c + 1 (which applies its deferred u update), so its OC step and the
+// next setup are enqueued after c + 1, and the new solve joins at c + 2.
+int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase, topt_iter_stats* stats,
+                 int32_t* done_now, int64_t* cg_now, hipStream_t s) {
+  const int n = pl->n;
+  for (int i = 0; i < n; ++i) {
+    if (phase[i] < 0 || phase[i] > 3) return fail(TOPT_EINVAL, "phase must be 0..3");
+    done_now[i] = 0;
+    cg_now[i] = 0;
+  }
+  auto setup = [&](const std::vector<int>& ids, int slot) {
+    return pl->dtype == TOPT_F32 ? subset_setup<float>(pl, ids, slot, sens_filter, s)
+                                 : subset_setup<double>(pl, ids, slot, sens_filter, s);
+  };
+  struct Pending { std::vector<int> ids; int slot; int64_t ready; };
+  std::vector<Pending> pend;
+  std::vector<int64_t> join(n, 0);   // first chunk whose snapshot describes the current solve
+  int slot = 0;
+  auto take_slot = [&]() { const int k = slot; slot = (slot + 1) % TOPT_SUBSLOTS; return k; };
+  bool became_done = false;
+  auto read_stats = [&](const Pending& p) {
+    for (int i : p.ids) {
+      stats[i] = pl->sub_hs[p.slot][i];
+      done_now[i] += 1;
+      cg_now[i] += stats[i].n_cg;
+      if (stats[i].done) {
+        phase[i] = 3;
+        pl->live[i] = 0;
+        became_done = true;
+      }
+    }
+  };
+  pl->q = 0;
+  pl->na = 0;
+  int64_t cnext = 0;   // index of the next chunk to enqueue
+  int rc;
+  // ---- initial boundary: OC steps still pending, new OC iterations
+  {
+    std::vector<int> p2, s0;
+    for (int i = 0; i < n; ++i) {
+      if (phase[i] == 2) p2.push_back(i);
+      if (phase[i] == 0 || phase[i] == 2) s0.push_back(i);
+    }
+    if (!p2.empty()) {
+      const int k = take_slot();
+      if ((rc = subset_post(pl, p2, k, sens_filter, s))) return rc;
+      pend.push_back(Pending{p2, k, 0});
+    }
+    if (!s0.empty() && (rc = setup(s0, take_slot()))) return rc;
+    for (int i : s0) phase[i] = 1;
+    for (int i = 0; i < n; ++i) pl->live[i] = phase[i] == 1;
+    if ((rc = rebuild_pcg_map(pl, s))) return rc;
+  }
+  int wait = 0;
+  bool any_live = pl->npcur > 0;
+  if (any_live) {
+    for (int k = 0; k < 2; ++k) {
+      if ((rc = enqueue_chunk(pl, TOPT_ASYNC_CHUNK, (int)(cnext & 1), s))) return rc;
+      ++cnext;
+    }
+  }
+  int64_t chunks = 0;
+  while (any_live) {
+    const int64_t c = cnext - 2;   // chunk whose snapshot is read now
+    HIP_TRY(hipEventSynchronize(pl->ev[wait]));
+    ++chunks;
+    // statistics of OC steps that completed before chunk c
+    for (auto it = pend.begin(); it != pend.end();) {
+      if (it->ready <= c) {
+        read_stats(*it);
+        it = pend.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    // solves that ended by chunk c: OC step and the next solve's setup now
+    // (after chunk c + 1, which still covers them, in stream order)
+    std::vector<int> fin;
+    bool changed = became_done;
+    for (int i = 0; i < n; ++i) {
+      if (!pl->live[i] || join[i] > c) continue;
+      const int st = pl->hstat[wait][2 * i];
+      if (st == ST_RUN) continue;
+      pl->live[i] = 0;
+      changed = true;
+      if (st == ST_DONE) {
+        phase[i] = 3;
+        became_done = true;
+      } else {
+        fin.push_back(i);
+      }
+    }
+    const bool stop = became_done || chunks >= max_chunks;
+    if (!fin.empty()) {
+      const int k = take_slot();
+      if ((rc = subset_post(pl, fin, k, sens_filter, s))) return rc;
+      pend.push_back(Pending{fin, k, cnext});
+      if (!stop) {
+        if ((rc = setup(fin, take_slot()))) return rc;
+        for (int i : fin) {
+          pl->live[i] = 1;
+          join[i] = cnext;
+        }
+      } else {
+        for (int i : fin) phase[i] = 0;   // the OC step is enqueued; the next iteration starts later
+      }
+    }
+    any_live = false;
+    for (int i = 0; i < n; ++i) any_live = any_live || pl->live[i];
+    if (stop || !any_live) break;
+    if (changed && (rc = rebuild_pcg_map(pl, s))) return rc;
+    if ((rc = enqueue_chunk(pl, TOPT_ASYNC_CHUNK, wait, s))) return rc;
+    ++cnext;
+    wait ^= 1;
+  }
+  // Drain what is enqueued. A problem still live stays in phase 1 even if
+  // its solve ended in the last chunk: no later slot A has applied its
+  // deferred u update yet, so the next call's first chunk covers it again.
+  HIP_TRY(hipStreamSynchronize(s));
+  for (const Pending& p : pend) read_stats(p);
+  pl->q = 0;
+  pl->na = 0;
+  pl->pcur = pl->dpfull;
+  pl->npcur = pl->nblk[G_PCG];
   return TOPT_OK;
 }
 
@@ -713,10 +942,30 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   if (hipMemcpy(pl->dtab, pl->host.data(), sizeof(ProbDev) * nprob, hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(TOPT_EHIP, "upload plan table"));
   for (int g = 0; g < TOPT_NGRID; ++g) {
-    if (hipMalloc(&pl->dmap[g], sizeof(int) * maps[g].size()) != hipSuccess)
+    std::vector<int2>& m = pl->hmap[g];
+    m.resize(maps[g].size());
+    for (size_t b = 0; b < maps[g].size(); ++b) {
+      const int i = maps[g][b];
+      m[b] = make_int2(i, (int)b - pl->host[i].b0[g]);
+    }
+    if (hipMalloc(&pl->dmap[g], sizeof(int2) * m.size()) != hipSuccess)
       return cleanup(fail(TOPT_ENOMEM, "hipMalloc block map"));
-    if (hipMemcpy(pl->dmap[g], maps[g].data(), sizeof(int) * maps[g].size(), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(pl->dmap[g], m.data(), sizeof(int2) * m.size(), hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(TOPT_EHIP, "upload block map"));
+    for (int k = 0; k < TOPT_SUBSLOTS; ++k) {
+      if (hipMalloc(&pl->sub_d[k][g], sizeof(int2) * m.size()) != hipSuccess)
+        return cleanup(fail(TOPT_ENOMEM, "hipMalloc subset map"));
+      if (hipHostMalloc(&pl->sub_h[k][g], sizeof(int2) * m.size(), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc subset map"));
+    }
+  }
+  for (int k = 0; k < TOPT_SUBSLOTS; ++k) {
+    if (hipMalloc(&pl->sub_di[k], sizeof(int) * nprob) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipMalloc subset index"));
+    if (hipHostMalloc(&pl->sub_hi[k], sizeof(int) * nprob, hipHostMallocDefault) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc subset index"));
+    if (hipHostMalloc(&pl->sub_hs[k], sizeof(topt_iter_stats) * nprob, hipHostMallocDefault) != hipSuccess)
+      return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc subset stats"));
   }
   if (hipMalloc(&pl->dstat, sizeof(int) * 2 * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc status"));
   if (hipMalloc(&pl->dstats, sizeof(topt_iter_stats) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc stats"));
@@ -730,13 +979,8 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
     return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc stats"));
   // PCG grid: (problem, local tile) pairs in the order of the block map
   const int npcg = pl->nblk[G_PCG];
-  pl->pfull.resize(npcg);
-  for (int i = 0, b = 0; i < nprob; ++i)
-    for (int t = 0; t < pl->host[i].nb[G_PCG]; ++t) pl->pfull[b++] = make_int2(i, t);
   pl->live.assign(nprob, 1);
-  if (hipMalloc(&pl->dpfull, sizeof(int2) * npcg) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc PCG map"));
-  if (hipMemcpy(pl->dpfull, pl->pfull.data(), sizeof(int2) * npcg, hipMemcpyHostToDevice) != hipSuccess)
-    return cleanup(fail(TOPT_EHIP, "upload PCG map"));
+  pl->dpfull = pl->dmap[G_PCG];
   for (int k = 0; k < 2; ++k) {
     if (hipMalloc(&pl->dpmap[k], sizeof(int2) * npcg) != hipSuccess)
       return cleanup(fail(TOPT_ENOMEM, "hipMalloc PCG map"));
@@ -763,7 +1007,15 @@ int topt_plan_destroy(topt_plan* pl) {
     if (pl->ev[k]) (void)hipEventDestroy(pl->ev[k]);
   }
   if (pl->hstats) (void)hipHostFree(pl->hstats);
-  (void)hipFree(pl->dpfull);
+  for (int k = 0; k < TOPT_SUBSLOTS; ++k) {
+    for (int g = 0; g < TOPT_NGRID; ++g) {
+      (void)hipFree(pl->sub_d[k][g]);
+      if (pl->sub_h[k][g]) (void)hipHostFree(pl->sub_h[k][g]);
+    }
+    (void)hipFree(pl->sub_di[k]);
+    if (pl->sub_hi[k]) (void)hipHostFree(pl->sub_hi[k]);
+    if (pl->sub_hs[k]) (void)hipHostFree(pl->sub_hs[k]);
+  }
   for (int k = 0; k < 2; ++k) {
     (void)hipFree(pl->dpmap[k]);
     if (pl->hpmap[k]) (void)hipHostFree(pl->hpmap[k]);
@@ -775,6 +1027,15 @@ int topt_plan_destroy(topt_plan* pl) {
   if (pl->cap) (void)hipStreamDestroy(pl->cap);
   delete pl;
   return TOPT_OK;
+}
+
+int topt_plan_advance(topt_plan* pl, int32_t sens_filter, int32_t max_chunks, int32_t* phase,
+                      topt_iter_stats* stats, int32_t* done_now, int64_t* cg_now, void* stream) {
+  if (pl == nullptr || phase == nullptr || stats == nullptr || done_now == nullptr || cg_now == nullptr ||
+      max_chunks < 1)
+    return fail(TOPT_EINVAL, "invalid advance call");
+  if (pl->use_graph) return fail(TOPT_EINVAL, "topt_plan_advance does not replay graphs");
+  return plan_advance(pl, sens_filter, max_chunks, phase, stats, done_now, cg_now, (hipStream_t)stream);
 }
 
 int topt_plan_set_graph(topt_plan* pl, int32_t enable) {
@@ -1066,7 +1327,7 @@ int topt_oc_update(int32_t nelx, int32_t nely, double* rho, const double* dcf, c
     return fail(TOPT_EINVAL, "topt_oc_update: dv must be NULL (dv = 1, fem/kernels.py:270)");
   }
   constexpr int D = 5;
-  hipLaunchKernelGGL((k_oc_reset<false>), dim3(1), dim3(64), 0, st, nullptr, 1, P);
+  hipLaunchKernelGGL((k_oc_reset<false>), dim3(1), dim3(64), 0, st, nullptr, nullptr, 1, P);
   for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
     hipLaunchKernelGGL((k_oc_eval<D, false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
   hipLaunchKernelGGL((k_oc_apply<false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);

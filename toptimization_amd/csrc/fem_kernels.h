@@ -85,12 +85,13 @@ __device__ __forceinline__ vec2<T> elemblock_node(PD& P, const NodeIdx& id,
 // s_e = E_min + rho_e^p (1 - E_min) from the physical density
 // (reference fem/kernels.py:92). Writes s in T and, for the mixed path, FP64.
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_scale(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_scale(const ProbDev* tab, const int2* bmap,
                                                       ProbDev single, int use_filtered) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   const long ne = (long)P.nelx * P.nely;
-  const long e = (long)(b - P.b0[G_ELEM]) * TOPT_BLOCK + threadIdx.x;
+  const long e = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (e >= ne) return;
   const GLB double* rho = gp(use_filtered ? P.rhof : P.rho);
   const double sv = P.Emin + pow(rho[e], P.penal) * (1.0 - P.Emin);
@@ -103,12 +104,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_scale(const ProbDev* tab, const 
 // 2x2 block: inverse of the assembled 2x2 diagonal block, fixed DOF -> identity
 // row/column (no reference counterpart).
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, const int2* bmap,
                                                         ProbDev single, int raw) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   if (P.precond != PC_JACOBI && P.precond != PC_BLOCK2) return;
-  const NodeIdx id = node_of(P, b - P.b0[G_NODE]);
+  const NodeIdx id = node_of(P, lb);
   if (!id.act) return;
   const T* s = (const T*)P.s;
   T sc[2][2];
@@ -152,11 +154,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, cons
 // problem has finished at least one OC iteration (fem/solver.py:77-83;
 // optimizer/simp.py:160 passes warm_start = it > 1).
 template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, const int* bmap,
-                                                           ProbDev single, int warm) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, const int2* bmap,
+                                                           ProbDev single, int warm, int auto_stop) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  const int lb = b - P.b0[G_NODE];
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   const bool zero_load = P.fnorm2 < 1e-60;
   const int w = (warm == 2) ? (P.oc != nullptr && P.oc->iter > 0) : warm;
   const NodeIdx id = node_of(P, lb);
@@ -177,7 +179,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
   }
   if (lb == 0 && threadIdx.x == 0) {
     CgState st{};
-    st.status = zero_load ? ST_ZERO : ST_RUN;
+    // auto_stop: a design whose OC loop has met its stopping rule runs no solve
+    st.status = zero_load ? ST_ZERO : ((auto_stop && P.oc->done) ? ST_DONE : ST_RUN);
     st.op = OP_START;
     st.first = 1;
     st.cflag = CF_CG;
@@ -187,21 +190,6 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
 }
 
 // ---------------------------------------------------------------- PCG slots
-// The PCG grid's block map holds (problem, local tile) pairs, so the host can
-// drop the tiles of problems whose solve has ended between chunks.
-template <bool B>
-__device__ __forceinline__ PD& pcg_prob(const ProbDev* tab, const int2* pmap, const ProbDev& single, int b,
-                                        int& lb) {
-  if constexpr (B) {
-    const int2 e = pmap[b];
-    lb = e.y;
-    return *(PD*)(tab + e.x);
-  } else {
-    lb = b;
-    return *(PD*)&single;
-  }
-}
-
 // The host launches slot kernels A, B[, C], A, B[, C], ... with sequence
 // numbers q = 0, 1, 2, ... Kernel q
 //   1. reads the state cg[q & 1] written by kernel q-1,
@@ -714,7 +702,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int lb;
-  PD& P = pcg_prob<B>(tab, pmap, single, b, lb);
+  PD& P = grid_prob<B>(tab, pmap, single, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* pold = gp((const vec2<T>*)(par ? P.p1 : P.p0));
   GLB vec2<T>* pnew = gp((vec2<T>*)(par ? P.p0 : P.p1));
@@ -822,7 +810,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int lb;
-  PD& P = pcg_prob<B>(tab, pmap, single, b, lb);
+  PD& P = grid_prob<B>(tab, pmap, single, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   const int pc = P.precond;
   GLB vec2<T>* r = gp((vec2<T>*)P.r);
@@ -907,7 +895,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int lb;
-  PD& P = pcg_prob<B>(tab, pmap, single, b, lb);
+  PD& P = grid_prob<B>(tab, pmap, single, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   PrevParts pp;
   parts_fetch(P, q, pp);
@@ -932,12 +920,12 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
 // dc_e = -p rho^(p-1) (1-E_min) u_e^T Ke u_e, dv_e = 1, c = sum s_e u_e^T Ke u_e.
 // rho is the physical density (filtered in density mode).
 template <bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const int2* bmap,
                                                      ProbDev single, int use_filtered) {
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  const int lb = b - P.b0[G_ELEM];
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   const long ne = (long)P.nelx * P.nely;
   const long e0 = (long)lb * TOPT_BLOCK + threadIdx.x;
   const bool act = e0 < ne;

@@ -23,12 +23,12 @@ namespace topt {
 //   F_SENS      v = rho dc              out: dcf  = num / (rho Hs + 1e-20) (filter.py:152-171)
 //   F_TRANSPOSE v = dc / (Hs + 1e-20)   out: dcf  = num   (exact chain rule H^T, no reference twin)
 template <bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_filter(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_filter(const ProbDev* tab, const int2* bmap,
                                                        ProbDev single, int mode) {
   extern __shared__ double tile[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
-  const int lb = b - P.b0[G_FILT];
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   const int nelx = P.nelx, nely = P.nely, R = P.R;
   const int nty = (nely + TOPT_FILT_TY - 1) / TOPT_FILT_TY;
   const int tx0 = (lb / nty) * TOPT_FILT_TX, ty0 = (lb % nty) * TOPT_FILT_TY;
@@ -117,9 +117,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_filter_tables(int nelx, int nely
 // are FP64 sums.
 
 template <bool B>
-__global__ void k_oc_reset(const ProbDev* tab, int nprob, ProbDev single) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (B ? nprob : 1)) return;
+__global__ void k_oc_reset(const ProbDev* tab, const int* idx, int nprob, ProbDev single) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (B ? nprob : 1)) return;
+  const int i = idx ? idx[t] : t;
   PD& P = B ? *(PD*)(tab + i) : *(PD*)&single;
   OcState* o = P.oc;
   o->lo = 0.0;
@@ -149,19 +150,19 @@ __device__ __forceinline__ double subtree_lambda(int j, double lo, double hi) {
 }
 
 template <int D, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, const int2* bmap,
                                                         ProbDev single) {
   constexpr int NL = (1 << D) - 1;
   __shared__ double qs[NL];
   __shared__ double red[4 * NL];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   OcState* o = P.oc;
   if (o->stopped) return;
   const double lo = o->lo, hi = o->hi;
   if (threadIdx.x < NL) qs[threadIdx.x] = 1.0 / sqrt(subtree_lambda(threadIdx.x, lo, hi) + 1e-20);
   __syncthreads();
-  const int lb = b - P.b0[G_OC];
   const long ne = (long)P.nelx * P.nely;
   const double mv = P.move;
   double acc[NL];
@@ -230,13 +231,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
 // evaluated lambda (optimizer/oc.py:40-47, 91-92), max |drho| and the mean
 // density (fem/kernels.py:330-336) by the last block.
 template <bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_oc_apply(const ProbDev* tab, const int* bmap,
+__global__ __launch_bounds__(TOPT_BLOCK) void k_oc_apply(const ProbDev* tab, const int2* bmap,
                                                          ProbDev single) {
   __shared__ double red[8];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  PD& P = get_prob<B>(tab, bmap, single, b);
+  int lb;
+  PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   OcState* o = P.oc;
-  const int lb = b - P.b0[G_OC];
   const long ne = (long)P.nelx * P.nely;
   const double lam = o->last_mid, mv = P.move;
   double vsum = 0.0, vmax = 0.0;
@@ -280,6 +281,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_apply(const ProbDev* tab, con
     o->vol = s / (double)ne;
     o->change = m;
     o->iter += 1;
+    // stopping rule of optimizer/simp.py:217 (used by the asynchronous driver)
+    o->done = ((o->iter > 5 && m < P.oc_tol) || o->iter >= P.oc_max_iter) ? 1 : 0;
   }
 }
 

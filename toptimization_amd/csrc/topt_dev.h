@@ -21,7 +21,7 @@ namespace topt {
 
 // ---------------------------------------------------------------- constants
 enum Precond : int { PC_NONE = 0, PC_JACOBI = 1, PC_ELEMBLOCK = 2, PC_BLOCK2 = 3 };
-enum Status : int { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MAXIT = 3, ST_ZERO = 4 };
+enum Status : int { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MAXIT = 3, ST_ZERO = 4, ST_DONE = 5 };
 enum Mode : int { MODE_CG = 0, MODE_REFINE = 1 };
 enum FilterMode : int { F_DENSITY = 0, F_DCAVG = 1, F_SENS = 2, F_TRANSPOSE = 3, F_HS = 4 };
 
@@ -43,7 +43,7 @@ enum CFlag : int { CF_CG = 0, CF_REFRESH = 1 };
 // OC / SIMP per-iteration scalars of one problem.
 struct OcState {
   double lo, hi, last_mid, c, vol, change;
-  int steps, stopped, iter, pad;
+  int steps, stopped, iter, done;   // done: the OC stopping rule has been met
 };
 
 // Block grids a batch is launched over; each problem owns nb[g] consecutive
@@ -68,7 +68,8 @@ struct ProbDev {
   float Ki32[64];        // regularised Ke inverse (element-block Jacobi)
   double Ki64[64];
   double kc[6];          // Ke = KS pattern x kc when kstruct (see q4_gather_s)
-  int kstruct, pad_k;
+  int kstruct, oc_max_iter;   // oc_max_iter, oc_tol: the OC stopping rule
+  double oc_tol;
   // FEM vectors (node interleaved). T = float (mixed) or double (plain).
   void* x;   // CG iterate: aliases the FP64 solution u64 (both paths)
   void* r; void* z; void* p0; void* p1; void* Ap;
@@ -132,6 +133,22 @@ __device__ __forceinline__ PD& get_prob(const ProbDev* tab, const int* bmap,
                                         const ProbDev& single, int b) {
   if constexpr (BATCHED) return *(PD*)(tab + bmap[b]);
   else return *(PD*)&single;
+}
+
+// Batched grids: the block map holds (problem, local block) pairs, so a
+// launch can cover any subset of a plan's problems (each with all its
+// blocks). Single-problem launches pass the descriptor by value.
+template <bool BATCHED>
+__device__ __forceinline__ PD& grid_prob(const ProbDev* tab, const int2* bmap, const ProbDev& single, int b,
+                                         int& lb) {
+  if constexpr (BATCHED) {
+    const int2 e = bmap[b];
+    lb = e.y;
+    return *(PD*)(tab + e.x);
+  } else {
+    lb = b;
+    return *(PD*)&single;
+  }
 }
 
 __device__ __forceinline__ unsigned fixbits(const uint32_t* fix, long n) {

@@ -63,6 +63,8 @@ class IterStats:
     cg_status: int
     oc_steps: int
     refines: int
+    iter: int = 0
+    done: int = 0
 
 
 class DesignState:
@@ -126,6 +128,8 @@ class DesignState:
         d.delta = float(delta)
         d.Ke[:] = [float(v) for v in self.Ke.ravel()]
         d.ws = self.ws.data_ptr()
+        d.oc_tol = float(problem.tol)
+        d.oc_max_iter = int(problem.max_iter)
         check(lib.topt_problem_init(C.byref(d), self.dtype_id, C.c_void_p(stream_ptr())),
               "topt_problem_init")
 
@@ -205,6 +209,26 @@ class Plan:
                                               self._stream()), "topt_plan_iteration")
         return [_stats(o) for o in out]
 
+    def advance(self, phase: list[int], sensitivity_filter: bool = False, max_chunks: int = 1 << 30):
+        """Asynchronous SIMP iterations (topt_plan_advance): each problem runs
+        its own OC iterations and PCG solves until one becomes done, until
+        max_chunks chunks of 16 CG iterations, or until all are done.
+        phase[i]: 0 start an OC iteration, 1 solving, 2 OC step pending,
+        3 done; updated in place. Returns (stats of each problem's latest
+        completed OC iteration, OC iterations completed per problem, their
+        PCG iterations)."""
+        n = len(self)
+        if len(phase) != n:
+            raise ValueError("one phase per problem")
+        ph = (C.c_int32 * n)(*phase)
+        out = (TIterStats * n)()
+        done = (C.c_int32 * n)()
+        cg = (C.c_int64 * n)()
+        check(_lib.load().topt_plan_advance(self.handle, int(bool(sensitivity_filter)), int(max_chunks), ph, out,
+                                            done, cg, self._stream()), "topt_plan_advance")
+        phase[:] = list(ph)
+        return [_stats(o) for o in out], list(done), list(cg)
+
     def time_slot(self, which: int, reps: int) -> float:
         """Average GPU ms of PCG slot kernel A (0) or B (1) over `reps`
         back-to-back launches (benchmark only; leaves the solve finished)."""
@@ -221,4 +245,4 @@ class Plan:
 
 def _stats(o: TIterStats) -> IterStats:
     return IterStats(o.compliance, o.volume, o.change, o.lam, o.n_cg, o.cg_status, o.oc_steps,
-                     o.refines)
+                     o.refines, o.iter, o.done)

@@ -101,6 +101,8 @@ class _Active:
     admitted: float
     it: int = 0
     n_cg: int = 0
+    phase: int = 0          # asynchronous driver phase (topt_plan_advance)
+    last: object = None     # statistics of the latest completed OC iteration
 
 
 @dataclass
@@ -120,21 +122,35 @@ class ServerStats:
 
 
 class DesignServer:
-    """Admission by slot count and HBM budget; one SIMP iteration per step
-    for every active design."""
+    """Admission by slot count and HBM budget.
+
+    mode='async' (default): every active design runs its own SIMP iterations
+    (topt_plan_advance); a design whose PCG solve ends gets its OC step and
+    its next solve at the next 16-iteration chunk boundary while the others
+    keep solving, and a finished design leaves at once.
+    mode='lockstep': one SIMP iteration of every active design per engine
+    step (each step waits for the slowest solve of the batch).
+    Both give bitwise the same designs."""
 
     def __init__(self, max_active: int = 64, hbm_budget: int = 32 << 30, precision: str = "mixed",
-                 preconditioner: str | None = None, keep_rho: bool = False):
+                 preconditioner: str | None = None, keep_rho: bool = False, mode: str = "async",
+                 admit_chunks: int = 8):
+        if mode not in ("async", "lockstep"):
+            raise ValueError("mode must be 'async' or 'lockstep'")
         self.max_active = int(max_active)
         self.hbm_budget = int(hbm_budget)
         self.precision = precision
         self.preconditioner = preconditioner
         self.keep_rho = keep_rho
+        self.mode = mode
+        self.admit_chunks = int(admit_chunks)   # chunks per call while requests wait
 
     def run(self, requests: list[Request], clock: str = "wall", progress=None) -> ServerStats:
         """Serve the stream. clock='wall': arrivals are real time offsets from
         the start; clock='instant': every request is available at t=0.
         progress(msg) is called whenever a design finishes."""
+        if self.mode == "async":
+            return self._run_async(requests, clock, progress)
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active: list[_Active] = []
         stats = ServerStats()
@@ -198,6 +214,85 @@ class DesignServer:
                 active = keep
                 if plan is not None:
                     plan.close()
+                plan = None
+        if plan is not None:
+            plan.close()
+        stats.makespan = now()
+        stats.results.sort(key=lambda r: r.rid)
+        return stats
+
+
+    def _admit(self, pending, active, used, t, clock):
+        while pending and len(active) < self.max_active:
+            r = pending[0]
+            if clock == "wall" and r.arrival > t:
+                break
+            need = footprint_bytes(r.problem, self.precision, self.preconditioner)
+            if active and used + need > self.hbm_budget:
+                break
+            pending.pop(0)
+            st = DesignState(r.problem, precision=self.precision, preconditioner=self.preconditioner)
+            active.append(_Active(r, st, t))
+            used += need
+        return used
+
+    def _run_async(self, requests, clock, progress) -> ServerStats:
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active: list[_Active] = []
+        stats = ServerStats()
+        plan: Plan | None = None
+        used = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while pending or active:
+            n_before = len(active)
+            used = self._admit(pending, active, used, now(), clock)
+            if not active:
+                if pending and clock == "wall":
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                continue
+            if plan is None or len(active) != n_before:
+                if plan is not None:
+                    plan.close()
+                plan = Plan([a.state for a in active])
+            stats.max_active = max(stats.max_active, len(active))
+            sens = [a.req.problem.filter_type == "sensitivity" for a in active]
+            if any(sens) and not all(sens):
+                raise NotImplementedError("a batch mixes density and sensitivity filtering")
+            # while requests wait for a slot that is free, return often to admit them
+            waiting = bool(pending) and len(active) < self.max_active
+            phase = [a.phase for a in active]
+            out, done_now, cg_now = plan.advance(phase, sensitivity_filter=sens[0],
+                                                 max_chunks=self.admit_chunks if waiting else 256)
+            stats.steps += 1
+            t = now()
+            if progress is not None and stats.steps % 25 == 1:
+                progress(f"call {stats.steps} at {t:.1f} s: {len(active)} active, "
+                         f"{len(stats.results)} finished, {len(pending)} waiting")
+            keep = []
+            for a, ph, s, dn, cg in zip(active, phase, out, done_now, cg_now):
+                a.phase = ph
+                a.it += dn
+                a.n_cg += cg
+                if dn:
+                    a.last = s
+                if ph == 3:
+                    s = a.last
+                    pr = a.req.problem
+                    stats.results.append(Result(
+                        a.req.rid, a.req.arrival, a.admitted, t, a.it, s.compliance, s.volume, s.change,
+                        a.n_cg, a.state.rho.cpu().numpy() if self.keep_rho else None))
+                    used -= footprint_bytes(pr, self.precision, self.preconditioner)
+                    if progress is not None:
+                        r = stats.results[-1]
+                        progress(f"design {r.rid} ({pr.nelx}x{pr.nely}) done after {r.iterations} it, "
+                                 f"latency {r.latency:.2f} s, {len(stats.results)} finished, "
+                                 f"{len(pending)} waiting")
+                else:
+                    keep.append(a)
+            if len(keep) != len(active):
+                active = keep
+                plan.close()
                 plan = None
         if plan is not None:
             plan.close()
