@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 # solution update), measured with rocprofv3 --pmc FETCH_SIZE and --pmc
 # WRITE_SIZE over `bench.py --probe`, each scaled by the counter's reading for
 # a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
-# profiles/pcg_slots_bridge1200x400_mixed.md. None until measured for the
-# current kernel.
-SLOT_A_TRAFFIC_BYTES = None
+# profiles/pcg_slots_bridge1200x400_mixed.md (FETCH 8759.4 KiB x 2.0 + WRITE
+# 15072.6 KiB, median of the 100 replayed launches).
+SLOT_A_TRAFFIC_BYTES = 33_372_090
 NELX, NELY = 1200, 400
 
 
@@ -104,18 +104,19 @@ def kernel_roofline(plan, nn, ne, log, reps=200):
     return ms, bytes_a, gbs
 
 
-def matvec_gdofs(nelx, nely, log, reps=200):
-    """Op-level FP32 K(s) p (no dot) back to back, torch events on the launch stream."""
+def matvec_gdofs(nelx, nely, log, reps=200, dtype="f32"):
+    """Op-level K(s) p (no dot) back to back, torch events on the launch
+    stream; rho ~ U[0.001, 1] (seed 0), p ~ N(0, 1) (seed 1)."""
     import numpy as np
     import torch
     from toptimization_amd import ops
     from toptimization_amd.material import compute_Ke_2d
     nn = (nelx + 1) * (nely + 1)
+    tdt = torch.float32 if dtype == "f32" else torch.float64
     g = ops.make_grid(nelx, nely, compute_Ke_2d(1.0, 0.3), 1e-9, 3.0)
-    rng = np.random.default_rng(0)
-    rho = torch.as_tensor(rng.uniform(0.001, 1.0, nelx * nely), device="cuda")
-    s = ops.scale(g, rho, torch.float32)
-    p = torch.as_tensor(rng.normal(size=2 * nn), dtype=torch.float32, device="cuda")
+    rho = torch.as_tensor(np.random.default_rng(0).uniform(0.001, 1.0, nelx * nely), device="cuda")
+    s = ops.scale(g, rho, tdt)
+    p = torch.as_tensor(np.random.default_rng(1).normal(size=2 * nn), dtype=tdt, device="cuda")
     fix = ops.fix_mask(2 * nn, [0, 1])
     Ap = torch.empty_like(p)
     for _ in range(5):
@@ -128,8 +129,9 @@ def matvec_gdofs(nelx, nely, log, reps=200):
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
     gdofs = 2 * nn / (ms * 1e-3) / 1e9
-    bytes_mv = 4 * (2 * 2 * nn + nelx * nely)
-    log(f"  matvec {nelx}x{nely}: {ms * 1e3:.2f} us, {gdofs:.2f} GDOF/s, "
+    ws = 4 if dtype == "f32" else 8
+    bytes_mv = ws * (2 * 2 * nn + nelx * nely)
+    log(f"  matvec {dtype} {nelx}x{nely}: {ms * 1e3:.2f} us, {gdofs:.2f} GDOF/s, "
         f"{bytes_mv / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
     return ms, gdofs, bytes_mv / (ms * 1e-3) / 1e9
 
@@ -207,6 +209,58 @@ def batch_bench(log, n=64, iters=100, seq_sample=None):
             "cg_iterations_batched": ncg, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
 
 
+def mbb_bench(log, iters=100):
+    """300x100 half MBB beam (left edge x-roller, bottom-right y-roller, unit
+    load at the top-left corner), volfrac 0.5, rmin 4, p 3, 100 OC iterations
+    from rho = 0.5: the mixed FP32/FP64 path with 2x2 block Jacobi (rtol 1e-6)
+    against the plain FP64 path (same preconditioner) and against mixed Jacobi
+    (CG count of the two preconditioners)."""
+    import numpy as np
+    import torch
+    from toptimization_amd.engine import DesignState, Plan
+    from toptimization_amd.problem import make_problem
+
+    def mbb(prec, pc):
+        return make_problem(
+            300, 100,
+            fixed=[{"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x"]},
+                   {"nodes": {"type": "point", "corner": "bottom-right"}, "dofs": ["y"]}],
+            loads=[{"nodes": {"type": "point", "corner": "top-left"}, "force": [0.0, -1.0]}],
+            name="mbb", volume_fraction=0.5, filter_radius=4.0, max_iterations=iters, tolerance=0.0,
+            cg_tolerance=1e-6, max_cg_iterations=50000, preconditioner=pc, precision=prec)
+
+    out = {}
+    for key, prec, pc in (("mixed", "mixed", "block_jacobi_2x2"), ("fp64", "fp64", "block_jacobi_2x2"),
+                          ("mixed_jacobi", "mixed", "jacobi")):
+        st = DesignState(mbb(prec, pc))
+        plan = Plan([st])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [plan.iteration()[0] for _ in range(iters)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        plan.close()
+        ncg = sum(x.n_cg for x in stats)
+        out[key] = {"seconds": dt, "n_cg": ncg, "c": np.array([x.compliance for x in stats])}
+        log(f"  [mbb 300x100 {key}] {iters} it in {dt:.3f} s: {iters / dt:.1f} OC it/s, {ncg} CG "
+            f"({ncg / dt:.0f} CG it/s), final c={stats[-1].compliance:.9e}")
+    rel = np.abs(out["mixed"]["c"] - out["fp64"]["c"]) / np.abs(out["fp64"]["c"])
+    nd, ne = 2 * 301 * 101, 300 * 100
+    # PCG-iteration algorithmic bytes of the mixed 2x2 block-Jacobi path:
+    # slot A 64 N_n + 4 N_e, slot B 44 N_n (N_n = N_d / 2 nodes)
+    b_it = 64 * nd // 2 + 4 * ne + 44 * nd // 2
+    m = out["mixed"]
+    return {"metric": "OC it/s (300x100 MBB, 100 iterations)", "value": round(iters / m["seconds"], 2),
+            "unit": "OC it/s", "pcg_its_per_s": round(m["n_cg"] / m["seconds"], 1), "cg_total": m["n_cg"],
+            "fp64_oc_its_per_s": round(iters / out["fp64"]["seconds"], 2), "fp64_cg_total": out["fp64"]["n_cg"],
+            "mixed_over_fp64": round(out["fp64"]["seconds"] / m["seconds"], 3),
+            "jacobi_cg_total": out["mixed_jacobi"]["n_cg"],
+            "jacobi_oc_its_per_s": round(iters / out["mixed_jacobi"]["seconds"], 2),
+            "compliance_rel_err_final": float(rel[-1]), "compliance_rel_err_max": float(rel.max()),
+            "pcg_gbs_algorithmic": round(b_it * m["n_cg"] / m["seconds"] / 1e9, 1),
+            "dtype": "f32 vectors + f64 dots/refresh/design fields", "data": "synthetic"}
+
+
 def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="async"):
     """Serving stream: n seeded requests, Poisson arrivals at `rate`/s, nelx
     uniform over multiples of 16 in [64, 768], aspect 1/4, 1/3 or 1/2, BC
@@ -255,6 +309,8 @@ def main() -> int:
                     help="64 x 128x64 cantilevers: one batched plan vs one at a time (own JSON line)")
     ap.add_argument("--serving", action="store_true",
                     help="512-request serving stream: designs/hour, p50/p99 latency (own JSON line)")
+    ap.add_argument("--mbb", action="store_true",
+                    help="300x100 MBB, 100 OC iterations: mixed vs FP64 vs Jacobi (own JSON line)")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--serving-mode", choices=["async", "lockstep"], default="async")
@@ -271,6 +327,9 @@ def main() -> int:
     if args.batch:
         print(json.dumps(batch_bench(log)), flush=True)
         return 0
+    if args.mbb:
+        print(json.dumps(mbb_bench(log)), flush=True)
+        return 0
     if args.serving:
         print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample, mode=args.serving_mode)), flush=True)
         return 0
@@ -283,6 +342,7 @@ def main() -> int:
         kernel_roofline(plan, nn, ne, log, reps=100)
         matvec_gdofs(NELX, NELY, log, reps=50)
         matvec_gdofs(8192, 4096, log, reps=10)
+        matvec_gdofs(8192, 4096, log, reps=10, dtype="f64")
         plan.close()
         # calibration for HBM byte counters: a plain device copy of 256 MiB
         # (reads and writes exactly 256 MiB each), three times
@@ -304,6 +364,7 @@ def main() -> int:
     rel_c = same_rho_compliance(rho_probe, log)
     mv_ms, mv_gdofs, mv_gbs = matvec_gdofs(NELX, NELY, log)
     mvb_ms, mvb_gdofs, mvb_gbs = matvec_gdofs(8192, 4096, log, reps=20)
+    _, mvd_gdofs, mvd_gbs = matvec_gdofs(8192, 4096, log, reps=10, dtype="f64")
     cpu = None if args.no_cpu else cpu_baseline(log)
 
     value = args.steps / mixed["seconds"]
@@ -335,6 +396,8 @@ def main() -> int:
         "matvec_gbs_algorithmic": round(mv_gbs, 1),
         "matvec_8192x4096_gdofs": round(mvb_gdofs, 2),
         "matvec_8192x4096_gbs": round(mvb_gbs, 1),
+        "matvec_f64_8192x4096_gdofs": round(mvd_gdofs, 2),
+        "matvec_f64_8192x4096_gbs": round(mvd_gbs, 1),
         "roofline": {"bound": "hbm", "kernel": "k_slot_a<float> (PCG slot A)",
                      "achieved": round(gbs_a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs_a / HBM_PEAK_GBS, 4), "traffic": SLOT_A_TRAFFIC_BYTES,

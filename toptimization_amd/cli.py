@@ -32,6 +32,9 @@ def main(argv=None) -> int:
     srv_p.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     srv_p.add_argument("--clock", choices=["wall", "instant"], default="instant",
                        help="wall: honour arrival times; instant: all requests available at t=0")
+    srv_p.add_argument("--mode", choices=["async", "lockstep"], default="async",
+                       help="async: every design runs its own SIMP iterations; lockstep: one "
+                            "iteration of every active design per step")
     args = ap.parse_args(argv)
     if args.command == "run":
         return _cmd_run(args)
@@ -116,7 +119,7 @@ def _cmd_serve(args) -> int:
     out = Path(args.output)
     out.mkdir(parents=True, exist_ok=True)
     server = DesignServer(max_active=args.max_active, hbm_budget=int(args.hbm_budget_gb * (1 << 30)),
-                          precision=args.precision, keep_rho=True)
+                          precision=args.precision, keep_rho=True, mode=args.mode)
     stats = server.run(reqs, clock=args.clock, progress=lambda m: print(m, flush=True))
     by_id = {r.rid: r for r in reqs}
     rows = []

@@ -55,7 +55,10 @@ enum Grid : int { G_NODE = 0, G_ELEM = 1, G_FILT = 2, G_OC = 3, G_PCG = 4 };
 #define TOPT_OC_PER_THREAD 8
 #define TOPT_OC_MAXD 6    // bisection levels evaluated per OC pass (<= 6)
 // nodes per thread in the PCG slot kernels (FP32 / FP64 vectors)
-template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
+#ifndef TOPT_NPT_F32
+#define TOPT_NPT_F32 4
+#endif
+template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? TOPT_NPT_F32 : 2; };
 
 struct ProbDev {
   int nelx, nely;
