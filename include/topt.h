@@ -29,6 +29,7 @@ extern "C" {
 #define TOPT_PC_JACOBI 1
 #define TOPT_PC_ELEMBLOCK 2   /* reference "block_jacobi" (preconditioner.py) */
 #define TOPT_PC_BLOCK2 3      /* nodal 2x2 block Jacobi */
+#define TOPT_PC_MG 4          /* geometric multigrid V(1,1) cycle, Galerkin coarse levels */
 
 /* filter modes (reference optimizer/filter.py) */
 #define TOPT_FILTER_DENSITY 0     /* rhof = H rho / Hs          filter.py:106-124 */
@@ -63,6 +64,7 @@ enum {
   TOPT_BUF_X, TOPT_BUF_R, TOPT_BUF_Z, TOPT_BUF_P0, TOPT_BUF_P1, TOPT_BUF_AP,
   TOPT_BUF_PRE, TOPT_BUF_PRE2, TOPT_BUF_S, TOPT_BUF_S64,
   TOPT_BUF_TAPS, TOPT_BUF_CG, TOPT_BUF_OC, TOPT_BUF_PART, TOPT_BUF_OPART, TOPT_BUF_CTR,
+  TOPT_BUF_MG,      /* multigrid levels (TOPT_PC_MG only): vectors, element matrices, coarse inverse */
   TOPT_BUF_COUNT
 };
 

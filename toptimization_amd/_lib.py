@@ -12,14 +12,14 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libtopt.so"
 
 F32, F64 = 0, 1
-PC_NONE, PC_JACOBI, PC_ELEMBLOCK, PC_BLOCK2 = 0, 1, 2, 3
+PC_NONE, PC_JACOBI, PC_ELEMBLOCK, PC_BLOCK2, PC_MG = 0, 1, 2, 3, 4
 PRECONDITIONER_IDS = {"none": PC_NONE, "jacobi": PC_JACOBI, "block_jacobi": PC_ELEMBLOCK,
-                      "block_jacobi_2x2": PC_BLOCK2}
+                      "block_jacobi_2x2": PC_BLOCK2, "multigrid": PC_MG}
 FILTER_DENSITY, FILTER_DCAVG, FILTER_SENS, FILTER_TRANSPOSE = 0, 1, 2, 3
 ST_RUN, ST_CONV, ST_BREAK, ST_MAXIT, ST_ZERO, ST_DONE = 0, 1, 2, 3, 4, 5
 
 BUF_NAMES = ["u", "f", "fix", "rho", "rhof", "dc", "dcf", "Hs", "x", "r", "z", "p0", "p1", "Ap",
-             "pre", "pre2", "s", "s64", "taps", "cg", "oc", "part", "opart", "ctr"]
+             "pre", "pre2", "s", "s64", "taps", "cg", "oc", "part", "opart", "ctr", "mg"]
 BUF = {n: i for i, n in enumerate(BUF_NAMES)}
 
 

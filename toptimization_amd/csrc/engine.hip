@@ -17,6 +17,7 @@
 #include "../../include/topt.h"
 #include "topt_dev.h"
 #include "fem_kernels.h"
+#include "mg_kernels.h"
 #include "opt_kernels.h"
 
 using namespace topt;
@@ -103,10 +104,50 @@ struct Layout {
   int64_t total;
 };
 
+// Multigrid hierarchy of a grid: level sizes and buffer offsets inside the
+// TOPT_BUF_MG region (level 0 keeps x0 and r1 there; its b is the PCG r and
+// its 1/diag the pre buffer).
+struct MgLayout {
+  int nlev = 0, ncd = 0;
+  int nx[TOPT_MG_MAXLEV], ny[TOPT_MG_MAXLEV];
+  int64_t K[TOPT_MG_MAXLEV], x[TOPT_MG_MAXLEV], b[TOPT_MG_MAXLEV], r[TOPT_MG_MAXLEV], d[TOPT_MG_MAXLEV];
+  int64_t inv = 0, total = 0;
+};
+
+int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
+  M.nlev = 1;
+  M.nx[0] = nelx;
+  M.ny[0] = nely;
+  for (;;) {
+    if (M.nlev >= TOPT_MG_MAXLEV) return fail(TOPT_EINVAL, "grid too large for the multigrid hierarchy");
+    const int l = M.nlev++;
+    M.nx[l] = (M.nx[l - 1] + 1) / 2;
+    M.ny[l] = (M.ny[l - 1] + 1) / 2;
+    if (2 * int64_t(M.nx[l] + 1) * (M.ny[l] + 1) <= TOPT_MG_DENSE) break;
+    if (M.nx[l] == M.nx[l - 1] && M.ny[l] == M.ny[l - 1])
+      return fail(TOPT_EINVAL, "multigrid coarsening makes no progress");
+  }
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) { const int64_t at = o; o += align256(bytes); return at; };
+  for (int l = 0; l < M.nlev; ++l) {
+    const int64_t nn = int64_t(M.nx[l] + 1) * (M.ny[l] + 1), ne = int64_t(M.nx[l]) * M.ny[l];
+    M.K[l] = l > 0 ? take(ne * 36 * ts) : -1;
+    M.x[l] = take(nn * 2 * ts);
+    M.r[l] = take(nn * 2 * ts);
+    M.b[l] = l > 0 ? take(nn * 2 * ts) : -1;
+    M.d[l] = l > 0 ? take(nn * 2 * ts) : -1;
+  }
+  const int L = M.nlev - 1;
+  M.ncd = 2 * (M.nx[L] + 1) * (M.ny[L] + 1);
+  M.inv = take(int64_t(M.ncd) * M.ncd * 8);
+  M.total = o;
+  return TOPT_OK;
+}
+
 int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout& L) {
   if (nelx < 1 || nely < 1) return fail(TOPT_EINVAL, "grid must have nelx, nely >= 1");
   if (dtype != TOPT_F32 && dtype != TOPT_F64) return fail(TOPT_EINVAL, "dtype must be TOPT_F32 or TOPT_F64");
-  if (precond < TOPT_PC_NONE || precond > TOPT_PC_BLOCK2) return fail(TOPT_EINVAL, "unknown preconditioner");
+  if (precond < TOPT_PC_NONE || precond > TOPT_PC_MG) return fail(TOPT_EINVAL, "unknown preconditioner");
   if (!(rmin > 0.0)) return fail(TOPT_EINVAL, "rmin must be > 0");
   const Geom g = geom(nelx, nely, dtype);
   const int64_t ts = dtype == TOPT_F32 ? 4 : 8;
@@ -118,7 +159,8 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   sz[TOPT_BUF_RHO] = sz[TOPT_BUF_RHOF] = sz[TOPT_BUF_DC] = sz[TOPT_BUF_DCF] = sz[TOPT_BUF_HS] = g.ne * 8;
   sz[TOPT_BUF_X] = 0;   // the PCG iterate is the FP64 solution u on both paths
   sz[TOPT_BUF_R] = sz[TOPT_BUF_Z] = sz[TOPT_BUF_P0] = sz[TOPT_BUF_P1] = sz[TOPT_BUF_AP] = g.nn * 2 * ts;
-  sz[TOPT_BUF_PRE] = (precond == TOPT_PC_JACOBI || precond == TOPT_PC_BLOCK2) ? g.nn * 2 * ts : 0;
+  sz[TOPT_BUF_PRE] = (precond == TOPT_PC_JACOBI || precond == TOPT_PC_BLOCK2 || precond == TOPT_PC_MG)
+                         ? g.nn * 2 * ts : 0;
   sz[TOPT_BUF_PRE2] = precond == TOPT_PC_BLOCK2 ? g.nn * ts : 0;
   sz[TOPT_BUF_S] = g.ne * ts;
   sz[TOPT_BUF_S64] = dtype == TOPT_F32 ? g.ne * 8 : 0;     // plain path: s64 aliases s
@@ -129,6 +171,12 @@ int make_layout(int nelx, int nely, int dtype, int precond, double rmin, Layout&
   sz[TOPT_BUF_PART] = std::max<int64_t>(tree_parts(g.ntree) * 2, 4 * int64_t(g.nb[G_PCG])) * 8;
   sz[TOPT_BUF_OPART] = std::max<int64_t>(tree_parts(g.nb[G_ELEM]), tree_parts(g.nb[G_OC]) * 32) * 8;
   sz[TOPT_BUF_CTR] = ctr_words(g) * 4;
+  if (precond == TOPT_PC_MG) {
+    MgLayout M;
+    const int rc = mg_layout(nelx, nely, ts, M);
+    if (rc) return rc;
+    sz[TOPT_BUF_MG] = M.total;
+  }
   int64_t o = 0;
   for (int i = 0; i < TOPT_BUF_COUNT; ++i) {
     L.off[i] = o;
@@ -253,6 +301,25 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
   P.oc = (OcState*)at(TOPT_BUF_OC);
   P.opart = (double*)at(TOPT_BUF_OPART);
   set_counters(P, (unsigned*)at(TOPT_BUF_CTR), g);
+  if (p.precond == TOPT_PC_MG) {
+    MgLayout M;
+    rc = mg_layout(p.nelx, p.nely, dtype == TOPT_F32 ? 4 : 8, M);
+    if (rc) return rc;
+    char* m = (char*)at(TOPT_BUF_MG);
+    P.mg_nlev = M.nlev;
+    P.mg_ncd = M.ncd;
+    P.mg_inv = (double*)(m + M.inv);
+    for (int l = 0; l < M.nlev; ++l) {
+      MgLevel& V = P.mg[l];
+      V.nx = M.nx[l];
+      V.ny = M.ny[l];
+      V.K = l > 0 ? (void*)(m + M.K[l]) : nullptr;
+      V.x = m + M.x[l];
+      V.r = m + M.r[l];
+      V.b = l > 0 ? (void*)(m + M.b[l]) : P.r;
+      V.dinv = l > 0 ? (void*)(m + M.d[l]) : P.pre;
+    }
+  }
   return TOPT_OK;
 }
 
@@ -335,7 +402,8 @@ __global__ void k_oc_out(const OcState* o, double* out) {
 struct topt_plan {
   int n = 0;
   int dtype = TOPT_F64;
-  bool any_elemblock = false;
+  bool three_slot = false;   // element-block Jacobi or multigrid: slots A, B, C
+  bool mg = false;           // multigrid (single problem): V-cycle between slots B and C
   bool kstruct = true;    // every problem's Ke follows the unit-square Q4 pattern
   int maxR = 0;
   int max_cg = 0;
@@ -390,13 +458,56 @@ struct GridRef {
 };
 GridRef grid_of(const topt_plan* pl, int g) { return GridRef{pl->nblk[g], pl->dmap[g]}; }
 
+// Multigrid (single-problem plans): the per-solve setup (Galerkin levels,
+// level diagonals, coarsest inverse) and one V-cycle z = M r.
+template <typename T>
+void mg_setup(const ProbDev& P, hipStream_t s) {
+  const int L = P.mg_nlev - 1;
+  for (int l = 0; l < L; ++l) {
+    const int nb = (int)cdiv(int64_t(P.mg[l + 1].nx) * P.mg[l + 1].ny, TOPT_BLOCK);
+    if (l == 0)
+      hipLaunchKernelGGL((k_mg_galerkin<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, P, l);
+    else
+      hipLaunchKernelGGL((k_mg_galerkin<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, P, l);
+  }
+  for (int l = 1; l <= L; ++l)
+    hipLaunchKernelGGL((k_mg_diag<T>), dim3(cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK)),
+                       dim3(TOPT_BLOCK), 0, s, P, l);
+  hipLaunchKernelGGL((k_mg_coarse_setup<T>), dim3(1), dim3(TOPT_BLOCK), 0, s, P);
+}
+
+template <typename T>
+void mg_vcycle(const ProbDev& P, bool kst, hipStream_t s) {
+  const int L = P.mg_nlev - 1;
+  auto nodes = [&](int l) { return cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK); };
+  const int nt = P.nb[G_PCG];
+  if (kst)
+    hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P);
+  else
+    hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P);
+  hipLaunchKernelGGL((k_mg_restrict<T>), dim3(nodes(1)), dim3(TOPT_BLOCK), 0, s, P, 0);
+  for (int l = 1; l < L; ++l) {
+    hipLaunchKernelGGL((k_mg_down<T>), dim3(nodes(l)), dim3(TOPT_BLOCK), 0, s, P, l);
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(nodes(l + 1)), dim3(TOPT_BLOCK), 0, s, P, l);
+  }
+  hipLaunchKernelGGL((k_mg_coarse<T>), dim3(cdiv(P.mg_ncd, TOPT_BLOCK / TOPT_WAVE)), dim3(TOPT_BLOCK), 0, s, P);
+  // level l's corrected iterate: the coarsest one in x, the others in r
+  auto xup = [&](int l) { return (const void*)(l == L ? P.mg[l].x : P.mg[l].r); };
+  for (int l = L - 1; l >= 1; --l)
+    hipLaunchKernelGGL((k_mg_up<T>), dim3(nodes(l)), dim3(TOPT_BLOCK), 0, s, P, l, xup(l + 1));
+  if (kst)
+    hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1));
+  else
+    hipLaunchKernelGGL((k_mg_up0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1));
+}
+
 template <typename T>
 int launch_slots(topt_plan* pl, int count, hipStream_t s) {
   const ProbDev dummy{};
   const int nb = pl->npcur;
   const int2* map = pl->pcur;
   if (nb == 0) {   // every solve has ended: the sequence numbers still advance
-    pl->q += (int64_t)(pl->any_elemblock ? 3 : 2) * count;
+    pl->q += (int64_t)(pl->three_slot ? 3 : 2) * count;
     pl->na += count;
     return TOPT_OK;
   }
@@ -413,7 +524,8 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
                          pa);
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                        (int)(pl->q++ & 1), pa ^ 1);
-    if (pl->any_elemblock)
+    if (pl->mg) mg_vcycle<T>(pl->host[0], pl->kstruct, s);
+    if (pl->three_slot)
       hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                          (int)(pl->q++ & 1), pa ^ 1);
   }
@@ -454,7 +566,7 @@ int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
       pl->pcur == pl->dpfull) {
     if ((rc = chunk_graph(pl))) return rc;
     HIP_TRY(hipGraphLaunch(pl->gexec, s));
-    const int per_it = pl->any_elemblock ? 3 : 2;
+    const int per_it = pl->three_slot ? 3 : 2;
     pl->q += (int64_t)per_it * TOPT_GRAPH_CHUNK;
     pl->na += TOPT_GRAPH_CHUNK;
   } else {
@@ -476,6 +588,7 @@ int launch_solve_setup(topt_plan* pl, int warm, int use_filtered, hipStream_t s)
                      grid_of(pl, G_ELEM).map, dummy, use_filtered);
   hipLaunchKernelGGL((k_precond<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, 0);
+  if (pl->mg) mg_setup<T>(pl->host[0], s);
   hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, warm, 0);
   LAUNCH_TRY();
@@ -631,6 +744,7 @@ int subset_setup(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_
   hipLaunchKernelGGL((k_scale<T, true>), dim3(ge.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, ge.map, dummy,
                      sens_filter ? 0 : 1);
   hipLaunchKernelGGL((k_precond<T, true>), dim3(gn.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, gn.map, dummy, 0);
+  if (pl->mg) mg_setup<T>(pl->host[ids[0]], s);
   hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(gn.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, gn.map, dummy, 2, 1);
   LAUNCH_TRY();
   return TOPT_OK;
@@ -923,12 +1037,18 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
       P.b0[g] = (int)maps[g].size();
       maps[g].insert(maps[g].end(), P.nb[g], i);
     }
-    if (i > 0 && (P.precond == PC_ELEMBLOCK) != pl->any_elemblock) {
+    const bool three = P.precond == PC_ELEMBLOCK || P.precond == PC_MG;
+    if (i > 0 && three != pl->three_slot) {
       delete pl;
-      return fail(TOPT_EINVAL, "a plan cannot mix the element-block preconditioner with others "
-                               "(its PCG iteration has three slots)");
+      return fail(TOPT_EINVAL, "a plan cannot mix the element-block or multigrid preconditioner with "
+                               "others (their PCG iteration has three slots)");
     }
-    pl->any_elemblock = (P.precond == PC_ELEMBLOCK);
+    if (P.precond == PC_MG && nprob > 1) {
+      delete pl;
+      return fail(TOPT_EINVAL, "a multigrid plan holds one problem");
+    }
+    pl->three_slot = three;
+    pl->mg = P.precond == PC_MG;
     pl->kstruct = pl->kstruct && P.kstruct;
     pl->maxR = std::max(pl->maxR, P.R);
     pl->max_cg = std::max(pl->max_cg, P.max_it);

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, cons
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int lb;
   PD& P = grid_prob<B>(tab, bmap, single, b, lb);
-  if (P.precond != PC_JACOBI && P.precond != PC_BLOCK2) return;
+  if (P.precond != PC_JACOBI && P.precond != PC_BLOCK2 && P.precond != PC_MG) return;
   const NodeIdx id = node_of(P, lb);
   if (!id.act) return;
   const T* s = (const T*)P.s;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_precond(const ProbDev* tab, cons
     }
   const unsigned fb = fixbits(P.fix, id.n);
   vec2<T> out;
-  if (P.precond == PC_JACOBI) {
+  if (P.precond == PC_JACOBI || P.precond == PC_MG) {   // MG: the level-0 smoother
     dxx = (fb & 1u) ? T(1) : (dxx < T(1e-20) ? T(1e-20) : dxx);
     dyy = (fb & 2u) ? T(1) : (dyy < T(1e-20) ? T(1e-20) : dyy);
     out.x = raw ? dxx : T(1) / dxx;
@@ -488,7 +488,7 @@ __device__ __forceinline__ void refresh_done(CgState& s, double rzn) {
 template <typename T>
 __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)[2]) {
   constexpr bool MIXED = sizeof(T) == 4;
-  const bool eb = P.precond == PC_ELEMBLOCK;
+  const bool eb = P.precond == PC_ELEMBLOCK || P.precond == PC_MG;   // z comes from slot C
   switch (s.op) {
     case OP_START:
       s.op = OP_A_REF;
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
     }
   }
   if (st.status != ST_RUN) return;
-  const bool eb = pc == PC_ELEMBLOCK;
+  const bool eb = pc == PC_ELEMBLOCK || pc == PC_MG;
   double v0 = 0.0, v1 = 0.0;
   if (st.op == OP_B_REF) {
 #pragma unroll
@@ -886,8 +886,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
   slot_end(P, lb, q, v0, v1, sh);
 }
 
-// Slot C (element-block Jacobi only): z = M^-1 r by gather, partial r.z
-// (fem/preconditioner.py:47-108 apply_and_rz_2d).
+// Slot C (element-block Jacobi and multigrid): z = M^-1 r by gather, partial
+// r.z (fem/preconditioner.py:47-108 apply_and_rz_2d); for multigrid the
+// V-cycle launched between slots B and C has written z, and slot C forms r.z.
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int2* pmap,
                                                        ProbDev single, int q, int par) {
@@ -907,9 +908,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
 #pragma unroll
   for (int j = 0; j < Tl::NPT; ++j) {
     const NodeIdx id = tile_node<T>(P, o, j);
-    const vec2<T> zz = elemblock_node<T>(P, id, r);
+    vec2<T> zz;
+    if (P.precond == PC_MG) zz = gp((const vec2<T>*)P.z)[id.n];
+    else zz = elemblock_node<T>(P, id, r);
     if (!id.act) continue;
-    gp((vec2<T>*)P.z)[id.n] = zz;
+    if (P.precond != PC_MG) gp((vec2<T>*)P.z)[id.n] = zz;
     const vec2<T> rv = r[id.n];
     v += (double)rv.x * (double)zz.x + (double)rv.y * (double)zz.y;
   }

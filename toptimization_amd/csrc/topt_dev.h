@@ -20,7 +20,7 @@
 namespace topt {
 
 // ---------------------------------------------------------------- constants
-enum Precond : int { PC_NONE = 0, PC_JACOBI = 1, PC_ELEMBLOCK = 2, PC_BLOCK2 = 3 };
+enum Precond : int { PC_NONE = 0, PC_JACOBI = 1, PC_ELEMBLOCK = 2, PC_BLOCK2 = 3, PC_MG = 4 };
 enum Status : int { ST_RUN = 0, ST_CONV = 1, ST_BREAK = 2, ST_MAXIT = 3, ST_ZERO = 4, ST_DONE = 5 };
 enum Mode : int { MODE_CG = 0, MODE_REFINE = 1 };
 enum FilterMode : int { F_DENSITY = 0, F_DCAVG = 1, F_SENS = 2, F_TRANSPOSE = 3, F_HS = 4 };
@@ -60,6 +60,20 @@ enum Grid : int { G_NODE = 0, G_ELEM = 1, G_FILT = 2, G_OC = 3, G_PCG = 4 };
 #endif
 template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? TOPT_NPT_F32 : 2; };
 
+// One level of the geometric multigrid hierarchy (PC_MG). Level 0 is the
+// design grid; level l+1 has ceil(nx/2) x ceil(ny/2) elements, each the
+// agglomerate of a 2x2 patch of level-l elements (those past the level-l
+// grid count as void). Node vectors hold 2 T per node, as the PCG vectors.
+#define TOPT_MG_MAXLEV 12
+struct MgLevel {
+  int nx, ny;     // elements
+  void* K;        // levels >= 1: Galerkin element matrices, 36 T each (packed upper triangle)
+  void* x;        // pre-smoothed iterate
+  void* b;        // right-hand side (level 0: the PCG residual r)
+  void* r;        // residual; after restriction it receives the post-smoothed iterate
+  void* dinv;     // 1 / diag (level 0: the Jacobi data in pre); 0 on decoupled DOFs
+};
+
 struct ProbDev {
   int nelx, nely;
   int nb[TOPT_NGRID], b0[TOPT_NGRID];
@@ -88,6 +102,11 @@ struct ProbDev {
   CgState* cg; double* part;   // cg[2]; partials: PCG slots 2 x nb[G_PCG] x 2, node-grid tree
   OcState* oc; double* opart;  // elem / OC-grid tree partials
   unsigned* ctr[4];            // tree counters per grid (G_NODE, G_ELEM, -, G_OC)
+  // multigrid preconditioner (PC_MG): levels 0..mg_nlev-1, the last one
+  // solved with the dense inverse mg_inv (mg_ncd x mg_ncd, FP64)
+  int mg_nlev, mg_ncd;
+  double* mg_inv;
+  MgLevel mg[TOPT_MG_MAXLEV];
 };
 
 // ------------------------------------------------------------ vector types

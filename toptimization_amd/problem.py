@@ -8,7 +8,8 @@ numbering (node ``(ix, iy)`` -> ``ix*(nely+1)+iy``, DOF ``dim*node + comp``).
 Additions for the GPU engine (documented, all optional in YAML):
 
 * ``solver.preconditioner`` also accepts ``block_jacobi_2x2`` (nodal 2x2 block
-  Jacobi, no reference counterpart).
+  Jacobi) and ``multigrid`` (geometric multigrid V-cycle with Galerkin coarse
+  levels; one design per plan); neither has a reference counterpart.
 * ``solver.precision``: ``fp64`` (plain FP64 PCG, default) or ``mixed``
   (FP32 matvec/vectors, FP64 dots and FP64 true-residual refinement).
 * ``make_problem(...)`` builds a ``Problem`` programmatically (serving requests).
@@ -23,7 +24,7 @@ from typing import Literal
 import numpy as np
 import yaml
 
-PRECONDITIONERS = ("jacobi", "block_jacobi", "none", "block_jacobi_2x2")
+PRECONDITIONERS = ("jacobi", "block_jacobi", "none", "block_jacobi_2x2", "multigrid")
 PRECISIONS = ("fp64", "mixed")
 
 
@@ -249,8 +250,8 @@ def problem_from_dict(raw: dict) -> Problem:
     precond = sol.get("preconditioner", "jacobi")
     if precond not in PRECONDITIONERS:
         raise ValueError(
-            "preconditioner must be 'jacobi', 'block_jacobi', 'none' or "
-            f"'block_jacobi_2x2', got {precond}")
+            "preconditioner must be 'jacobi', 'block_jacobi', 'none', "
+            f"'block_jacobi_2x2' or 'multigrid', got {precond}")
     precision = sol.get("precision", "fp64")
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision}")
