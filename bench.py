@@ -53,32 +53,36 @@ def bridge(nelx, nely, precision, preconditioner="block_jacobi_2x2", tol=1e-6):
         precision=precision)
 
 
-def run_path(precision, warmup, steps, log):
+MAIN_PC = "multigrid"   # preconditioner of the headline path
+
+
+def run_path(precision, warmup, steps, log, preconditioner=MAIN_PC):
     import torch
     from toptimization_amd.engine import DesignState, Plan
-    prob = bridge(NELX, NELY, precision)
+    prob = bridge(NELX, NELY, precision, preconditioner=preconditioner)
     st = DesignState(prob)
     plan = Plan([st])
+    tag = f"{precision} {preconditioner}"
     for k in range(warmup):
         s = plan.iteration()[0]
-        log(f"  [{precision}] warmup it {k + 1}: c={s.compliance:.6e} n_cg={s.n_cg} refines={s.refines}")
+        log(f"  [{tag}] warmup it {k + 1}: c={s.compliance:.6e} n_cg={s.n_cg} refines={s.refines}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = [plan.iteration()[0] for _ in range(steps)]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ncg = sum(s.n_cg for s in stats)
-    log(f"  [{precision}] timed {steps} it: {dt:.3f} s, {ncg} CG iterations, "
+    log(f"  [{tag}] timed {steps} it: {dt:.3f} s ({steps / dt:.2f} OC it/s), {ncg} CG iterations, "
         f"last c={stats[-1].compliance:.6e}")
     return {"seconds": dt, "n_cg": ncg, "stats": stats, "state": st, "plan": plan, "problem": prob}
 
 
-def same_rho_compliance(rho, log):
+def same_rho_compliance(rho, log, preconditioner=MAIN_PC):
     """Solve the same density with the mixed and the plain FP64 path; compare c."""
     from toptimization_amd.engine import DesignState, Plan
     out = {}
     for prec in ("mixed", "fp64"):
-        st = DesignState(bridge(NELX, NELY, prec), rho0=rho)
+        st = DesignState(bridge(NELX, NELY, prec, preconditioner=preconditioner), rho0=rho)
         plan = Plan([st])
         plan.filter(0)
         it, status = plan.solve(warm=0, use_filtered=True)
@@ -299,7 +303,7 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="asy
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
@@ -335,10 +339,16 @@ def main() -> int:
         return 0
     if args.probe:
         from toptimization_amd.engine import DesignState, Plan
+        st = DesignState(bridge(NELX, NELY, "mixed", preconditioner="multigrid"))
+        plan = Plan([st])
+        for k in range(3):
+            s = plan.iteration()[0]
+            log(f"[probe] multigrid OC iteration {k + 1}: c={s.compliance:.6e} n_cg={s.n_cg}")
+        plan.close()
         st = DesignState(bridge(NELX, NELY, "mixed"))
         plan = Plan([st])
         s = plan.iteration()[0]
-        log(f"[probe] OC iteration 1: c={s.compliance:.6e} n_cg={s.n_cg}")
+        log(f"[probe] 2x2 block Jacobi OC iteration 1: c={s.compliance:.6e} n_cg={s.n_cg}")
         kernel_roofline(plan, nn, ne, log, reps=100)
         matvec_gdofs(NELX, NELY, log, reps=50)
         matvec_gdofs(8192, 4096, log, reps=10)
@@ -361,6 +371,10 @@ def main() -> int:
     mixed["plan"].close()
     fp64 = run_path("fp64", args.warmup, args.steps, log)
     fp64["plan"].close()
+    # the same workload with the nodal 2x2 block-Jacobi preconditioner (a
+    # few iterations: ~4,800 CG iterations each)
+    bj = run_path("mixed", 1, min(args.steps, 5), log, preconditioner="block_jacobi_2x2")
+    bj["plan"].close()
     rel_c = same_rho_compliance(rho_probe, log)
     mv_ms, mv_gdofs, mv_gbs = matvec_gdofs(NELX, NELY, log)
     mvb_ms, mvb_gdofs, mvb_gbs = matvec_gdofs(8192, 4096, log, reps=20)
@@ -383,7 +397,7 @@ def main() -> int:
         "dtype": "f32 vectors + f64 dots/refresh/design fields",
         "data": "synthetic (bridge load case, rho0 = volfrac)",
         "config": {"workload": f"SIMP bridge {NELX}x{NELY} Q4, volfrac 0.3, rmin 6, p 3, "
-                               "PCG rtol 1e-6 mixed FP32/FP64 2x2 block Jacobi",
+                               "PCG rtol 1e-6 mixed FP32/FP64, geometric multigrid V(1,1) preconditioner",
                    "nelx": NELX, "nely": NELY, "dofs": 2 * nn, "elements": ne,
                    "parallelism": "single-gpu"},
         "pcg_its_per_s": round(mixed["n_cg"] / mixed["seconds"], 1),
@@ -391,6 +405,11 @@ def main() -> int:
         "fp64_oc_its_per_s": round(fp64_value, 3),
         "fp64_cg_per_oc_it": round(fp64["n_cg"] / args.steps, 1),
         "mixed_over_fp64": round(value / fp64_value, 3),
+        "us_per_cg_it": round(1e6 * mixed["seconds"] / max(mixed["n_cg"], 1), 1),
+        "block_jacobi_2x2_oc_its_per_s": round(len(bj["stats"]) / bj["seconds"], 3),
+        "block_jacobi_2x2_cg_per_oc_it": round(bj["n_cg"] / len(bj["stats"]), 1),
+        "block_jacobi_2x2_oc_iterations": f"2-{1 + len(bj['stats'])}",
+        "multigrid_over_block_jacobi_2x2": round(value / (len(bj["stats"]) / bj["seconds"]), 2),
         "same_rho_compliance_rel_err": rel_c,
         "matvec_gdofs": round(mv_gdofs, 2),
         "matvec_gbs_algorithmic": round(mv_gbs, 1),

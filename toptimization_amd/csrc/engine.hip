@@ -425,6 +425,8 @@ struct topt_plan {
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t mgraph = nullptr;       // multigrid plans: one 2-iteration chunk
+  hipGraphExec_t mgexec = nullptr;
   // PCG grid map of (problem, local tile) pairs: the full map, and two
   // compacted copies (alternating, each filled from its own pinned staging
   // buffer) that leave out problems whose solve has ended
@@ -477,28 +479,32 @@ void mg_setup(const ProbDev& P, hipStream_t s) {
 }
 
 template <typename T>
-void mg_vcycle(const ProbDev& P, bool kst, hipStream_t s) {
+void mg_vcycle(const ProbDev& P, bool kst, int par, hipStream_t s) {
   const int L = P.mg_nlev - 1;
-  auto nodes = [&](int l) { return cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK); };
+  auto nodes = [&](int l) { return int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
+  auto blocks = [&](int l) { return (int)cdiv(nodes(l), TOPT_BLOCK); };
+  // first level handled by the single-workgroup tail
+  int lt = 1;
+  while (lt < L && nodes(lt) > TOPT_MG_TAIL) ++lt;
   const int nt = P.nb[G_PCG];
   if (kst)
-    hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, par);
   else
-    hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P);
-  hipLaunchKernelGGL((k_mg_restrict<T>), dim3(nodes(1)), dim3(TOPT_BLOCK), 0, s, P, 0);
-  for (int l = 1; l < L; ++l) {
-    hipLaunchKernelGGL((k_mg_down<T>), dim3(nodes(l)), dim3(TOPT_BLOCK), 0, s, P, l);
-    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(nodes(l + 1)), dim3(TOPT_BLOCK), 0, s, P, l);
+    hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, par);
+  hipLaunchKernelGGL((k_mg_restrict<T>), dim3(blocks(1)), dim3(TOPT_BLOCK), 0, s, P, 0, par);
+  for (int l = 1; l < lt; ++l) {
+    hipLaunchKernelGGL((k_mg_down<T>), dim3(blocks(l)), dim3(TOPT_BLOCK), 0, s, P, l, par);
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(blocks(l + 1)), dim3(TOPT_BLOCK), 0, s, P, l, par);
   }
-  hipLaunchKernelGGL((k_mg_coarse<T>), dim3(cdiv(P.mg_ncd, TOPT_BLOCK / TOPT_WAVE)), dim3(TOPT_BLOCK), 0, s, P);
+  hipLaunchKernelGGL((k_mg_tail<T>), dim3(1), dim3(TOPT_MG_TAIL_BLOCK), 0, s, P, lt, par);
   // level l's corrected iterate: the coarsest one in x, the others in r
   auto xup = [&](int l) { return (const void*)(l == L ? P.mg[l].x : P.mg[l].r); };
-  for (int l = L - 1; l >= 1; --l)
-    hipLaunchKernelGGL((k_mg_up<T>), dim3(nodes(l)), dim3(TOPT_BLOCK), 0, s, P, l, xup(l + 1));
+  for (int l = lt - 1; l >= 1; --l)
+    hipLaunchKernelGGL((k_mg_up<T>), dim3(blocks(l)), dim3(TOPT_BLOCK), 0, s, P, l, xup(l + 1), par);
   if (kst)
-    hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1));
+    hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1), par);
   else
-    hipLaunchKernelGGL((k_mg_up0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1));
+    hipLaunchKernelGGL((k_mg_up0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1), par);
 }
 
 template <typename T>
@@ -524,7 +530,7 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
                          pa);
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                        (int)(pl->q++ & 1), pa ^ 1);
-    if (pl->mg) mg_vcycle<T>(pl->host[0], pl->kstruct, s);
+    if (pl->mg) mg_vcycle<T>(pl->host[0], pl->kstruct, (int)(pl->q & 1), s);
     if (pl->three_slot)
       hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                          (int)(pl->q++ & 1), pa ^ 1);
@@ -534,18 +540,20 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
 }
 
 #define TOPT_GRAPH_CHUNK 32
+#define TOPT_MG_CHUNK 2   // CG iterations per chunk of a multigrid solve
 
-// The slot launches of one full chunk, captured into a graph. Every full chunk
-// starts at an even sequence number and an even slot-A count (the ramp-up
-// chunks 4 + 4 + 8 + 16 = 32 iterations leave both even), so its kernel
+// The slot launches of one full chunk (count iterations), captured into a
+// graph. Every full chunk starts at an even sequence number and an even
+// slot-A count (the ramp-up chunks 4 + 4 + 8 + 16 = 32 iterations leave both
+// even; multigrid chunks are 2 iterations of 3 slots), so its kernel
 // arguments are the same each time and one graph serves the whole solve.
-int chunk_graph(topt_plan* pl) {
-  if (pl->gexec) return TOPT_OK;
+int chunk_graph(topt_plan* pl, int count, hipGraph_t* graph, hipGraphExec_t* exec) {
+  if (*exec) return TOPT_OK;
   if (!pl->cap) HIP_TRY(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
   const int64_t q0 = pl->q, na0 = pl->na;
   HIP_TRY(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
-  const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, TOPT_GRAPH_CHUNK, pl->cap)
-                                       : launch_slots<double>(pl, TOPT_GRAPH_CHUNK, pl->cap);
+  const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, pl->cap)
+                                       : launch_slots<double>(pl, count, pl->cap);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(pl->cap, &g);
   pl->q = q0;     // capturing launched nothing
@@ -555,20 +563,28 @@ int chunk_graph(topt_plan* pl) {
     return rc;
   }
   if (e != hipSuccess) return fail(TOPT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  pl->graph = g;
-  HIP_TRY(hipGraphInstantiate(&pl->gexec, pl->graph, nullptr, nullptr, 0));
+  *graph = g;
+  HIP_TRY(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
   return TOPT_OK;
 }
 
 int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   int rc;
-  if (pl->use_graph && count == TOPT_GRAPH_CHUNK && (pl->q & 1) == 0 && (pl->na & 1) == 0 &&
-      pl->pcur == pl->dpfull) {
-    if ((rc = chunk_graph(pl))) return rc;
-    HIP_TRY(hipGraphLaunch(pl->gexec, s));
+  const bool whole = (pl->q & 1) == 0 && (pl->na & 1) == 0 && pl->pcur == pl->dpfull;
+  const bool user_graph = pl->use_graph && count == TOPT_GRAPH_CHUNK && whole;
+  // a multigrid iteration is ~28 launches: replaying them cuts the host's share
+  const bool mg_graph = pl->mg && count == TOPT_MG_CHUNK && whole;
+  if (user_graph || mg_graph) {
+    if (user_graph) {
+      if ((rc = chunk_graph(pl, count, &pl->graph, &pl->gexec))) return rc;
+      HIP_TRY(hipGraphLaunch(pl->gexec, s));
+    } else {
+      if ((rc = chunk_graph(pl, count, &pl->mgraph, &pl->mgexec))) return rc;
+      HIP_TRY(hipGraphLaunch(pl->mgexec, s));
+    }
     const int per_it = pl->three_slot ? 3 : 2;
-    pl->q += (int64_t)per_it * TOPT_GRAPH_CHUNK;
-    pl->na += TOPT_GRAPH_CHUNK;
+    pl->q += (int64_t)per_it * count;
+    pl->na += count;
   } else {
     rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, s) : launch_slots<double>(pl, count, s);
     if (rc) return rc;
@@ -637,7 +653,10 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
   // slots issued so far; a solve needs at most max_cg CG slots plus refresh slots
   const int64_t slot_cap = 4 * int64_t(pl->max_cg) + 4096;
   int64_t issued = 0;
-  int chunk = 4;
+  // multigrid solves take tens of iterations of ~100 us: short chunks keep
+  // the overshoot past convergence small
+  int chunk = pl->mg ? TOPT_MG_CHUNK : 4;
+  const int chunk_max = pl->mg ? TOPT_MG_CHUNK : TOPT_GRAPH_CHUNK;
   for (int k = 0; k < 2; ++k) {
     rc = enqueue_chunk(pl, chunk, k, s);
     if (rc) return rc;
@@ -672,7 +691,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
       pl->pcur = pl->dpmap[k];
       pl->npcur = m;
     }
-    chunk = std::min(chunk * 2, 32);
+    chunk = std::min(chunk * 2, chunk_max);
     rc = enqueue_chunk(pl, chunk, wait, s);
     if (rc) return rc;
     issued += chunk;
@@ -850,7 +869,7 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
   bool any_live = pl->npcur > 0;
   if (any_live) {
     for (int k = 0; k < 2; ++k) {
-      if ((rc = enqueue_chunk(pl, TOPT_ASYNC_CHUNK, (int)(cnext & 1), s))) return rc;
+      if ((rc = enqueue_chunk(pl, pl->mg ? TOPT_MG_CHUNK : TOPT_ASYNC_CHUNK, (int)(cnext & 1), s))) return rc;
       ++cnext;
     }
   }
@@ -904,7 +923,7 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
     for (int i = 0; i < n; ++i) any_live = any_live || pl->live[i];
     if (stop || !any_live) break;
     if (changed && (rc = rebuild_pcg_map(pl, s))) return rc;
-    if ((rc = enqueue_chunk(pl, TOPT_ASYNC_CHUNK, wait, s))) return rc;
+    if ((rc = enqueue_chunk(pl, pl->mg ? TOPT_MG_CHUNK : TOPT_ASYNC_CHUNK, wait, s))) return rc;
     ++cnext;
     wait ^= 1;
   }
@@ -1144,6 +1163,8 @@ int topt_plan_destroy(topt_plan* pl) {
   (void)hipFree(pl->dpar);
   if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->mgexec) (void)hipGraphExecDestroy(pl->mgexec);
+  if (pl->mgraph) (void)hipGraphDestroy(pl->mgraph);
   if (pl->cap) (void)hipStreamDestroy(pl->cap);
   delete pl;
   return TOPT_OK;

@@ -21,7 +21,7 @@
 
 namespace topt {
 
-#define TOPT_MG_DENSE 128      // max DOFs of the coarsest level
+#define TOPT_MG_DENSE 64       // max DOFs of the coarsest level (inverted in LDS)
 #define TOPT_MG_OMEGA 0.6      // Jacobi damping
 
 // Packed index of entry (i, j) of a symmetric 8x8 matrix (upper triangle,
@@ -146,16 +146,17 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(ProbDev single, int l) {
   gp((vec2<T>*)P.mg[l].dinv)[n] = d;
 }
 
-// Dense inverse of the coarsest operator (one workgroup): assemble into
-// mg_inv (decoupled DOFs get a unit diagonal), then Gauss-Jordan in place
-// without pivoting (the matrix is symmetric positive definite).
+// Dense inverse of the coarsest operator (one workgroup, in LDS): assemble
+// (decoupled DOFs get a unit diagonal), Gauss-Jordan in place without
+// pivoting (the matrix is symmetric positive definite), store to mg_inv.
 template <typename T>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse_setup(ProbDev single) {
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   const int L = P.mg_nlev - 1;
   const int nx = P.mg[L].nx, ny = P.mg[L].ny, nny = ny + 1;
-  const int N = P.mg_ncd;
-  GLB double* A = gp(P.mg_inv);
+  const int N = P.mg_ncd;   // <= TOPT_MG_DENSE
+  __shared__ double A[TOPT_MG_DENSE * TOPT_MG_DENSE];
+  __shared__ double piv;
   const GLB T* K = gp((const T*)P.mg[L].K);
   for (int idx = threadIdx.x; idx < N * N; idx += TOPT_BLOCK) {
     const int i = idx / N, j = idx - (idx / N) * N;
@@ -177,36 +178,42 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse_setup(ProbDev single) 
   }
   __syncthreads();
   for (int i = threadIdx.x; i < N; i += TOPT_BLOCK)
-    if (!(A[(long)i * N + i] > 0.0)) A[(long)i * N + i] = 1.0;
+    if (!(A[i * N + i] > 0.0)) A[i * N + i] = 1.0;
   __syncthreads();
-  __shared__ double piv;
   for (int k = 0; k < N; ++k) {
-    if (threadIdx.x == 0) piv = 1.0 / A[(long)k * N + k];
+    if (threadIdx.x == 0) piv = 1.0 / A[k * N + k];
     __syncthreads();
     const double p = piv;
     for (int j = threadIdx.x; j < N; j += TOPT_BLOCK)
-      if (j != k) A[(long)k * N + j] *= p;
+      if (j != k) A[k * N + j] *= p;
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += TOPT_BLOCK) {
-      if (i == k) continue;
-      GLB double* row = A + (long)i * N;
-      const double f = row[k];
-      const GLB double* rk = A + (long)k * N;
-      for (int j = 0; j < N; ++j)
-        if (j != k) row[j] = fma(-f, rk[j], row[j]);
-      row[k] = -f * p;
+    // entries (i, j), i != k: a_ij -= a_ik a_kj (j != k), a_ik = -a_ik p
+    for (int idx = threadIdx.x; idx < N * N; idx += TOPT_BLOCK) {
+      const int i = idx / N, j = idx - (idx / N) * N;
+      if (i == k || j == k) continue;
+      A[idx] = fma(-A[i * N + k], A[k * N + j], A[idx]);
     }
     __syncthreads();
-    if (threadIdx.x == 0) A[(long)k * N + k] = p;
+    for (int i = threadIdx.x; i < N; i += TOPT_BLOCK)
+      if (i != k) A[i * N + k] *= -p;
+    if (threadIdx.x == 0) A[k * N + k] = p;
     __syncthreads();
   }
+  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_BLOCK) gp(P.mg_inv)[idx] = A[idx];
 }
 
 // ---------------------------------------------------------------- V-cycle
+// The V-cycle kernels run between PCG slots B and C; cg[par] is the state
+// slot B wrote. Once the solve has stopped they return at once (the host
+// enqueues whole chunks of iterations).
+__device__ __forceinline__ bool mg_stopped(PD& P, int par) {
+  return cp((const CgState*)P.cg)[par].status != ST_RUN;
+}
 // Coarsest level: x = A^-1 b, one wavefront per row.
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse(ProbDev single) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse(ProbDev single, int par) {
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
   const int L = P.mg_nlev - 1;
   const int N = P.mg_ncd;
   const int row = (int)(blockIdx.x * (TOPT_BLOCK / TOPT_WAVE) + (threadIdx.x >> 6));
@@ -220,14 +227,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse(ProbDev single) {
   if (lane == 0) gp((T*)P.mg[L].x)[row] = (T)acc;
 }
 
-// Restriction b_{l+1} = P^T r_l (level 0: fixed DOFs masked).
+// Restriction b_{l+1} = P^T r_l at coarse node N (level 0: fixed DOFs masked).
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(ProbDev single, int l) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__device__ __forceinline__ void mg_restrict_at(PD& P, int l, long N) {
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny;
   const int cny = P.mg[l + 1].ny;
-  const long N = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
-  if (N >= (long)(P.mg[l + 1].nx + 1) * (cny + 1)) return;
   const int I = (int)(N / (cny + 1)), J = (int)(N - (long)I * (cny + 1));
   const GLB vec2<T>* r = gp((const vec2<T>*)P.mg[l].r);
   double bx = 0.0, by = 0.0;
@@ -253,6 +257,15 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(ProbDev single, int 
   o.x = (T)bx;
   o.y = (T)by;
   gp((vec2<T>*)P.mg[l + 1].b)[N] = o;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(ProbDev single, int l, int par) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
+  const long N = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  if (N >= (long)(P.mg[l + 1].nx + 1) * (P.mg[l + 1].ny + 1)) return;
+  mg_restrict_at<T>(P, l, N);
 }
 
 // Interpolated coarse correction (P x_c) at fine node (i, j); xc holds the
@@ -308,13 +321,11 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, int ix, int iy, XF xv
   return d2v{ax, ay};
 }
 
-// Down sweep at level l >= 1: x = omega D^-1 b (from zero), r = b - K x.
+// Down sweep at node n of level l >= 1: x = omega D^-1 b (from zero),
+// r = b - K x.
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(ProbDev single, int l) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__device__ __forceinline__ void mg_down_at(PD& P, int l, long n) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
-  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
-  if (n >= (long)(nx + 1) * nny) return;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
   const GLB vec2<T>* b = gp((const vec2<T>*)P.mg[l].b);
   const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
@@ -338,18 +349,23 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(ProbDev single, int l) {
   gp((vec2<T>*)P.mg[l].r)[n] = ro;
 }
 
-// Up sweep at level l >= 1 (not the coarsest): x' = x + P x_{l+1},
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(ProbDev single, int l, int par) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
+  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
+  mg_down_at<T>(P, l, n);
+}
+
+// Up sweep at node n of level l >= 1 (not the coarsest): x' = x + P x_{l+1},
 // out = x' + omega D^-1 (b - K x'), written to r (free after restriction).
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(ProbDev single, int l, const void* xc_) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__device__ __forceinline__ void mg_up_at(PD& P, int l, const GLB vec2<T>* xc, long n) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
   const int cny = P.mg[l + 1].ny;
-  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
-  if (n >= (long)(nx + 1) * nny) return;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
   const GLB vec2<T>* x = gp((const vec2<T>*)P.mg[l].x);
-  const GLB vec2<T>* xc = gp((const vec2<T>*)xc_);
   d2v xs[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
@@ -370,13 +386,60 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(ProbDev single, int l, con
   gp((vec2<T>*)P.mg[l].r)[n] = o;
 }
 
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(ProbDev single, int l, const void* xc_, int par) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
+  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
+  mg_up_at<T>(P, l, gp((const vec2<T>*)xc_), n);
+}
+
+// Levels lt..L (the small ones) in one workgroup: down sweeps and
+// restrictions to the coarsest level, its dense solve, up sweeps back to lt.
+// Phases are separated by workgroup barriers (all waves of a workgroup share
+// one CU and its L1, so global writes before a barrier are seen after it).
+// b of level lt is written by the restriction launched before this kernel.
+#define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
+#define TOPT_MG_TAIL_BLOCK 1024  // each phase is one memory round trip)
+template <typename T>
+__global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(ProbDev single, int lt, int par) {
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
+  const int L = P.mg_nlev - 1;
+  auto nodes = [&](int l) { return (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
+  for (int l = lt; l < L; ++l) {
+    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_down_at<T>(P, l, n);
+    __syncthreads();
+    for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK) mg_restrict_at<T>(P, l, n);
+    __syncthreads();
+  }
+  {
+    const int N = P.mg_ncd;
+    const GLB T* b = gp((const T*)P.mg[L].b);
+    for (int row = threadIdx.x; row < N; row += TOPT_MG_TAIL_BLOCK) {
+      const GLB double* A = gp(P.mg_inv) + (long)row * N;
+      double acc = 0.0;
+      for (int j = 0; j < N; ++j) acc = fma(A[j], (double)b[j], acc);
+      gp((T*)P.mg[L].x)[row] = (T)acc;
+    }
+  }
+  __syncthreads();
+  for (int l = L - 1; l >= lt; --l) {
+    const GLB vec2<T>* xc = gp((const vec2<T>*)(l + 1 == L ? P.mg[L].x : P.mg[l + 1].r));
+    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_up_at<T>(P, l, xc, n);
+    __syncthreads();
+  }
+}
+
 // Level 0 down sweep on the PCG tiles: x0 = omega D^-1 r, r1 = r - K(s) x0
 // (fixed rows identity, so r1 = 0 there).
 template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(ProbDev single) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(ProbDev single, int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -419,10 +482,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(ProbDev single) {
 // Level 0 up sweep: x' = x0 + (masked) P x_1, z = x' + omega D^-1 (r - K x'),
 // z = 0 on fixed DOFs.
 template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(ProbDev single, const void* xc_) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(ProbDev single, const void* xc_, int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  if (mg_stopped(P, par)) return;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);

@@ -64,10 +64,11 @@ template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? TOP
 // design grid; level l+1 has ceil(nx/2) x ceil(ny/2) elements, each the
 // agglomerate of a 2x2 patch of level-l elements (those past the level-l
 // grid count as void). Node vectors hold 2 T per node, as the PCG vectors.
-#define TOPT_MG_MAXLEV 12
+#define TOPT_MG_MAXLEV 14
 struct MgLevel {
   int nx, ny;     // elements
-  void* K;        // levels >= 1: Galerkin element matrices, 36 T each (packed upper triangle)
+  void* K;        // levels >= 1: Galerkin element matrices, 36 T each (packed upper
+                  // triangle, element-major: one element's entries share a cache line)
   void* x;        // pre-smoothed iterate
   void* b;        // right-hand side (level 0: the PCG residual r)
   void* r;        // residual; after restriction it receives the post-smoothed iterate

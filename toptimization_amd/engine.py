@@ -75,7 +75,7 @@ class DesignState:
     """
 
     def __init__(self, problem: Problem, precision: str | None = None,
-                 preconditioner: str | None = None, delta: float = 0.1,
+                 preconditioner: str | None = None, delta: float | None = None,
                  device: torch.device | None = None, rho0: np.ndarray | None = None,
                  Ke: np.ndarray | None = None, f: np.ndarray | None = None):
         if problem.dim != 2:
@@ -125,6 +125,12 @@ class DesignState:
         d.cg_tol = float(problem.cg_tol)
         d.volfrac, d.move, d.rmin = float(problem.volfrac), float(problem.move_limit), float(problem.rmin)
         d.fnorm2 = float(f_np @ f_np)
+        # reliable-update ratio of the mixed path: the FP64 residual is
+        # recomputed when the recursive one has fallen by delta. Multigrid
+        # solves converge in tens of iterations, so a refresh every three
+        # decades costs less than the drift it prevents.
+        if delta is None:
+            delta = 1e-3 if self.precond_id == _lib.PC_MG else 0.1
         d.delta = float(delta)
         d.Ke[:] = [float(v) for v in self.Ke.ravel()]
         d.ws = self.ws.data_ptr()
