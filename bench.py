@@ -163,7 +163,7 @@ def cpu_baseline(log, budget_s=25.0):
                       f"restatement of the reference algorithm"}
 
 
-def batch_bench(log, n=64, iters=100, seq_sample=None):
+def batch_bench(log, n=64, iters=100, seq_sample=None, preconditioner=MAIN_PC):
     """64 independent 128x64 cantilevers (seeded loads, volfrac U[0.3, 0.6],
     rmin 2.5, mixed FP32/FP64 PCG rtol 1e-6, 2x2 block Jacobi) advanced
     together by one plan (one launch per kernel for all of them) against the
@@ -173,7 +173,7 @@ def batch_bench(log, n=64, iters=100, seq_sample=None):
     from toptimization_amd.engine import DesignState, Plan
     from toptimization_amd.serving import batch_cantilevers
     probs = batch_cantilevers(n, max_iter=iters, cg_tolerance=1e-6, max_cg_iterations=50000,
-                              preconditioner="block_jacobi_2x2", precision="mixed")
+                              preconditioner=preconditioner, precision="mixed")
     states = [DesignState(p) for p in probs]
     plan = Plan(states)
     plan.iteration()   # warm-up: first iteration (setup, cold caches)
@@ -208,6 +208,7 @@ def batch_bench(log, n=64, iters=100, seq_sample=None):
     seq = m * (iters - 1) / ts
     log(f"  sequential ({m} problems): {seq:.1f} OC it/s")
     return {"metric": f"batched OC it/s, {n} x 128x64 cantilevers", "value": round(batched, 2),
+            "preconditioner": preconditioner,
             "unit": "OC it/s", "sequential_oc_its_per_s": round(seq, 2),
             "sequential_sample": m, "speedup": round(batched / seq, 2), "oc_iterations_each": iters - 1,
             "cg_iterations_batched": ncg, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
@@ -265,7 +266,7 @@ def mbb_bench(log, iters=100):
             "dtype": "f32 vectors + f64 dots/refresh/design fields", "data": "synthetic"}
 
 
-def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="async"):
+def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="async", preconditioner=MAIN_PC):
     """Serving stream: n seeded requests, Poisson arrivals at `rate`/s, nelx
     uniform over multiples of 16 in [64, 768], aspect 1/4, 1/3 or 1/2, BC
     cantilever / MBB / bridge, volfrac U[0.3, 0.5], rmin 0.008 nelx + 1, run to
@@ -275,7 +276,7 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="asy
     Sequential baseline: the first seq_sample requests of the same stream
     served one at a time (designs/hour of that sample)."""
     from toptimization_amd.serving import DesignServer, synthetic_stream
-    solver = dict(cg_tolerance=1e-6, max_cg_iterations=50000, preconditioner="block_jacobi_2x2",
+    solver = dict(cg_tolerance=1e-6, max_cg_iterations=50000, preconditioner=preconditioner,
                   precision="mixed")
     reqs = synthetic_stream(n=n, rate=rate, seed=0, nelx_range=(64, 768), max_iter=100, tol=0.01, **solver)
     server = DesignServer(max_active=max_active, hbm_budget=64 << 30, precision="mixed", mode=mode)
@@ -296,6 +297,7 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="asy
             "unit": "designs/hour", "p50_latency_s": round(s["p50_latency_s"], 3),
             "p99_latency_s": round(s["p99_latency_s"], 3), "makespan_s": round(s["makespan_s"], 2),
             "requests": n, "arrival_rate_per_s": rate, "max_active": s["max_active"], "mode": mode,
+            "preconditioner": preconditioner,
             "engine_steps": s["engine_steps"], "sequential_designs_per_hour": None if seq is None else round(seq, 1),
             "sequential_sample": seq_sample, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
 
@@ -318,6 +320,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--serving-mode", choices=["async", "lockstep"], default="async")
+    ap.add_argument("--pc", default=None, help="preconditioner of the --batch / --serving runs "
+                    "(default: multigrid)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); a design problem is "
@@ -329,13 +333,14 @@ def main() -> int:
     torch.cuda.set_device(0)
     nn, ne = (NELX + 1) * (NELY + 1), NELX * NELY
     if args.batch:
-        print(json.dumps(batch_bench(log)), flush=True)
+        print(json.dumps(batch_bench(log, preconditioner=args.pc or MAIN_PC)), flush=True)
         return 0
     if args.mbb:
         print(json.dumps(mbb_bench(log)), flush=True)
         return 0
     if args.serving:
-        print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample, mode=args.serving_mode)), flush=True)
+        print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample, mode=args.serving_mode,
+                                         preconditioner=args.pc or MAIN_PC)), flush=True)
         return 0
     if args.probe:
         from toptimization_amd.engine import DesignState, Plan

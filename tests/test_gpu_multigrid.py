@@ -97,10 +97,61 @@ def test_mg_async_equals_lockstep():
     np.testing.assert_array_equal(st.rho.cpu().numpy(), ref.rho.cpu().numpy())
 
 
-def test_mg_plan_holds_one_problem():
-    from toptimization_amd._lib import ToptError
+def test_mg_batch_equals_solo():
+    """Ragged batch of multigrid problems (different depths and tail levels):
+    every problem's iterates are bitwise those of its solo plan."""
     from toptimization_amd.engine import DesignState, Plan
-    prob = mbb(32, 16, preconditioner="multigrid")
-    a, b = DesignState(prob, precision="mixed"), DesignState(prob, precision="mixed")
-    with pytest.raises(ToptError):
-        Plan([a, b])
+    from toptimization_amd.serving import synthetic_stream
+    reqs = synthetic_stream(n=5, rate=100.0, seed=21, nelx_range=(32, 400), max_iter=6, tol=0.0,
+                            cg_tolerance=1e-6, max_cg_iterations=20000, preconditioner="multigrid",
+                            precision="mixed")
+    states = [DesignState(r.problem, precision="mixed") for r in reqs]
+    plan = Plan(states)
+    out = [plan.iteration() for _ in range(6)]
+    plan.close()
+    for i, r in enumerate(reqs):
+        print(f"multigrid batch member {i}: {r.problem.nelx}x{r.problem.nely}", flush=True)
+        st = DesignState(r.problem, precision="mixed")
+        p1 = Plan([st])
+        solo = [p1.iteration()[0] for _ in range(6)]
+        p1.close()
+        assert [o[i].compliance for o in out] == [x.compliance for x in solo]
+        assert [o[i].n_cg for o in out] == [x.n_cg for x in solo]
+        np.testing.assert_array_equal(states[i].rho.cpu().numpy(), st.rho.cpu().numpy())
+
+
+def test_mg_serving_async_equals_solo():
+    from toptimization_amd.engine import DesignState, Plan
+    from toptimization_amd.serving import DesignServer, synthetic_stream
+    reqs = synthetic_stream(n=6, rate=50.0, seed=4, nelx_range=(32, 160), max_iter=10, tol=0.01,
+                            cg_tolerance=1e-6, max_cg_iterations=20000, preconditioner="multigrid",
+                            precision="mixed")
+    server = DesignServer(max_active=3, hbm_budget=1 << 30, precision="mixed", keep_rho=True)
+    stats = server.run(reqs, clock="wall")
+    assert stats.summary()["designs"] == 6
+    for res in stats.results:
+        st = DesignState(reqs[res.rid].problem, precision="mixed")
+        plan = Plan([st])
+        for _ in range(res.iterations):
+            o = plan.iteration()[0]
+        plan.close()
+        assert o.compliance == res.compliance, res.rid
+        np.testing.assert_array_equal(st.rho.cpu().numpy(), res.rho)
+
+
+def test_mg_with_elemblock_in_one_plan():
+    """Multigrid and element-block Jacobi share the three-slot iteration and
+    can share a plan; each problem matches its solo run."""
+    from toptimization_amd.engine import DesignState, Plan
+    probs = [mbb(48, 16, preconditioner="multigrid", cg_tolerance=1e-8, max_cg_iterations=20000),
+             mbb(40, 20, preconditioner="block_jacobi", cg_tolerance=1e-8, max_cg_iterations=20000)]
+    states = [DesignState(p, precision="fp64") for p in probs]
+    plan = Plan(states)
+    both = [plan.iteration() for _ in range(3)]
+    plan.close()
+    for i, p in enumerate(probs):
+        st = DesignState(p, precision="fp64")
+        p1 = Plan([st])
+        solo = [p1.iteration()[0] for _ in range(3)]
+        p1.close()
+        assert [b[i].compliance for b in both] == [x.compliance for x in solo]

@@ -308,6 +308,10 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
     char* m = (char*)at(TOPT_BUF_MG);
     P.mg_nlev = M.nlev;
     P.mg_ncd = M.ncd;
+    const int L = M.nlev - 1;
+    int lt = 1;   // first level small enough for the single-workgroup tail
+    while (lt < L && int64_t(M.nx[lt] + 1) * (M.ny[lt] + 1) > TOPT_MG_TAIL) ++lt;
+    P.mg_lt = lt;
     P.mg_inv = (double*)(m + M.inv);
     for (int l = 0; l < M.nlev; ++l) {
       MgLevel& V = P.mg[l];
@@ -403,7 +407,17 @@ struct topt_plan {
   int n = 0;
   int dtype = TOPT_F64;
   bool three_slot = false;   // element-block Jacobi or multigrid: slots A, B, C
-  bool mg = false;           // multigrid (single problem): V-cycle between slots B and C
+  bool mg = false;           // some problem uses multigrid: V-cycle between slots B and C
+  // multigrid V-cycle maps, (problem, block) pairs: down / up sweeps at level
+  // l (problems with l < their tail level), restrictions l -> l+1 (problems
+  // with l < their tail level; grid of level l+1), and one block per
+  // multigrid problem for the tail
+  int mg_lt_max = 0;
+  int2* mg_dn[TOPT_MG_MAXLEV] = {};
+  int2* mg_rs[TOPT_MG_MAXLEV] = {};
+  int n_dn[TOPT_MG_MAXLEV] = {}, n_rs[TOPT_MG_MAXLEV] = {};
+  int2* mg_tail = nullptr;
+  int n_tail = 0;
   bool kstruct = true;    // every problem's Ke follows the unit-square Q4 pattern
   int maxR = 0;
   int max_cg = 0;
@@ -479,32 +493,29 @@ void mg_setup(const ProbDev& P, hipStream_t s) {
 }
 
 template <typename T>
-void mg_vcycle(const ProbDev& P, bool kst, int par, hipStream_t s) {
-  const int L = P.mg_nlev - 1;
-  auto nodes = [&](int l) { return int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
-  auto blocks = [&](int l) { return (int)cdiv(nodes(l), TOPT_BLOCK); };
-  // first level handled by the single-workgroup tail
-  int lt = 1;
-  while (lt < L && nodes(lt) > TOPT_MG_TAIL) ++lt;
-  const int nt = P.nb[G_PCG];
-  if (kst)
-    hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, par);
+void mg_vcycle(topt_plan* pl, int par, hipStream_t s) {
+  const ProbDev* tab = pl->dtab;
+  const int nt = pl->npcur;        // live problems' tiles (the PCG map)
+  const int2* tmap = pl->pcur;
+  if (pl->kstruct)
+    hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
   else
-    hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, par);
-  hipLaunchKernelGGL((k_mg_restrict<T>), dim3(blocks(1)), dim3(TOPT_BLOCK), 0, s, P, 0, par);
-  for (int l = 1; l < lt; ++l) {
-    hipLaunchKernelGGL((k_mg_down<T>), dim3(blocks(l)), dim3(TOPT_BLOCK), 0, s, P, l, par);
-    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(blocks(l + 1)), dim3(TOPT_BLOCK), 0, s, P, l, par);
+    hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
+  hipLaunchKernelGGL((k_mg_restrict<T>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[0], 0, par);
+  for (int l = 1; l < pl->mg_lt_max; ++l) {
+    if (pl->n_dn[l] > 0)
+      hipLaunchKernelGGL((k_mg_down<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_dn[l], l, par);
+    if (pl->n_rs[l] > 0)
+      hipLaunchKernelGGL((k_mg_restrict<T>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[l], l, par);
   }
-  hipLaunchKernelGGL((k_mg_tail<T>), dim3(1), dim3(TOPT_MG_TAIL_BLOCK), 0, s, P, lt, par);
-  // level l's corrected iterate: the coarsest one in x, the others in r
-  auto xup = [&](int l) { return (const void*)(l == L ? P.mg[l].x : P.mg[l].r); };
-  for (int l = lt - 1; l >= 1; --l)
-    hipLaunchKernelGGL((k_mg_up<T>), dim3(blocks(l)), dim3(TOPT_BLOCK), 0, s, P, l, xup(l + 1), par);
-  if (kst)
-    hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1), par);
+  hipLaunchKernelGGL((k_mg_tail<T>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, tab, pl->mg_tail, par);
+  for (int l = pl->mg_lt_max - 1; l >= 1; --l)
+    if (pl->n_dn[l] > 0)
+      hipLaunchKernelGGL((k_mg_up<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_dn[l], l, par);
+  if (pl->kstruct)
+    hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
   else
-    hipLaunchKernelGGL((k_mg_up0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, P, xup(1), par);
+    hipLaunchKernelGGL((k_mg_up0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
 }
 
 template <typename T>
@@ -530,7 +541,7 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
                          pa);
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                        (int)(pl->q++ & 1), pa ^ 1);
-    if (pl->mg) mg_vcycle<T>(pl->host[0], pl->kstruct, (int)(pl->q & 1), s);
+    if (pl->mg) mg_vcycle<T>(pl, (int)(pl->q & 1), s);
     if (pl->three_slot)
       hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                          (int)(pl->q++ & 1), pa ^ 1);
@@ -604,7 +615,8 @@ int launch_solve_setup(topt_plan* pl, int warm, int use_filtered, hipStream_t s)
                      grid_of(pl, G_ELEM).map, dummy, use_filtered);
   hipLaunchKernelGGL((k_precond<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, 0);
-  if (pl->mg) mg_setup<T>(pl->host[0], s);
+  for (const ProbDev& P : pl->host)
+    if (P.precond == PC_MG) mg_setup<T>(P, s);
   hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, warm, 0);
   LAUNCH_TRY();
@@ -763,7 +775,8 @@ int subset_setup(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_
   hipLaunchKernelGGL((k_scale<T, true>), dim3(ge.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, ge.map, dummy,
                      sens_filter ? 0 : 1);
   hipLaunchKernelGGL((k_precond<T, true>), dim3(gn.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, gn.map, dummy, 0);
-  if (pl->mg) mg_setup<T>(pl->host[ids[0]], s);
+  for (int i : ids)
+    if (pl->host[i].precond == PC_MG) mg_setup<T>(pl->host[i], s);
   hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(gn.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, gn.map, dummy, 2, 1);
   LAUNCH_TRY();
   return TOPT_OK;
@@ -1062,12 +1075,8 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
       return fail(TOPT_EINVAL, "a plan cannot mix the element-block or multigrid preconditioner with "
                                "others (their PCG iteration has three slots)");
     }
-    if (P.precond == PC_MG && nprob > 1) {
-      delete pl;
-      return fail(TOPT_EINVAL, "a multigrid plan holds one problem");
-    }
     pl->three_slot = three;
-    pl->mg = P.precond == PC_MG;
+    pl->mg = pl->mg || P.precond == PC_MG;
     pl->kstruct = pl->kstruct && P.kstruct;
     pl->maxR = std::max(pl->maxR, P.R);
     pl->max_cg = std::max(pl->max_cg, P.max_it);
@@ -1131,6 +1140,34 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   if (hipMalloc(&pl->dpar, sizeof(int) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc parity"));
   pl->pcur = pl->dpfull;
   pl->npcur = npcg;
+  if (pl->mg) {
+    std::vector<int2> dn[TOPT_MG_MAXLEV], rs[TOPT_MG_MAXLEV], tail;
+    for (int i = 0; i < nprob; ++i) {
+      const ProbDev& P = pl->host[i];
+      if (P.precond != PC_MG) continue;
+      auto blocks = [&](int l) { return (int)cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK); };
+      for (int l = 0; l < P.mg_lt; ++l) {
+        if (l >= 1)
+          for (int b = 0; b < blocks(l); ++b) dn[l].push_back(make_int2(i, b));
+        for (int b = 0; b < blocks(l + 1); ++b) rs[l].push_back(make_int2(i, b));
+      }
+      tail.push_back(make_int2(i, 0));
+      pl->mg_lt_max = std::max(pl->mg_lt_max, P.mg_lt);
+    }
+    auto upload = [&](const std::vector<int2>& v, int2** d, int* n) {
+      *n = (int)v.size();
+      if (v.empty()) return TOPT_OK;
+      if (hipMalloc(d, sizeof(int2) * v.size()) != hipSuccess) return fail(TOPT_ENOMEM, "hipMalloc multigrid map");
+      if (hipMemcpy(*d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(TOPT_EHIP, "upload multigrid map");
+      return TOPT_OK;
+    };
+    int rc;
+    for (int l = 0; l < TOPT_MG_MAXLEV; ++l)
+      if ((rc = upload(dn[l], &pl->mg_dn[l], &pl->n_dn[l])) || (rc = upload(rs[l], &pl->mg_rs[l], &pl->n_rs[l])))
+        return cleanup(rc);
+    if ((rc = upload(tail, &pl->mg_tail, &pl->n_tail))) return cleanup(rc);
+  }
   *out = pl;
   return TOPT_OK;
 }
@@ -1165,6 +1202,11 @@ int topt_plan_destroy(topt_plan* pl) {
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
   if (pl->mgexec) (void)hipGraphExecDestroy(pl->mgexec);
   if (pl->mgraph) (void)hipGraphDestroy(pl->mgraph);
+  for (int l = 0; l < TOPT_MG_MAXLEV; ++l) {
+    (void)hipFree(pl->mg_dn[l]);
+    (void)hipFree(pl->mg_rs[l]);
+  }
+  (void)hipFree(pl->mg_tail);
   if (pl->cap) (void)hipStreamDestroy(pl->cap);
   delete pl;
   return TOPT_OK;

@@ -207,26 +207,22 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse_setup(ProbDev single) 
 // slot B wrote. Once the solve has stopped they return at once (the host
 // enqueues whole chunks of iterations).
 __device__ __forceinline__ bool mg_stopped(PD& P, int par) {
-  return cp((const CgState*)P.cg)[par].status != ST_RUN;
+  // the level-0 sweeps use the PCG tile map, which may hold problems of
+  // the plan with another three-slot preconditioner
+  return P.precond != PC_MG || cp((const CgState*)P.cg)[par].status != ST_RUN;
 }
-// Coarsest level: x = A^-1 b, one wavefront per row.
-template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse(ProbDev single, int par) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
-  if (mg_stopped(P, par)) return;
-  const int L = P.mg_nlev - 1;
-  const int N = P.mg_ncd;
-  const int row = (int)(blockIdx.x * (TOPT_BLOCK / TOPT_WAVE) + (threadIdx.x >> 6));
-  if (row >= N) return;   // whole wavefronts leave together
-  const int lane = threadIdx.x & 63;
-  const GLB double* A = gp(P.mg_inv) + (long)row * N;
-  const GLB T* b = gp((const T*)P.mg[L].b);
-  double acc = 0.0;
-  for (int j = lane; j < N; j += TOPT_WAVE) acc = fma(A[j], (double)b[j], acc);
-  acc = wave_sum(acc);
-  if (lane == 0) gp((T*)P.mg[L].x)[row] = (T)acc;
+// The V-cycle kernels cover a batch through (problem, local block) maps: one
+// map per level and kind, holding only the problems that take that step.
+__device__ __forceinline__ PD& mg_entry(const ProbDev* tab, const int2* map, int b, int& lb) {
+  const int2 e = map[b];
+  lb = e.y;
+  return *(PD*)(tab + e.x);
 }
-
+// Level l's corrected iterate after its up sweep: the coarsest level's in x,
+// the others' in r.
+__device__ __forceinline__ const void* mg_xup(PD& P, int l) {
+  return l == P.mg_nlev - 1 ? P.mg[l].x : P.mg[l].r;
+}
 // Restriction b_{l+1} = P^T r_l at coarse node N (level 0: fixed DOFs masked).
 template <typename T>
 __device__ __forceinline__ void mg_restrict_at(PD& P, int l, long N) {
@@ -260,10 +256,11 @@ __device__ __forceinline__ void mg_restrict_at(PD& P, int l, long N) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(ProbDev single, int l, int par) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(const ProbDev* tab, const int2* map, int l, int par) {
+  int lb;
+  PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
-  const long N = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  const long N = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (N >= (long)(P.mg[l + 1].nx + 1) * (P.mg[l + 1].ny + 1)) return;
   mg_restrict_at<T>(P, l, N);
 }
@@ -350,10 +347,11 @@ __device__ __forceinline__ void mg_down_at(PD& P, int l, long n) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(ProbDev single, int l, int par) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(const ProbDev* tab, const int2* map, int l, int par) {
+  int lb;
+  PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
-  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
   mg_down_at<T>(P, l, n);
 }
@@ -387,25 +385,29 @@ __device__ __forceinline__ void mg_up_at(PD& P, int l, const GLB vec2<T>* xc, lo
 }
 
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(ProbDev single, int l, const void* xc_, int par) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const int2* map, int l, int par) {
+  int lb;
+  PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
-  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
-  mg_up_at<T>(P, l, gp((const vec2<T>*)xc_), n);
+  mg_up_at<T>(P, l, gp((const vec2<T>*)mg_xup(P, l + 1)), n);
 }
 
 // Levels lt..L (the small ones) in one workgroup: down sweeps and
 // restrictions to the coarsest level, its dense solve, up sweeps back to lt.
 // Phases are separated by workgroup barriers (all waves of a workgroup share
 // one CU and its L1, so global writes before a barrier are seen after it).
-// b of level lt is written by the restriction launched before this kernel.
+// b of level lt (P.mg_lt) is written by the restriction launched before
+// this kernel. One workgroup per problem.
 #define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
 #define TOPT_MG_TAIL_BLOCK 1024  // each phase is one memory round trip)
 template <typename T>
-__global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(ProbDev single, int lt, int par) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+__global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
+  int lb;
+  PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
+  const int lt = P.mg_lt;
   const int L = P.mg_nlev - 1;
   auto nodes = [&](int l) { return (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
   for (int l = lt; l < L; ++l) {
@@ -435,14 +437,14 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(ProbDev single, 
 // Level 0 down sweep on the PCG tiles: x0 = omega D^-1 r, r1 = r - K(s) x0
 // (fixed rows identity, so r1 = 0 there).
 template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(ProbDev single, int par) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(const ProbDev* tab, const int2* map, int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  int lb;
+  PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   if (mg_stopped(P, par)) return;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
   const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
@@ -482,17 +484,17 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(ProbDev single, int par
 // Level 0 up sweep: x' = x0 + (masked) P x_1, z = x' + omega D^-1 (r - K x'),
 // z = 0 on fixed DOFs.
 template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(ProbDev single, const void* xc_, int par) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const ProbDev* tab, const int2* map, int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  int lb;
+  PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   if (mg_stopped(P, par)) return;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* x0 = gp((const vec2<T>*)P.mg[0].x);
-  const GLB vec2<T>* xc = gp((const vec2<T>*)xc_);
+  const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, 1));
   const int cny = P.mg[1].ny;
   TileShare<T> S;
   tile_share<T>(P, o, S);

@@ -106,6 +106,7 @@ struct ProbDev {
   // multigrid preconditioner (PC_MG): levels 0..mg_nlev-1, the last one
   // solved with the dense inverse mg_inv (mg_ncd x mg_ncd, FP64)
   int mg_nlev, mg_ncd;
+  int mg_lt, mg_pad;          // first level of the single-workgroup tail
   double* mg_inv;
   MgLevel mg[TOPT_MG_MAXLEV];
 };

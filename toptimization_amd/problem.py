@@ -9,7 +9,7 @@ Additions for the GPU engine (documented, all optional in YAML):
 
 * ``solver.preconditioner`` also accepts ``block_jacobi_2x2`` (nodal 2x2 block
   Jacobi) and ``multigrid`` (geometric multigrid V-cycle with Galerkin coarse
-  levels; one design per plan); neither has a reference counterpart.
+  levels); neither has a reference counterpart.
 * ``solver.precision``: ``fp64`` (plain FP64 PCG, default) or ``mixed``
   (FP32 matvec/vectors, FP64 dots and FP64 true-residual refinement).
 * ``make_problem(...)`` builds a ``Problem`` programmatically (serving requests).
