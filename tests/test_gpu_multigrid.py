@@ -155,3 +155,27 @@ def test_mg_with_elemblock_in_one_plan():
         solo = [p1.iteration()[0] for _ in range(3)]
         p1.close()
         assert [b[i].compliance for b in both] == [x.compliance for x in solo]
+
+
+def test_mg_deterministic_beyond_one_wave_of_workgroups():
+    """1024x320 (390 PCG tiles, more workgroups than the GPU runs at once):
+    the same design alone and next to another one in a batch (so its
+    workgroups run at other times) gives bitwise the same iterates; a kernel
+    that read values other workgroups of its own launch write would not."""
+    from toptimization_amd.engine import DesignState, Plan
+    big = bridge(1024, 320, preconditioner="multigrid", cg_tolerance=1e-6, max_cg_iterations=20000,
+                 volume_fraction=0.3, filter_radius=5.0, max_iterations=3, tolerance=0.0, precision="mixed")
+    other = mbb(256, 96, preconditioner="multigrid", cg_tolerance=1e-6, max_cg_iterations=20000,
+                volume_fraction=0.5, filter_radius=3.0, max_iterations=3, tolerance=0.0, precision="mixed")
+    a = DesignState(big, precision="mixed")
+    p = Plan([a])
+    solo = [p.iteration()[0] for _ in range(2)]
+    p.close()
+    b, c = DesignState(other, precision="mixed"), DesignState(big, precision="mixed")
+    p = Plan([b, c])
+    both = [p.iteration()[1] for _ in range(2)]
+    p.close()
+    print(f"1024x320 multigrid CG per OC iteration: {[s.n_cg for s in solo]}", flush=True)
+    assert [s.n_cg for s in solo] == [s.n_cg for s in both]
+    assert [s.compliance for s in solo] == [s.compliance for s in both]
+    assert solo[0].n_cg <= 40

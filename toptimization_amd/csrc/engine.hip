@@ -408,6 +408,7 @@ struct topt_plan {
   int dtype = TOPT_F64;
   bool three_slot = false;   // element-block Jacobi or multigrid: slots A, B, C
   bool mg = false;           // some problem uses multigrid: V-cycle between slots B and C
+  bool all_mg = false;       // every problem does: slot C fused with the level-0 up sweep
   // multigrid V-cycle maps, (problem, block) pairs: down / up sweeps at level
   // l (problems with l < their tail level), restrictions l -> l+1 (problems
   // with l < their tail level; grid of level l+1), and one block per
@@ -492,8 +493,9 @@ void mg_setup(const ProbDev& P, hipStream_t s) {
   hipLaunchKernelGGL((k_mg_coarse_setup<T>), dim3(1), dim3(TOPT_BLOCK), 0, s, P);
 }
 
+// fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)
 template <typename T>
-void mg_vcycle(topt_plan* pl, int par, hipStream_t s) {
+void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
   const ProbDev* tab = pl->dtab;
   const int nt = pl->npcur;        // live problems' tiles (the PCG map)
   const int2* tmap = pl->pcur;
@@ -512,6 +514,7 @@ void mg_vcycle(topt_plan* pl, int par, hipStream_t s) {
   for (int l = pl->mg_lt_max - 1; l >= 1; --l)
     if (pl->n_dn[l] > 0)
       hipLaunchKernelGGL((k_mg_up<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_dn[l], l, par);
+  if (fused) return;
   if (pl->kstruct)
     hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
   else
@@ -539,9 +542,22 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
     else
       hipLaunchKernelGGL((k_slot_a<T, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, qa,
                          pa);
+    if (pl->all_mg) {
+      // slot C fused with the level-0 up sweep
+      hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
+                         (int)(pl->q++ & 1), pa ^ 1);
+      mg_vcycle<T>(pl, (int)(pl->q & 1), true, s);
+      if (pl->kstruct)
+        hipLaunchKernelGGL((k_slot_c_mg<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
+                           (int)(pl->q++ & 1));
+      else
+        hipLaunchKernelGGL((k_slot_c_mg<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
+                           (int)(pl->q++ & 1));
+      continue;
+    }
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                        (int)(pl->q++ & 1), pa ^ 1);
-    if (pl->mg) mg_vcycle<T>(pl, (int)(pl->q & 1), s);
+    if (pl->mg) mg_vcycle<T>(pl, (int)(pl->q & 1), false, s);
     if (pl->three_slot)
       hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                          (int)(pl->q++ & 1), pa ^ 1);
@@ -1077,6 +1093,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
     }
     pl->three_slot = three;
     pl->mg = pl->mg || P.precond == PC_MG;
+    pl->all_mg = (i == 0 || pl->all_mg) && P.precond == PC_MG;
     pl->kstruct = pl->kstruct && P.kstruct;
     pl->maxR = std::max(pl->maxR, P.R);
     pl->max_cg = std::max(pl->max_cg, P.max_it);

@@ -539,3 +539,82 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const ProbDev* tab, const
 }
 
 }  // namespace topt
+
+namespace topt {
+
+// ---------------------------------------------------------------- fused slot
+// Plans whose problems all use multigrid run the level-0 up sweep and slot C
+// as one kernel (one launch less per CG iteration; arithmetic of the
+// separate kernels). Slot B is not fused with the down sweep: that sweep
+// needs r at the tile's halo ring, which neighbouring workgroups of the
+// same launch update, so it would read old or new values depending on
+// workgroup timing.
+
+// Level-0 up sweep + slot C: z = x0 + P x_1 + omega D^-1 (r - K x'), z = 0
+// on fixed DOFs, partial r.z.
+template <typename T, bool KST>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, const int2* pmap, int q) {
+  using Tl = Tile<T>;
+  constexpr int NPT = Tl::NPT;
+  __shared__ d2v hal[Tl::HN];
+  __shared__ double esc[Tl::EN];
+  __shared__ double sh[16];
+  int lb;
+  PD& P = mg_entry(tab, pmap, xcd_remap(blockIdx.x, gridDim.x), lb);
+  const TileOrg o = tile_org<T>(P, lb);
+  const GLB vec2<T>* x0 = gp((const vec2<T>*)P.mg[0].x);
+  const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, 1));
+  const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
+  const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
+  const int cny = P.mg[1].ny;
+  // state-independent loads first
+  PrevParts pp;
+  parts_fetch(P, q, pp);
+  TileShare<T> S;
+  tile_share<T>(P, o, S);
+  d2v hv[NPT];
+  vec2<T> orr[NPT], od[NPT];
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    const vec2<T> xv = x0[S.ids[j].n];
+    d2v c = mg_prolong<T>(xc, cny, S.ids[j].ix, S.ids[j].iy);
+    if (S.fb[j] & 1u) c.x = 0.0;
+    if (S.fb[j] & 2u) c.y = 0.0;
+    hv[j] = d2v{(double)xv.x, (double)xv.y} + c;
+    orr[j] = r[S.ids[j].n];
+    od[j] = dv[S.ids[j].n];
+  }
+  d2v rv;
+  {
+    const long m = S.rn;
+    const int nny = P.nely + 1;
+    const int jx = (int)(m / nny), jy = (int)(m - (long)jx * nny);
+    const vec2<T> xv = x0[m];
+    d2v c = mg_prolong<T>(xc, cny, jx, jy);
+    const unsigned fb = fixbits(P.fix, m);
+    if (fb & 1u) c.x = 0.0;
+    if (fb & 2u) c.y = 0.0;
+    rv = d2v{(double)xv.x, (double)xv.y} + c;
+  }
+  bool upd;
+  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
+  if (st.status != ST_RUN) return;
+  tile_store<T>(S, hv, rv, hal, esc);
+  __syncthreads();
+  double ax[NPT], ay[NPT];
+  tile_apply<T, KST>(P, hal, esc, ax, ay);
+  GLB vec2<T>* z = gp((vec2<T>*)P.z);
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    if (!S.ids[j].act) continue;
+    vec2<T> zo;
+    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + TOPT_MG_OMEGA * (double)od[j].x * ((double)orr[j].x - ax[j]));
+    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + TOPT_MG_OMEGA * (double)od[j].y * ((double)orr[j].y - ay[j]));
+    z[S.ids[j].n] = zo;
+    v += (double)orr[j].x * (double)zo.x + (double)orr[j].y * (double)zo.y;
+  }
+  slot_end(P, lb, q, v, 0.0, sh);
+}
+
+}  // namespace topt
