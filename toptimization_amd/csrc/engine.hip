@@ -490,7 +490,7 @@ void mg_setup(const ProbDev& P, hipStream_t s) {
   for (int l = 1; l <= L; ++l)
     hipLaunchKernelGGL((k_mg_diag<T>), dim3(cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK)),
                        dim3(TOPT_BLOCK), 0, s, P, l);
-  hipLaunchKernelGGL((k_mg_coarse_setup<T>), dim3(1), dim3(TOPT_BLOCK), 0, s, P);
+  hipLaunchKernelGGL((k_mg_coarse_setup<T>), dim3(1), dim3(TOPT_MG_SETUP_BLOCK), 0, s, P);
 }
 
 // fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)

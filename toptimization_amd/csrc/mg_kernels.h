@@ -21,7 +21,7 @@
 
 namespace topt {
 
-#define TOPT_MG_DENSE 64       // max DOFs of the coarsest level (inverted in LDS)
+#define TOPT_MG_DENSE 128      // max DOFs of the coarsest level (inverted in LDS, FP64: 128 KB)
 #define TOPT_MG_OMEGA 0.6      // Jacobi damping
 
 // Packed index of entry (i, j) of a symmetric 8x8 matrix (upper triangle,
@@ -149,8 +149,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(ProbDev single, int l) {
 // Dense inverse of the coarsest operator (one workgroup, in LDS): assemble
 // (decoupled DOFs get a unit diagonal), Gauss-Jordan in place without
 // pivoting (the matrix is symmetric positive definite), store to mg_inv.
+#define TOPT_MG_SETUP_BLOCK 1024
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse_setup(ProbDev single) {
+__global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(ProbDev single) {
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   const int L = P.mg_nlev - 1;
   const int nx = P.mg[L].nx, ny = P.mg[L].ny, nny = ny + 1;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse_setup(ProbDev single) 
   __shared__ double A[TOPT_MG_DENSE * TOPT_MG_DENSE];
   __shared__ double piv;
   const GLB T* K = gp((const T*)P.mg[L].K);
-  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_BLOCK) {
+  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
     const int i = idx / N, j = idx - (idx / N) * N;
     const int ni = i >> 1, di = i & 1, nj = j >> 1, dj = j & 1;
     const int ix = ni / nny, iy = ni - (ni / nny) * nny;
@@ -177,29 +178,29 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_coarse_setup(ProbDev single) 
     A[idx] = v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < N; i += TOPT_BLOCK)
+  for (int i = threadIdx.x; i < N; i += TOPT_MG_SETUP_BLOCK)
     if (!(A[i * N + i] > 0.0)) A[i * N + i] = 1.0;
   __syncthreads();
   for (int k = 0; k < N; ++k) {
     if (threadIdx.x == 0) piv = 1.0 / A[k * N + k];
     __syncthreads();
     const double p = piv;
-    for (int j = threadIdx.x; j < N; j += TOPT_BLOCK)
+    for (int j = threadIdx.x; j < N; j += TOPT_MG_SETUP_BLOCK)
       if (j != k) A[k * N + j] *= p;
     __syncthreads();
     // entries (i, j), i != k: a_ij -= a_ik a_kj (j != k), a_ik = -a_ik p
-    for (int idx = threadIdx.x; idx < N * N; idx += TOPT_BLOCK) {
+    for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
       const int i = idx / N, j = idx - (idx / N) * N;
       if (i == k || j == k) continue;
       A[idx] = fma(-A[i * N + k], A[k * N + j], A[idx]);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += TOPT_BLOCK)
+    for (int i = threadIdx.x; i < N; i += TOPT_MG_SETUP_BLOCK)
       if (i != k) A[i * N + k] *= -p;
     if (threadIdx.x == 0) A[k * N + k] = p;
     __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_BLOCK) gp(P.mg_inv)[idx] = A[idx];
+  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) gp(P.mg_inv)[idx] = A[idx];
 }
 
 // ---------------------------------------------------------------- V-cycle
@@ -420,9 +421,11 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     const int N = P.mg_ncd;
     const GLB T* b = gp((const T*)P.mg[L].b);
     for (int row = threadIdx.x; row < N; row += TOPT_MG_TAIL_BLOCK) {
-      const GLB double* A = gp(P.mg_inv) + (long)row * N;
+      // the inverse is symmetric: read column `row` (consecutive threads,
+      // consecutive addresses)
+      const GLB double* A = gp(P.mg_inv) + row;
       double acc = 0.0;
-      for (int j = 0; j < N; ++j) acc = fma(A[j], (double)b[j], acc);
+      for (int j = 0; j < N; ++j) acc = fma(A[(long)j * N], (double)b[j], acc);
       gp((T*)P.mg[L].x)[row] = (T)acc;
     }
   }
