@@ -309,8 +309,17 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
     P.mg_nlev = M.nlev;
     P.mg_ncd = M.ncd;
     const int L = M.nlev - 1;
-    int lt = 1;   // first level small enough for the single-workgroup tail
-    while (lt < L && int64_t(M.nx[lt] + 1) * (M.ny[lt] + 1) > TOPT_MG_TAIL) ++lt;
+    // first level of the single-workgroup tail: small enough for one node
+    // per thread, and all tail levels' vectors within its LDS buffers
+    auto nodes = [&](int l) { return int64_t(M.nx[l] + 1) * (M.ny[l] + 1); };
+    auto tail_sum = [&](int from) {
+      int64_t t = 0;
+      for (int l = from; l <= L; ++l) t += nodes(l);
+      return t;
+    };
+    int lt = 1;
+    while (lt < L && (nodes(lt) > TOPT_MG_TAIL || tail_sum(lt) > TOPT_MG_TAIL_SUM)) ++lt;
+    if (tail_sum(lt) > TOPT_MG_TAIL_SUM) return fail(TOPT_EINVAL, "multigrid tail exceeds its LDS buffers");
     P.mg_lt = lt;
     P.mg_inv = (double*)(m + M.inv);
     for (int l = 0; l < M.nlev; ++l) {

@@ -231,13 +231,15 @@ __device__ __forceinline__ PD& mg_entry(const ProbDev* tab, const int2* map, int
 __device__ __forceinline__ const void* mg_xup(PD& P, int l) {
   return l == P.mg_nlev - 1 ? P.mg[l].x : P.mg[l].r;
 }
+// The per-node sweep bodies take their node-vector pointers as template
+// parameters: global memory in the per-level kernels, LDS in the tail.
+
 // Restriction b_{l+1} = P^T r_l at coarse node N (level 0: fixed DOFs masked).
-template <typename T>
-__device__ __forceinline__ void mg_restrict_at(PD& P, int l, long N) {
+template <typename T, typename VR, typename VB>
+__device__ __forceinline__ void mg_restrict_p(PD& P, int l, long N, VR r, VB bout) {
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny;
   const int cny = P.mg[l + 1].ny;
   const int I = (int)(N / (cny + 1)), J = (int)(N - (long)I * (cny + 1));
-  const GLB vec2<T>* r = gp((const vec2<T>*)P.mg[l].r);
   double bx = 0.0, by = 0.0;
 #pragma unroll
   for (int di = -1; di <= 1; ++di)
@@ -260,7 +262,7 @@ __device__ __forceinline__ void mg_restrict_at(PD& P, int l, long N) {
   vec2<T> o;
   o.x = (T)bx;
   o.y = (T)by;
-  gp((vec2<T>*)P.mg[l + 1].b)[N] = o;
+  bout[N] = o;
 }
 
 template <typename T>
@@ -270,13 +272,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(const ProbDev* tab, 
   if (mg_stopped(P, par)) return;
   const long N = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (N >= (long)(P.mg[l + 1].nx + 1) * (P.mg[l + 1].ny + 1)) return;
-  mg_restrict_at<T>(P, l, N);
+  mg_restrict_p<T>(P, l, N, gp((const vec2<T>*)P.mg[l].r), gp((vec2<T>*)P.mg[l + 1].b));
 }
 
 // Interpolated coarse correction (P x_c) at fine node (i, j); xc holds the
 // coarse level's iterate, cny its element rows.
-template <typename T>
-__device__ __forceinline__ d2v mg_prolong(const GLB vec2<T>* xc, int cny, int i, int j) {
+template <typename T, typename VC>
+__device__ __forceinline__ d2v mg_prolong(VC xc, int cny, int i, int j) {
   const int I = i >> 1, J = j >> 1;
   const bool ox = i & 1, oy = j & 1;
   const long c = (long)I * (cny + 1) + J;
@@ -328,11 +330,10 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, int ix, int iy, XF xv
 
 // Down sweep at node n of level l >= 1: x = omega D^-1 b (from zero),
 // r = b - K x.
-template <typename T>
-__device__ __forceinline__ void mg_down_at(PD& P, int l, long n) {
+template <typename T, typename VB, typename VX>
+__device__ __forceinline__ void mg_down_p(PD& P, int l, long n, VB b, VX x, VX r) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-  const GLB vec2<T>* b = gp((const vec2<T>*)P.mg[l].b);
   const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
   d2v xs[3][3];
 #pragma unroll
@@ -350,8 +351,8 @@ __device__ __forceinline__ void mg_down_at(PD& P, int l, long n) {
   xo.y = (T)xs[1][1].y;
   ro.x = (T)((double)bn.x - ax.x);
   ro.y = (T)((double)bn.y - ax.y);
-  gp((vec2<T>*)P.mg[l].x)[n] = xo;
-  gp((vec2<T>*)P.mg[l].r)[n] = ro;
+  x[n] = xo;
+  r[n] = ro;
 }
 
 template <typename T>
@@ -361,17 +362,16 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(const ProbDev* tab, cons
   if (mg_stopped(P, par)) return;
   const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
-  mg_down_at<T>(P, l, n);
+  mg_down_p<T>(P, l, n, gp((const vec2<T>*)P.mg[l].b), gp((vec2<T>*)P.mg[l].x), gp((vec2<T>*)P.mg[l].r));
 }
 
 // Up sweep at node n of level l >= 1 (not the coarsest): x' = x + P x_{l+1},
-// out = x' + omega D^-1 (b - K x'), written to r (free after restriction).
-template <typename T>
-__device__ __forceinline__ void mg_up_at(PD& P, int l, const GLB vec2<T>* xc, long n) {
+// out = x' + omega D^-1 (b - K x').
+template <typename T, typename VX, typename VC, typename VB, typename VO>
+__device__ __forceinline__ void mg_up_p(PD& P, int l, long n, VX x, VC xc, VB b, VO out) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
   const int cny = P.mg[l + 1].ny;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-  const GLB vec2<T>* x = gp((const vec2<T>*)P.mg[l].x);
   d2v xs[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
@@ -384,14 +384,15 @@ __device__ __forceinline__ void mg_up_at(PD& P, int l, const GLB vec2<T>* xc, lo
       xs[dx][dy] = d2v{(double)xv.x, (double)xv.y} + mg_prolong<T>(xc, cny, jx, jy);
     }
   const d2v ax = mg_apply_rows<T>(P, l, ix, iy, [&](int dx, int dy) { return xs[dx + 1][dy + 1]; });
-  const vec2<T> bn = gp((const vec2<T>*)P.mg[l].b)[n];
+  const vec2<T> bn = b[n];
   const vec2<T> dd = gp((const vec2<T>*)P.mg[l].dinv)[n];
   vec2<T> o;
   o.x = (T)(xs[1][1].x + TOPT_MG_OMEGA * (double)dd.x * ((double)bn.x - ax.x));
   o.y = (T)(xs[1][1].y + TOPT_MG_OMEGA * (double)dd.y * ((double)bn.y - ax.y));
-  gp((vec2<T>*)P.mg[l].r)[n] = o;
+  out[n] = o;
 }
 
+// The corrected iterate goes to r (free after the restriction).
 template <typename T>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const int2* map, int l, int par) {
   int lb;
@@ -399,48 +400,76 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   if (mg_stopped(P, par)) return;
   const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
-  mg_up_at<T>(P, l, gp((const vec2<T>*)mg_xup(P, l + 1)), n);
+  mg_up_p<T>(P, l, n, gp((const vec2<T>*)P.mg[l].x), gp((const vec2<T>*)mg_xup(P, l + 1)),
+             gp((const vec2<T>*)P.mg[l].b), gp((vec2<T>*)P.mg[l].r));
 }
 
 // Levels lt..L (the small ones) in one workgroup: down sweeps and
 // restrictions to the coarsest level, its dense solve, up sweeps back to lt.
-// Phases are separated by workgroup barriers (all waves of a workgroup share
-// one CU and its L1, so global writes before a barrier are seen after it).
-// b of level lt (P.mg_lt) is written by the restriction launched before
-// this kernel. One workgroup per problem.
+// The levels' x, b and r live in LDS, so a phase hands its results to the
+// next through LDS and a workgroup barrier; element matrices, 1/diag and
+// the coarse inverse are read from global memory. b of level lt comes from
+// the restriction launched before this kernel, and level lt's corrected
+// iterate goes back to global memory (mg_xup) for the up sweep after it.
+// One workgroup per problem.
 #define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
-#define TOPT_MG_TAIL_BLOCK 1024  // each phase is one memory round trip)
+#define TOPT_MG_TAIL_BLOCK 1024  // each phase is one round trip)
+#define TOPT_MG_TAIL_SUM 2560    // nodes of all tail levels together (LDS: 3 vectors each)
 template <typename T>
 __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
+  __shared__ vec2<T> sx[TOPT_MG_TAIL_SUM], sb[TOPT_MG_TAIL_SUM], sr[TOPT_MG_TAIL_SUM];
   int lb;
   PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
   const int lt = P.mg_lt;
   const int L = P.mg_nlev - 1;
   auto nodes = [&](int l) { return (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
+  int off[TOPT_MG_MAXLEV];
+  {
+    int o = 0;
+    for (int l = lt; l <= L; ++l) {
+      off[l] = o;
+      o += (int)nodes(l);
+    }
+  }
+  {
+    const GLB vec2<T>* bg = gp((const vec2<T>*)P.mg[lt].b);
+    for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) sb[off[lt] + n] = bg[n];
+  }
+  __syncthreads();
   for (int l = lt; l < L; ++l) {
-    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_down_at<T>(P, l, n);
+    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
+      mg_down_p<T>(P, l, n, (const vec2<T>*)(sb + off[l]), (vec2<T>*)(sx + off[l]), (vec2<T>*)(sr + off[l]));
     __syncthreads();
-    for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK) mg_restrict_at<T>(P, l, n);
+    for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK)
+      mg_restrict_p<T>(P, l, n, (const vec2<T>*)(sr + off[l]), (vec2<T>*)(sb + off[l + 1]));
     __syncthreads();
   }
   {
+    // x = A^-1 b; A is symmetric: read column `row` (consecutive threads,
+    // consecutive addresses)
     const int N = P.mg_ncd;
-    const GLB T* b = gp((const T*)P.mg[L].b);
+    const T* b = (const T*)(sb + off[L]);
+    T* x = (T*)(sx + off[L]);
     for (int row = threadIdx.x; row < N; row += TOPT_MG_TAIL_BLOCK) {
-      // the inverse is symmetric: read column `row` (consecutive threads,
-      // consecutive addresses)
       const GLB double* A = gp(P.mg_inv) + row;
       double acc = 0.0;
       for (int j = 0; j < N; ++j) acc = fma(A[(long)j * N], (double)b[j], acc);
-      gp((T*)P.mg[L].x)[row] = (T)acc;
+      x[row] = (T)acc;
     }
   }
   __syncthreads();
   for (int l = L - 1; l >= lt; --l) {
-    const GLB vec2<T>* xc = gp((const vec2<T>*)(l + 1 == L ? P.mg[L].x : P.mg[l + 1].r));
-    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_up_at<T>(P, l, xc, n);
+    const vec2<T>* xc = (const vec2<T>*)(l + 1 == L ? sx + off[L] : sr + off[l + 1]);
+    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
+      mg_up_p<T>(P, l, n, (const vec2<T>*)(sx + off[l]), xc, (const vec2<T>*)(sb + off[l]),
+                 (vec2<T>*)(sr + off[l]));
     __syncthreads();
+  }
+  {
+    const vec2<T>* src = (const vec2<T>*)(lt == L ? sx + off[L] : sr + off[lt]);
+    GLB vec2<T>* dst = gp((vec2<T>*)mg_xup(P, lt));
+    for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) dst[n] = src[n];
   }
 }
 
