@@ -512,7 +512,8 @@ void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
     hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
   else
     hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
-  hipLaunchKernelGGL((k_mg_restrict<T>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[0], 0, par);
+  if (pl->n_rs[0] > 0)
+    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[0], 0, par);
   for (int l = 1; l < pl->mg_lt_max; ++l) {
     if (pl->n_dn[l] > 0)
       hipLaunchKernelGGL((k_mg_down<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_dn[l], l, par);
@@ -1172,10 +1173,13 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
       const ProbDev& P = pl->host[i];
       if (P.precond != PC_MG) continue;
       auto blocks = [&](int l) { return (int)cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK); };
+      // down / up sweeps at levels 1 .. lt-1; restrictions l -> l+1 for
+      // l < lt-1 (the tail restricts lt-1 -> lt itself)
       for (int l = 0; l < P.mg_lt; ++l) {
         if (l >= 1)
           for (int b = 0; b < blocks(l); ++b) dn[l].push_back(make_int2(i, b));
-        for (int b = 0; b < blocks(l + 1); ++b) rs[l].push_back(make_int2(i, b));
+        if (l + 1 < P.mg_lt)
+          for (int b = 0; b < blocks(l + 1); ++b) rs[l].push_back(make_int2(i, b));
       }
       tail.push_back(make_int2(i, 0));
       pl->mg_lt_max = std::max(pl->mg_lt_max, P.mg_lt);

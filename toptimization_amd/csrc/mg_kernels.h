@@ -408,9 +408,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
 // restrictions to the coarsest level, its dense solve, up sweeps back to lt.
 // The levels' x, b and r live in LDS, so a phase hands its results to the
 // next through LDS and a workgroup barrier; element matrices, 1/diag and
-// the coarse inverse are read from global memory. b of level lt comes from
-// the restriction launched before this kernel, and level lt's corrected
-// iterate goes back to global memory (mg_xup) for the up sweep after it.
+// the coarse inverse are read from global memory. The kernel starts with
+// the restriction of level lt-1's residual (written by the sweep launched
+// before it), and level lt's corrected iterate goes back to global memory
+// (mg_xup) for the up sweep after it.
 // One workgroup per problem.
 #define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
 #define TOPT_MG_TAIL_BLOCK 1024  // each phase is one round trip)
@@ -432,10 +433,9 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
       o += (int)nodes(l);
     }
   }
-  {
-    const GLB vec2<T>* bg = gp((const vec2<T>*)P.mg[lt].b);
-    for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) sb[off[lt] + n] = bg[n];
-  }
+  // b of level lt: restriction of level lt-1's residual
+  for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK)
+    mg_restrict_p<T>(P, lt - 1, n, gp((const vec2<T>*)P.mg[lt - 1].r), (vec2<T>*)(sb + off[lt]));
   __syncthreads();
   for (int l = lt; l < L; ++l) {
     for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
