@@ -428,6 +428,10 @@ struct topt_plan {
   int n_dn[TOPT_MG_MAXLEV] = {}, n_rs[TOPT_MG_MAXLEV] = {};
   int2* mg_tail = nullptr;
   int n_tail = 0;
+  // setup maps: Galerkin l -> l+1 (level l+1 elements), diagonal of level l
+  int2* mg_ga[TOPT_MG_MAXLEV] = {};
+  int2* mg_dg[TOPT_MG_MAXLEV] = {};
+  int n_ga[TOPT_MG_MAXLEV] = {}, n_dg[TOPT_MG_MAXLEV] = {};
   bool kstruct = true;    // every problem's Ke follows the unit-square Q4 pattern
   int maxR = 0;
   int max_cg = 0;
@@ -484,22 +488,45 @@ struct GridRef {
 };
 GridRef grid_of(const topt_plan* pl, int g) { return GridRef{pl->nblk[g], pl->dmap[g]}; }
 
-// Multigrid (single-problem plans): the per-solve setup (Galerkin levels,
-// level diagonals, coarsest inverse) and one V-cycle z = M r.
+// Multigrid per-solve setup (Galerkin levels, level diagonals, coarsest
+// inverse) of one problem (asynchronous driver: the problems whose solve
+// starts at a chunk boundary) ...
 template <typename T>
 void mg_setup(const ProbDev& P, hipStream_t s) {
   const int L = P.mg_nlev - 1;
   for (int l = 0; l < L; ++l) {
     const int nb = (int)cdiv(int64_t(P.mg[l + 1].nx) * P.mg[l + 1].ny, TOPT_BLOCK);
     if (l == 0)
-      hipLaunchKernelGGL((k_mg_galerkin<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, P, l);
+      hipLaunchKernelGGL((k_mg_galerkin<T, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, l);
     else
-      hipLaunchKernelGGL((k_mg_galerkin<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, P, l);
+      hipLaunchKernelGGL((k_mg_galerkin<T, false, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, l);
   }
   for (int l = 1; l <= L; ++l)
-    hipLaunchKernelGGL((k_mg_diag<T>), dim3(cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK)),
-                       dim3(TOPT_BLOCK), 0, s, P, l);
-  hipLaunchKernelGGL((k_mg_coarse_setup<T>), dim3(1), dim3(TOPT_MG_SETUP_BLOCK), 0, s, P);
+    hipLaunchKernelGGL((k_mg_diag<T, false>), dim3(cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK)),
+                       dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, l);
+  hipLaunchKernelGGL((k_mg_coarse_setup<T, false>), dim3(1), dim3(TOPT_MG_SETUP_BLOCK), 0, s, nullptr, nullptr, P);
+}
+
+// ... and of every multigrid problem of the plan, each step one launch
+// through the per-level maps (lockstep solves).
+template <typename T>
+void mg_setup_all(topt_plan* pl, hipStream_t s) {
+  const ProbDev dummy{};
+  for (int l = 0; l + 1 < TOPT_MG_MAXLEV; ++l) {
+    if (pl->n_ga[l] == 0) continue;
+    if (l == 0)
+      hipLaunchKernelGGL((k_mg_galerkin<T, true, true>), dim3(pl->n_ga[l]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                         pl->mg_ga[l], dummy, l);
+    else
+      hipLaunchKernelGGL((k_mg_galerkin<T, false, true>), dim3(pl->n_ga[l]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                         pl->mg_ga[l], dummy, l);
+  }
+  for (int l = 1; l < TOPT_MG_MAXLEV; ++l)
+    if (pl->n_dg[l] > 0)
+      hipLaunchKernelGGL((k_mg_diag<T, true>), dim3(pl->n_dg[l]), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->mg_dg[l],
+                         dummy, l);
+  hipLaunchKernelGGL((k_mg_coarse_setup<T, true>), dim3(pl->n_tail), dim3(TOPT_MG_SETUP_BLOCK), 0, s, pl->dtab,
+                     pl->mg_tail, dummy);
 }
 
 // fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)
@@ -641,8 +668,7 @@ int launch_solve_setup(topt_plan* pl, int warm, int use_filtered, hipStream_t s)
                      grid_of(pl, G_ELEM).map, dummy, use_filtered);
   hipLaunchKernelGGL((k_precond<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, 0);
-  for (const ProbDev& P : pl->host)
-    if (P.precond == PC_MG) mg_setup<T>(P, s);
+  if (pl->mg) mg_setup_all<T>(pl, s);
   hipLaunchKernelGGL((k_cg_prepare<T, true>), dim3(grid_of(pl, G_NODE).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                      grid_of(pl, G_NODE).map, dummy, warm, 0);
   LAUNCH_TRY();
@@ -1168,7 +1194,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   pl->pcur = pl->dpfull;
   pl->npcur = npcg;
   if (pl->mg) {
-    std::vector<int2> dn[TOPT_MG_MAXLEV], rs[TOPT_MG_MAXLEV], tail;
+    std::vector<int2> dn[TOPT_MG_MAXLEV], rs[TOPT_MG_MAXLEV], tail, ga[TOPT_MG_MAXLEV], dg[TOPT_MG_MAXLEV];
     for (int i = 0; i < nprob; ++i) {
       const ProbDev& P = pl->host[i];
       if (P.precond != PC_MG) continue;
@@ -1183,6 +1209,13 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
       }
       tail.push_back(make_int2(i, 0));
       pl->mg_lt_max = std::max(pl->mg_lt_max, P.mg_lt);
+      const int L = P.mg_nlev - 1;
+      for (int l = 0; l < L; ++l) {
+        const int ne = (int)cdiv(int64_t(P.mg[l + 1].nx) * P.mg[l + 1].ny, TOPT_BLOCK);
+        for (int b = 0; b < ne; ++b) ga[l].push_back(make_int2(i, b));
+      }
+      for (int l = 1; l <= L; ++l)
+        for (int b = 0; b < blocks(l); ++b) dg[l].push_back(make_int2(i, b));
     }
     auto upload = [&](const std::vector<int2>& v, int2** d, int* n) {
       *n = (int)v.size();
@@ -1194,7 +1227,8 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
     };
     int rc;
     for (int l = 0; l < TOPT_MG_MAXLEV; ++l)
-      if ((rc = upload(dn[l], &pl->mg_dn[l], &pl->n_dn[l])) || (rc = upload(rs[l], &pl->mg_rs[l], &pl->n_rs[l])))
+      if ((rc = upload(dn[l], &pl->mg_dn[l], &pl->n_dn[l])) || (rc = upload(rs[l], &pl->mg_rs[l], &pl->n_rs[l])) ||
+          (rc = upload(ga[l], &pl->mg_ga[l], &pl->n_ga[l])) || (rc = upload(dg[l], &pl->mg_dg[l], &pl->n_dg[l])))
         return cleanup(rc);
     if ((rc = upload(tail, &pl->mg_tail, &pl->n_tail))) return cleanup(rc);
   }
@@ -1235,6 +1269,8 @@ int topt_plan_destroy(topt_plan* pl) {
   for (int l = 0; l < TOPT_MG_MAXLEV; ++l) {
     (void)hipFree(pl->mg_dn[l]);
     (void)hipFree(pl->mg_rs[l]);
+    (void)hipFree(pl->mg_ga[l]);
+    (void)hipFree(pl->mg_dg[l]);
   }
   (void)hipFree(pl->mg_tail);
   if (pl->cap) (void)hipStreamDestroy(pl->cap);

@@ -51,12 +51,14 @@ __host__ __device__ constexpr int mg_me(int a, int b) { return (a == 1) ? ((b ==
 // ---------------------------------------------------------------- setup
 // Coarse element matrices of level l+1 from level l (FINE: l = 0, children
 // s_e Ke with fixed DOFs masked). One thread per coarse element, FP64 sums.
-template <typename T, bool FINE>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(ProbDev single, int l) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+template <typename T, bool FINE, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, const int2* map, ProbDev single,
+                                                           int l) {
+  int lb;
+  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny;
   const int cnx = P.mg[l + 1].nx, cny = P.mg[l + 1].ny;
-  const long E = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  const long E = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (E >= (long)cnx * cny) return;
   const int I = (int)(E / cny), J = (int)(E - (long)I * cny);
   double kc[36];
@@ -127,11 +129,12 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(ProbDev single, int 
 }
 
 // 1/diag of the assembled level-l operator (l >= 1); 0 on decoupled DOFs.
-template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(ProbDev single, int l) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, const int2* map, ProbDev single, int l) {
+  int lb;
+  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
   const int nx = P.mg[l].nx, ny = P.mg[l].ny;
-  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (n >= (long)(nx + 1) * (ny + 1)) return;
   const int ix = (int)(n / (ny + 1)), iy = (int)(n - (long)ix * (ny + 1));
   const GLB T* K = gp((const T*)P.mg[l].K);
@@ -157,9 +160,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(ProbDev single, int l) {
 // (decoupled DOFs get a unit diagonal), Gauss-Jordan in place without
 // pivoting (the matrix is symmetric positive definite), store to mg_inv.
 #define TOPT_MG_SETUP_BLOCK 1024
-template <typename T>
-__global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(ProbDev single) {
-  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const ProbDev* tab, const int2* map,
+                                                                         ProbDev single) {
+  int lb;
+  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
   const int L = P.mg_nlev - 1;
   const int nx = P.mg[L].nx, ny = P.mg[L].ny, nny = ny + 1;
   const int N = P.mg_ncd;   // <= TOPT_MG_DENSE
