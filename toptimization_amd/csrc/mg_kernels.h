@@ -156,26 +156,52 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
   gp((vec2<T>*)P.mg[l].dinv)[n] = d;
 }
 
-// Dense inverse of the coarsest operator (one workgroup, in LDS): assemble
-// (decoupled DOFs get a unit diagonal), Gauss-Jordan in place without
-// pivoting (the matrix is symmetric positive definite), store to mg_inv.
-#define TOPT_MG_SETUP_BLOCK 1024
+// Inverse of the coarsest operator (one workgroup, in LDS), formed per solve.
+// The nodes are numbered along the longer side of the grid (permutation
+// pos), which makes the matrix banded with half-bandwidth w = 2 (m + 1) + 1
+// DOFs, m = nodes across the shorter side (<= 8 when N <= 128, so w <= 19).
+// Steps: assemble A (decoupled DOFs get a unit diagonal); banded Cholesky
+// A = L L^T in place (diagonal kept apart); copy the band of L to a compact
+// array; solve L L^T x = e_c for every column c (one thread per column, the
+// columns overwrite A); store A^-1 in the natural DOF order.
+// Modified pivots: a pivot a_kk becomes max(|a_kk|, tau d_k), d_k the
+// assembled diagonal. With FP32 level matrices the rounding of the Galerkin
+// chain (relative 6e-8 per stored entry) exceeds the energy of near-null
+// modes (a support held only through void elements has pivot ratios
+// a_kk / d_k near 2e-8), so such pivots come out as noise of either sign, at
+// about 1e-3 of d_k; tau = 1e-3 keeps the coarse solve symmetric positive
+// definite and bounded there, as PCG needs. FP64 levels keep those pivots
+// (tau = 1e-12); genuine pivots are far above tau in both cases.
+#define TOPT_MG_SETUP_BLOCK 128   // >= TOPT_MG_DENSE: one thread per column
+#define TOPT_MG_BAND 19           // max half-bandwidth (DOFs) for TOPT_MG_DENSE DOFs
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const ProbDev* tab, const int2* map,
                                                                          ProbDev single) {
   int lb;
   PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
   const int L = P.mg_nlev - 1;
-  const int nx = P.mg[L].nx, ny = P.mg[L].ny, nny = ny + 1;
-  const int N = P.mg_ncd;   // <= TOPT_MG_DENSE
+  const int nx = P.mg[L].nx, ny = P.mg[L].ny, nnx = nx + 1, nny = ny + 1;
+  const int N = P.mg_ncd;   // 2 nnx nny <= TOPT_MG_DENSE
+  const bool tr = nny > nnx;
+  const int m = tr ? nnx : nny;
+  const int w = min(2 * (m + 1) + 1, N - 1);
   __shared__ double A[TOPT_MG_DENSE * TOPT_MG_DENSE];
-  __shared__ double piv;
+  __shared__ double Lb[TOPT_MG_DENSE * (TOPT_MG_BAND + 1)];
+  __shared__ double Ld[TOPT_MG_DENSE];
+  __shared__ double D0[TOPT_MG_DENSE];
+  const double tau = sizeof(T) == 4 ? 1e-3 : 1e-12;
+  // permuted node p -> grid (ix, iy)
+  auto node_of = [&](int p, int& ix, int& iy) {
+    if (tr) { iy = p / nnx; ix = p - iy * nnx; }
+    else { ix = p / nny; iy = p - ix * nny; }
+  };
   const GLB T* K = gp((const T*)P.mg[L].K);
   for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
     const int i = idx / N, j = idx - (idx / N) * N;
-    const int ni = i >> 1, di = i & 1, nj = j >> 1, dj = j & 1;
-    const int ix = ni / nny, iy = ni - (ni / nny) * nny;
-    const int jx = nj / nny, jy = nj - (nj / nny) * nny;
+    const int di = i & 1, dj = j & 1;
+    int ix, iy, jx, jy;
+    node_of(i >> 1, ix, iy);
+    node_of(j >> 1, jx, jy);
     double v = 0.0;
     if (abs(ix - jx) <= 1 && abs(iy - jy) <= 1) {
       const int ex0 = max(ix, jx) - 1, ex1 = min(ix, jx);
@@ -190,29 +216,65 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
     A[idx] = v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < N; i += TOPT_MG_SETUP_BLOCK)
+  for (int i = threadIdx.x; i < N; i += TOPT_MG_SETUP_BLOCK) {
     if (!(A[i * N + i] > 0.0)) A[i * N + i] = 1.0;
+    D0[i] = A[i * N + i];
+  }
   __syncthreads();
+  // banded Cholesky: column k scaled by 1 / l_kk, then the trailing band updated
   for (int k = 0; k < N; ++k) {
-    if (threadIdx.x == 0) piv = 1.0 / A[k * N + k];
+    const int kmax = min(N - 1, k + w);
+    const double lkk = sqrt(fmax(fabs(A[k * N + k]), tau * D0[k]));
+    if (threadIdx.x == 0) Ld[k] = lkk;
+    for (int i = k + 1 + threadIdx.x; i <= kmax; i += TOPT_MG_SETUP_BLOCK) A[i * N + k] /= lkk;
     __syncthreads();
-    const double p = piv;
-    for (int j = threadIdx.x; j < N; j += TOPT_MG_SETUP_BLOCK)
-      if (j != k) A[k * N + j] *= p;
-    __syncthreads();
-    // entries (i, j), i != k: a_ij -= a_ik a_kj (j != k), a_ik = -a_ik p
-    for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
-      const int i = idx / N, j = idx - (idx / N) * N;
-      if (i == k || j == k) continue;
-      A[idx] = fma(-A[i * N + k], A[k * N + j], A[idx]);
+    const int nr = kmax - k;   // rows below the diagonal in the band
+    for (int t = threadIdx.x; t < nr * (nr + 1) / 2; t += TOPT_MG_SETUP_BLOCK) {
+      // t -> (i, j), k < j <= i <= kmax
+      int ri = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+      while ((ri + 1) * (ri + 2) / 2 <= t) ++ri;
+      while (ri * (ri + 1) / 2 > t) --ri;
+      const int rj = t - ri * (ri + 1) / 2;
+      const int i = k + 1 + ri, j = k + 1 + rj;
+      A[i * N + j] = fma(-A[i * N + k], A[j * N + k], A[i * N + j]);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += TOPT_MG_SETUP_BLOCK)
-      if (i != k) A[i * N + k] *= -p;
-    if (threadIdx.x == 0) A[k * N + k] = p;
-    __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) gp(P.mg_inv)[idx] = A[idx];
+  // compact band of L: Lb[i][d] = L(i, i - d), d = 1..w
+  for (int idx = threadIdx.x; idx < N * w; idx += TOPT_MG_SETUP_BLOCK) {
+    const int i = idx / w, d = idx - (idx / w) * w + 1;
+    Lb[i * (TOPT_MG_BAND + 1) + d] = i - d >= 0 ? A[i * N + (i - d)] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK)
+    A[idx] = (idx / N) == (idx - (idx / N) * N) ? 1.0 : 0.0;
+  __syncthreads();
+  // column c of the inverse: forward L y = e_c, then backward L^T x = y
+  const int c = threadIdx.x;
+  if (c < N) {
+    for (int j = c; j < N; ++j) {
+      const double yj = A[j * N + c] / Ld[j];
+      A[j * N + c] = yj;
+      for (int i = j + 1; i <= min(N - 1, j + w); ++i)
+        A[i * N + c] = fma(-Lb[i * (TOPT_MG_BAND + 1) + (i - j)], yj, A[i * N + c]);
+    }
+    for (int j = N - 1; j >= 0; --j) {
+      const double xj = A[j * N + c] / Ld[j];
+      A[j * N + c] = xj;
+      for (int i = max(0, j - w); i < j; ++i)
+        A[i * N + c] = fma(-Lb[j * (TOPT_MG_BAND + 1) + (j - i)], xj, A[i * N + c]);
+    }
+  }
+  __syncthreads();
+  // back to the natural DOF order (node ix * nny + iy)
+  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
+    const int i = idx / N, j = idx - (idx / N) * N;
+    int ix, iy, jx, jy;
+    node_of(i >> 1, ix, iy);
+    node_of(j >> 1, jx, jy);
+    const int ni = 2 * (ix * nny + iy) + (i & 1), nj = 2 * (jx * nny + jy) + (j & 1);
+    gp(P.mg_inv)[(long)ni * N + nj] = A[idx];
+  }
 }
 
 // ---------------------------------------------------------------- V-cycle
