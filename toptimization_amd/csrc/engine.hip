@@ -535,18 +535,17 @@ void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
   const ProbDev* tab = pl->dtab;
   const int nt = pl->npcur;        // live problems' tiles (the PCG map)
   const int2* tmap = pl->pcur;
+  // down sweeps fused with the restrictions: b_{l+1} from b_l (b_0 = r)
   if (pl->kstruct)
-    hipLaunchKernelGGL((k_mg_down0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
+    hipLaunchKernelGGL((k_mg_down_restrict0<T, true>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[0],
+                       par);
   else
-    hipLaunchKernelGGL((k_mg_down0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
-  if (pl->n_rs[0] > 0)
-    hipLaunchKernelGGL((k_mg_restrict<T>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[0], 0, par);
-  for (int l = 1; l < pl->mg_lt_max; ++l) {
-    if (pl->n_dn[l] > 0)
-      hipLaunchKernelGGL((k_mg_down<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_dn[l], l, par);
+    hipLaunchKernelGGL((k_mg_down_restrict0<T, false>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab,
+                       pl->mg_rs[0], par);
+  for (int l = 1; l < pl->mg_lt_max; ++l)
     if (pl->n_rs[l] > 0)
-      hipLaunchKernelGGL((k_mg_restrict<T>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[l], l, par);
-  }
+      hipLaunchKernelGGL((k_mg_down_restrict<T>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[l], l,
+                         par);
   hipLaunchKernelGGL((k_mg_tail<T>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, tab, pl->mg_tail, par);
   for (int l = pl->mg_lt_max - 1; l >= 1; --l)
     if (pl->n_dn[l] > 0)
@@ -1199,13 +1198,12 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
       const ProbDev& P = pl->host[i];
       if (P.precond != PC_MG) continue;
       auto blocks = [&](int l) { return (int)cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK); };
-      // down / up sweeps at levels 1 .. lt-1; restrictions l -> l+1 for
-      // l < lt-1 (the tail restricts lt-1 -> lt itself)
+      // fused down sweeps + restrictions l -> l+1 for l < lt (tiles of
+      // level l+1 nodes); up sweeps at levels 1 .. lt-1
       for (int l = 0; l < P.mg_lt; ++l) {
         if (l >= 1)
-          for (int b = 0; b < blocks(l); ++b) dn[l].push_back(make_int2(i, b));
-        if (l + 1 < P.mg_lt)
-          for (int b = 0; b < blocks(l + 1); ++b) rs[l].push_back(make_int2(i, b));
+          for (int b = 0; b < mg_up_tiles(P.mg[l].nx, P.mg[l].ny); ++b) dn[l].push_back(make_int2(i, b));
+        for (int b = 0; b < mg_dr_tiles(P.mg[l + 1].nx, P.mg[l + 1].ny); ++b) rs[l].push_back(make_int2(i, b));
       }
       tail.push_back(make_int2(i, 0));
       pl->mg_lt_max = std::max(pl->mg_lt_max, P.mg_lt);

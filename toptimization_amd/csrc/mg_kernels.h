@@ -332,16 +332,6 @@ __device__ __forceinline__ void mg_restrict_p(PD& P, int l, long N, VR r, VB bou
   bout[N] = o;
 }
 
-template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_restrict(const ProbDev* tab, const int2* map, int l, int par) {
-  int lb;
-  PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
-  if (mg_stopped(P, par)) return;
-  const long N = (long)lb * TOPT_BLOCK + threadIdx.x;
-  if (N >= (long)(P.mg[l + 1].nx + 1) * (P.mg[l + 1].ny + 1)) return;
-  mg_restrict_p<T>(P, l, N, gp((const vec2<T>*)P.mg[l].r), gp((vec2<T>*)P.mg[l + 1].b));
-}
-
 // Interpolated coarse correction (P x_c) at fine node (i, j); xc holds the
 // coarse level's iterate, cny its element rows.
 template <typename T, typename VC>
@@ -374,23 +364,29 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, int ix, int iy, XF xv
   const int nx = P.mg[l].nx, ny = P.mg[l].ny;
   const GLB T* K = gp((const T*)P.mg[l].K);
   double ax = 0.0, ay = 0.0;
+  // branch-free: elements outside the grid are read at a clamped address and
+  // weighted 0, so all the loads can be in flight together
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int ex = ix - 1 + a, ey = iy - 1 + b;
-      if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+      const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
+      const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
       const int me = mg_me(a, b);
-      const GLB T* k = K + ((long)ex * ny + ey) * 36;
+      const GLB T* k = K + ((long)cx * ny + cy) * 36;
+      double tx = 0.0, ty = 0.0;
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         // element corner kk is neighbour (a - 1 + lx, b - 1 + ly) of the node
         const d2v xv_ = xv(a - 1 + mg_lx(kk), b - 1 + mg_ly(kk));
-        ax = fma((double)k[sym8(2 * me, 2 * kk)], xv_.x, ax);
-        ax = fma((double)k[sym8(2 * me, 2 * kk + 1)], xv_.y, ax);
-        ay = fma((double)k[sym8(2 * me + 1, 2 * kk)], xv_.x, ay);
-        ay = fma((double)k[sym8(2 * me + 1, 2 * kk + 1)], xv_.y, ay);
+        tx = fma((double)k[sym8(2 * me, 2 * kk)], xv_.x, tx);
+        tx = fma((double)k[sym8(2 * me, 2 * kk + 1)], xv_.y, tx);
+        ty = fma((double)k[sym8(2 * me + 1, 2 * kk)], xv_.x, ty);
+        ty = fma((double)k[sym8(2 * me + 1, 2 * kk + 1)], xv_.y, ty);
       }
+      ax += ok ? tx : 0.0;
+      ay += ok ? ty : 0.0;
     }
   return d2v{ax, ay};
 }
@@ -422,23 +418,195 @@ __device__ __forceinline__ void mg_down_p(PD& P, int l, long n, VB b, VX x, VX r
   r[n] = ro;
 }
 
+// ------------------------------------------------- down sweep + restriction
+// The pre-smoothing step starts from zero, so level l's smoothed iterate
+// x_l = omega D^-1 b_l is a pointwise function of b_l, and one kernel can
+// take b_l to b_{l+1} = P^T (b_l - K_l x_l) reading only data written before
+// its launch. A workgroup covers a tile of DR_CX x DR_CY coarse nodes: it
+// forms x_l on the fine patch around them in LDS, r_l at the
+// (2 DR_CX + 1) x (2 DR_CY + 1) fine nodes their restrictions gather (one
+// per thread; the element rows each thread needs are fetched before the
+// barrier), then the restrictions from LDS (level 0: fixed DOFs masked).
+// Fine nodes on a tile's border are formed by both neighbouring tiles (255
+// residuals per tile against its 4 x 56 = 224 fine nodes). x_l is not
+// stored: the up sweep forms it again from b_l.
+struct DrTile {
+  static constexpr int CX = 7, CY = 8;                     // coarse nodes
+  static constexpr int RX = 2 * CX + 1, RY = 2 * CY + 1;   // fine residuals (15 x 17)
+  static constexpr int XX = RX + 2, XY = RY + 2;           // fine iterates (17 x 19)
+};
+static_assert(DrTile::RX * DrTile::RY <= TOPT_BLOCK, "one fine residual per thread");
+// Workgroups of the fused sweep for a coarse level of cnx x cny elements.
+__host__ __device__ inline int mg_dr_tiles(int cnx, int cny) {
+  return ((cnx + DrTile::CX) / DrTile::CX) * ((cny + DrTile::CY) / DrTile::CY);
+}
+
+// Element rows of level l >= 1 (stored Galerkin matrices): the two rows of
+// local node me, fetched from a clamped address; factor 0 outside the grid.
 template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down(const ProbDev* tab, const int2* map, int l, int par) {
+struct MgElems {
+  const GLB T* K;
+  int nx, ny;
+  struct R {
+    T k[16];
+    double f;
+  };
+  __device__ __forceinline__ R fetch(int ex, int ey, int me) const {
+    const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
+    const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
+    const GLB T* k = K + ((long)cx * ny + cy) * 36;
+    R r;
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) r.k[d * 8 + c] = k[sym8(2 * me + d, c)];
+    r.f = ok ? 1.0 : 0.0;
+    return r;
+  }
+  __device__ __forceinline__ double at(const R& r, int, int d, int c) const { return (double)r.k[d * 8 + c]; }
+};
+// Element rows of level 0: s_e Ke (KST: Ke from its six magnitudes); the
+// factor is the element scale, 0 outside the grid.
+template <typename T, bool KST>
+struct MgElems0 {
+  PD& P;
+  const GLB T* s;
+  struct R {
+    double f;
+  };
+  __device__ __forceinline__ R fetch(int ex, int ey, int) const {
+    const bool ok = ex >= 0 && ex < P.nelx && ey >= 0 && ey < P.nely;
+    const int cx = ex < 0 ? 0 : (ex >= P.nelx ? P.nelx - 1 : ex), cy = ey < 0 ? 0 : (ey >= P.nely ? P.nely - 1 : ey);
+    const double v = (double)s[(long)cx * P.nely + cy];
+    return R{ok ? v : 0.0};
+  }
+  __device__ __forceinline__ double at(const R&, int me, int d, int c) const {
+    const int i = 2 * me + d;
+    if constexpr (KST) return KS[i][c] < 0 ? -P.kc[-KS[i][c]] : P.kc[KS[i][c]];
+    else return P.Ke64[i * 8 + c];
+  }
+};
+
+// One tile (workgroup lb) of the fused sweep from a fine level of fnx x fny
+// elements (b, 1/diag dv) to the coarse level of cnx x cny elements (bout).
+template <typename T, bool MASK, typename EK, typename VB, typename VD, typename VO>
+__device__ __forceinline__ void mg_down_restrict_tile(PD& P, int lb, int fnx, int fny, int cnx, int cny, VB b, VD dv,
+                                                      const EK& ek, VO bout) {
+  using D = DrTile;
+  __shared__ d2v xs[D::XX * D::XY];
+  __shared__ d2v rs[D::RX * D::RY];
+  const int nty = (cny + D::CY) / D::CY;
+  const int tx = lb / nty, ty = lb - tx * nty;
+  const int I0 = tx * D::CX, J0 = ty * D::CY;
+  const int fnny = fny + 1;
+  const int t = (int)threadIdx.x;
+  // this thread's fine residual node (residual tile position (qx, qy))
+  const bool hasr = t < D::RX * D::RY;
+  const int qx = hasr ? t / D::RY : 0, qy = hasr ? t - (t / D::RY) * D::RY : 0;
+  const int i = 2 * I0 - 1 + qx, j = 2 * J0 - 1 + qy;
+  const bool inr = hasr && i >= 0 && i <= fnx && j >= 0 && j <= fny;
+  const int ic = i < 0 ? 0 : (i > fnx ? fnx : i), jc = j < 0 ? 0 : (j > fny ? fny : j);
+  const long mn = (long)ic * fnny + jc;
+  const vec2<T> bn = b[mn];
+  typename EK::R er[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) er[a][c] = ek.fetch(i - 1 + a, j - 1 + c, mg_me(a, c));
+  unsigned fb = 0u;
+  if constexpr (MASK) fb = fixbits(P.fix, mn);
+  // x on the fine patch (clamped into the grid: a clamped value meets only
+  // elements outside it, whose factor is 0)
+  for (int u = t; u < D::XX * D::XY; u += TOPT_BLOCK) {
+    const int px = u / D::XY, py = u - (u / D::XY) * D::XY;
+    int jx = 2 * I0 - 2 + px, jy = 2 * J0 - 2 + py;
+    jx = jx < 0 ? 0 : (jx > fnx ? fnx : jx);
+    jy = jy < 0 ? 0 : (jy > fny ? fny : jy);
+    const long m = (long)jx * fnny + jy;
+    const vec2<T> bb = b[m], dd = dv[m];
+    xs[u] = d2v{TOPT_MG_OMEGA * (double)dd.x * (double)bb.x, TOPT_MG_OMEGA * (double)dd.y * (double)bb.y};
+  }
+  __syncthreads();
+  if (hasr) {
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int me = mg_me(a, c);
+        double tx_ = 0.0, ty_ = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          // corner kk of element (i - 1 + a, j - 1 + c): patch position below
+          const d2v v = xs[(qx + a + mg_lx(kk)) * D::XY + qy + c + mg_ly(kk)];
+          tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk), v.x, tx_);
+          tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk + 1), v.y, tx_);
+          ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk), v.x, ty_);
+          ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk + 1), v.y, ty_);
+        }
+        ax = fma(er[a][c].f, tx_, ax);
+        ay = fma(er[a][c].f, ty_, ay);
+      }
+    d2v r = d2v{(double)bn.x - ax, (double)bn.y - ay};
+    if (!inr || (fb & 1u)) r.x = 0.0;
+    if (!inr || (fb & 2u)) r.y = 0.0;
+    rs[t] = r;
+  }
+  __syncthreads();
+  if (t < D::CX * D::CY) {
+    const int cx = t / D::CY, cy = t - (t / D::CY) * D::CY;
+    const int I = I0 + cx, J = J0 + cy;
+    if (I <= cnx && J <= cny) {
+      double bx = 0.0, by = 0.0;
+#pragma unroll
+      for (int di = 0; di < 3; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          const double w = (di == 1 ? 1.0 : 0.5) * (dj == 1 ? 1.0 : 0.5);
+          const d2v r = rs[(2 * cx + di) * D::RY + 2 * cy + dj];
+          bx = fma(w, r.x, bx);
+          by = fma(w, r.y, by);
+        }
+      vec2<T> o;
+      o.x = (T)bx;
+      o.y = (T)by;
+      bout[(long)I * (cny + 1) + J] = o;
+    }
+  }
+}
+
+// Levels 1 .. lt-1: b_{l+1} from b_l.
+template <typename T>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const ProbDev* tab, const int2* map, int l, int par) {
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   if (mg_stopped(P, par)) return;
-  const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
-  if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
-  mg_down_p<T>(P, l, n, gp((const vec2<T>*)P.mg[l].b), gp((vec2<T>*)P.mg[l].x), gp((vec2<T>*)P.mg[l].r));
+  const MgElems<T> ek{gp((const T*)P.mg[l].K), P.mg[l].nx, P.mg[l].ny};
+  mg_down_restrict_tile<T, false>(P, lb, P.mg[l].nx, P.mg[l].ny, P.mg[l + 1].nx, P.mg[l + 1].ny,
+                                  gp((const vec2<T>*)P.mg[l].b), gp((const vec2<T>*)P.mg[l].dinv), ek,
+                                  gp((vec2<T>*)P.mg[l + 1].b));
 }
 
-// Up sweep at node n of level l >= 1 (not the coarsest): x' = x + P x_{l+1},
-// out = x' + omega D^-1 (b - K x').
+// Level 0: b_1 from the PCG residual r (x0 = omega D^-1 r; K(s) has fixed
+// rows identity, so r1 = 0 on fixed DOFs, and r, hence x0, is zero there).
+template <typename T, bool KST>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const ProbDev* tab, const int2* map, int par) {
+  int lb;
+  PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
+  if (mg_stopped(P, par)) return;
+  const MgElems0<T, KST> ek{P, gp((const T*)P.s)};
+  mg_down_restrict_tile<T, true>(P, lb, P.nelx, P.nely, P.mg[1].nx, P.mg[1].ny, gp((const vec2<T>*)P.r),
+                                 gp((const vec2<T>*)P.pre), ek, gp((vec2<T>*)P.mg[1].b));
+}
+
+// Up sweep at node n of level l >= 1 (not the coarsest) inside the tail:
+// x' = x + P x_{l+1}, out = x' + omega D^-1 (b - K x').
 template <typename T, typename VX, typename VC, typename VB, typename VO>
 __device__ __forceinline__ void mg_up_p(PD& P, int l, long n, VX x, VC xc, VB b, VO out) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
   const int cny = P.mg[l + 1].ny;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
+  const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
   d2v xs[3][3];
 #pragma unroll
   for (int dx = 0; dx < 3; ++dx)
@@ -447,38 +615,108 @@ __device__ __forceinline__ void mg_up_p(PD& P, int l, long n, VX x, VC xc, VB b,
       int jx = ix + dx - 1, jy = iy + dy - 1;
       jx = jx < 0 ? 0 : (jx > nx ? nx : jx);
       jy = jy < 0 ? 0 : (jy > ny ? ny : jy);
-      const vec2<T> xv = x[(long)jx * nny + jy];
-      xs[dx][dy] = d2v{(double)xv.x, (double)xv.y} + mg_prolong<T>(xc, cny, jx, jy);
+      const vec2<T> v = x[(long)jx * nny + jy];
+      xs[dx][dy] = d2v{(double)v.x, (double)v.y} + mg_prolong<T>(xc, cny, jx, jy);
     }
   const d2v ax = mg_apply_rows<T>(P, l, ix, iy, [&](int dx, int dy) { return xs[dx + 1][dy + 1]; });
   const vec2<T> bn = b[n];
-  const vec2<T> dd = gp((const vec2<T>*)P.mg[l].dinv)[n];
+  const vec2<T> dd = dv[n];
   vec2<T> o;
   o.x = (T)(xs[1][1].x + TOPT_MG_OMEGA * (double)dd.x * ((double)bn.x - ax.x));
   o.y = (T)(xs[1][1].y + TOPT_MG_OMEGA * (double)dd.y * ((double)bn.y - ax.y));
   out[n] = o;
 }
 
-// The corrected iterate goes to r (free after the restriction).
+// Up sweep at levels 1 .. lt-1 on tiles of UpTile::TX x UpTile::TY nodes
+// (one per thread): x' = omega D^-1 b + P x_{l+1} (the pre-smoothed iterate
+// formed again from b) on the tile's patch in LDS, the element rows of each
+// thread's node fetched before the barrier, then out = x' + omega D^-1
+// (b - K x'). The corrected iterate goes to r (the fused down sweep stores
+// no residual).
+struct UpTile {
+  static constexpr int TX = 8, TY = 32;              // nodes (TY along the memory order)
+  static constexpr int PX = TX + 2, PY = TY + 2;     // patch with its ring
+};
+static_assert(UpTile::TX * UpTile::TY == TOPT_BLOCK, "one node per thread");
+// Workgroups of the up sweep for a level of nx x ny elements.
+__host__ __device__ inline int mg_up_tiles(int nx, int ny) {
+  return ((nx + UpTile::TX) / UpTile::TX) * ((ny + UpTile::TY) / UpTile::TY);
+}
 template <typename T>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const int2* map, int l, int par) {
+  using U = UpTile;
+  __shared__ d2v xs[U::PX * U::PY];
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   if (mg_stopped(P, par)) return;
-  const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
-  if (n >= (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1)) return;
-  mg_up_p<T>(P, l, n, gp((const vec2<T>*)P.mg[l].x), gp((const vec2<T>*)mg_xup(P, l + 1)),
-             gp((const vec2<T>*)P.mg[l].b), gp((vec2<T>*)P.mg[l].r));
+  const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
+  const int cny = P.mg[l + 1].ny;
+  const int nty = (ny + U::TY) / U::TY;
+  const int tx = lb / nty, ty = lb - tx * nty;
+  const int x0 = tx * U::TX, y0 = ty * U::TY;
+  const int t = (int)threadIdx.x;
+  const int ox = t / U::TY, oy = t - (t / U::TY) * U::TY;
+  const int ix = x0 + ox, iy = y0 + oy;
+  const bool act = ix <= nx && iy <= ny;
+  const long n = (long)(ix <= nx ? ix : nx) * nny + (iy <= ny ? iy : ny);
+  const GLB vec2<T>* b = gp((const vec2<T>*)P.mg[l].b);
+  const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
+  const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, l + 1));
+  const vec2<T> bn = b[n], dn = dv[n];
+  const MgElems<T> ek{gp((const T*)P.mg[l].K), nx, ny};
+  typename MgElems<T>::R er[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) er[a][c] = ek.fetch(ix - 1 + a, iy - 1 + c, mg_me(a, c));
+  // x' on the patch (clamped into the grid: a clamped value meets only
+  // elements outside it, whose factor is 0)
+  for (int u = t; u < U::PX * U::PY; u += TOPT_BLOCK) {
+    const int px = u / U::PY, py = u - (u / U::PY) * U::PY;
+    int jx = x0 - 1 + px, jy = y0 - 1 + py;
+    jx = jx < 0 ? 0 : (jx > nx ? nx : jx);
+    jy = jy < 0 ? 0 : (jy > ny ? ny : jy);
+    const long m = (long)jx * nny + jy;
+    const vec2<T> bb = b[m], dd = dv[m];
+    xs[u] = d2v{TOPT_MG_OMEGA * (double)dd.x * (double)bb.x, TOPT_MG_OMEGA * (double)dd.y * (double)bb.y} +
+            mg_prolong<T>(xc, cny, jx, jy);
+  }
+  __syncthreads();
+  if (!act) return;
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int me = mg_me(a, c);
+      double tx_ = 0.0, ty_ = 0.0;
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        // corner kk of element (ix - 1 + a, iy - 1 + c): patch position below
+        const d2v v = xs[(ox + a + mg_lx(kk)) * U::PY + oy + c + mg_ly(kk)];
+        tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk), v.x, tx_);
+        tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk + 1), v.y, tx_);
+        ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk), v.x, ty_);
+        ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk + 1), v.y, ty_);
+      }
+      ax = fma(er[a][c].f, tx_, ax);
+      ay = fma(er[a][c].f, ty_, ay);
+    }
+  const d2v xo = xs[(ox + 1) * U::PY + oy + 1];
+  vec2<T> o;
+  o.x = (T)(xo.x + TOPT_MG_OMEGA * (double)dn.x * ((double)bn.x - ax));
+  o.y = (T)(xo.y + TOPT_MG_OMEGA * (double)dn.y * ((double)bn.y - ay));
+  gp((vec2<T>*)P.mg[l].r)[n] = o;
 }
 
 // Levels lt..L (the small ones) in one workgroup: down sweeps and
 // restrictions to the coarsest level, its dense solve, up sweeps back to lt.
 // The levels' x, b and r live in LDS, so a phase hands its results to the
 // next through LDS and a workgroup barrier; element matrices, 1/diag and
-// the coarse inverse are read from global memory. The kernel starts with
-// the restriction of level lt-1's residual (written by the sweep launched
-// before it), and level lt's corrected iterate goes back to global memory
-// (mg_xup) for the up sweep after it.
+// the coarse inverse are read from global memory. The kernel starts from
+// b_lt (written by the fused sweep launched before it), and level lt's
+// corrected iterate goes back to global memory (mg_xup) for the up sweep
+// after it.
 // One workgroup per problem.
 #define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
 #define TOPT_MG_TAIL_BLOCK 1024  // each phase is one round trip)
@@ -486,6 +724,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
 template <typename T>
 __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
   __shared__ vec2<T> sx[TOPT_MG_TAIL_SUM], sb[TOPT_MG_TAIL_SUM], sr[TOPT_MG_TAIL_SUM];
+  __shared__ double dpart[TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE][TOPT_MG_DENSE];
+  static_assert(TOPT_MG_TAIL_BLOCK % TOPT_MG_DENSE == 0, "tail block: whole groups of coarse rows");
   int lb;
   PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
@@ -500,9 +740,9 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
       o += (int)nodes(l);
     }
   }
-  // b of level lt: restriction of level lt-1's residual
+  // b of level lt, formed by the fused sweep of level lt-1
   for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK)
-    mg_restrict_p<T>(P, lt - 1, n, gp((const vec2<T>*)P.mg[lt - 1].r), (vec2<T>*)(sb + off[lt]));
+    sb[off[lt] + n] = gp((const vec2<T>*)P.mg[lt].b)[n];
   __syncthreads();
   for (int l = lt; l < L; ++l) {
     for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
@@ -513,16 +753,31 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     __syncthreads();
   }
   {
-    // x = A^-1 b; A is symmetric: read column `row` (consecutive threads,
-    // consecutive addresses)
+    // x = A^-1 b; A is symmetric, so row `row` is read as column `row`
+    // (consecutive threads, consecutive addresses). Thread group g (of
+    // TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE) takes columns j = g mod groups; its
+    // loads are independent and issued together, then the groups' partial
+    // sums meet in LDS.
+    constexpr int G = TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE;
     const int N = P.mg_ncd;
     const T* b = (const T*)(sb + off[L]);
-    T* x = (T*)(sx + off[L]);
-    for (int row = threadIdx.x; row < N; row += TOPT_MG_TAIL_BLOCK) {
+    const int row = threadIdx.x % TOPT_MG_DENSE, g = threadIdx.x / TOPT_MG_DENSE;
+    if (row < N) {
       const GLB double* A = gp(P.mg_inv) + row;
       double acc = 0.0;
-      for (int j = 0; j < N; ++j) acc = fma(A[(long)j * N], (double)b[j], acc);
-      x[row] = (T)acc;
+#pragma unroll
+      for (int k = 0; k < TOPT_MG_DENSE / G; ++k) {
+        const int j = g + k * G;
+        if (j < N) acc = fma(A[(long)j * N], (double)b[j], acc);
+      }
+      dpart[g][row] = acc;
+    }
+    __syncthreads();
+    if (g == 0 && row < N) {
+      double acc = dpart[0][row];
+#pragma unroll
+      for (int h = 1; h < G; ++h) acc += dpart[h][row];
+      ((T*)(sx + off[L]))[row] = (T)acc;
     }
   }
   __syncthreads();
@@ -540,55 +795,26 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   }
 }
 
-// Level 0 down sweep on the PCG tiles: x0 = omega D^-1 r, r1 = r - K(s) x0
-// (fixed rows identity, so r1 = 0 there).
-template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down0(const ProbDev* tab, const int2* map, int par) {
-  using Tl = Tile<T>;
-  constexpr int NPT = Tl::NPT;
-  int lb;
-  PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
-  if (mg_stopped(P, par)) return;
-  __shared__ d2v hal[Tl::HN];
-  __shared__ double esc[Tl::EN];
-  const TileOrg o = tile_org<T>(P, lb);
-  const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
-  const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
-  TileShare<T> S;
-  tile_share<T>(P, o, S);
-  vec2<T> orr[NPT], od[NPT];
-#pragma unroll
-  for (int j = 0; j < NPT; ++j) {
-    orr[j] = r[S.ids[j].n];
-    od[j] = dv[S.ids[j].n];
-  }
-  const vec2<T> rr = r[S.rn], rd = dv[S.rn];
-  d2v hv[NPT];
-#pragma unroll
-  for (int j = 0; j < NPT; ++j)
-    hv[j] = d2v{TOPT_MG_OMEGA * (double)od[j].x * (double)orr[j].x, TOPT_MG_OMEGA * (double)od[j].y * (double)orr[j].y};
-  tile_store<T>(S, hv, d2v{TOPT_MG_OMEGA * (double)rd.x * (double)rr.x, TOPT_MG_OMEGA * (double)rd.y * (double)rr.y},
-                hal, esc);
-  __syncthreads();
-  double ax[NPT], ay[NPT];
-  tile_apply<T, KST>(P, hal, esc, ax, ay);
-  GLB vec2<T>* x0 = gp((vec2<T>*)P.mg[0].x);
-  GLB vec2<T>* r1 = gp((vec2<T>*)P.mg[0].r);
-#pragma unroll
-  for (int j = 0; j < NPT; ++j) {
-    if (!S.ids[j].act) continue;
-    vec2<T> xo, ro;
-    xo.x = (T)hv[j].x;
-    xo.y = (T)hv[j].y;
-    ro.x = (S.fb[j] & 1u) ? T(0) : (T)((double)orr[j].x - ax[j]);
-    ro.y = (S.fb[j] & 2u) ? T(0) : (T)((double)orr[j].y - ay[j]);
-    x0[S.ids[j].n] = xo;
-    r1[S.ids[j].n] = ro;
-  }
+// Level 0's pre-smoothed iterate at a node from r and 1/diag.
+template <typename T>
+__device__ __forceinline__ d2v mg_x0(vec2<T> rr, vec2<T> dd) {
+  return d2v{TOPT_MG_OMEGA * (double)dd.x * (double)rr.x, TOPT_MG_OMEGA * (double)dd.y * (double)rr.y};
+}
+// x' = x0 + (masked) P x_1 at the tile's ring node m.
+template <typename T>
+__device__ __forceinline__ d2v mg_ring_x(PD& P, long m, const GLB vec2<T>* r, const GLB vec2<T>* dv,
+                                         const GLB vec2<T>* xc, int cny) {
+  const int nny = P.nely + 1;
+  const int jx = (int)(m / nny), jy = (int)(m - (long)jx * nny);
+  d2v c = mg_prolong<T>(xc, cny, jx, jy);
+  const unsigned fb = fixbits(P.fix, m);
+  if (fb & 1u) c.x = 0.0;
+  if (fb & 2u) c.y = 0.0;
+  return mg_x0<T>(r[m], dv[m]) + c;
 }
 
 // Level 0 up sweep: x' = x0 + (masked) P x_1, z = x' + omega D^-1 (r - K x'),
-// z = 0 on fixed DOFs.
+// z = 0 on fixed DOFs; x0 = omega D^-1 r, formed again from r.
 template <typename T, bool KST>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const ProbDev* tab, const int2* map, int par) {
   using Tl = Tile<T>;
@@ -599,48 +825,36 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const ProbDev* tab, const
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   const TileOrg o = tile_org<T>(P, lb);
-  const GLB vec2<T>* x0 = gp((const vec2<T>*)P.mg[0].x);
   const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, 1));
+  const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
+  const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
   const int cny = P.mg[1].ny;
   TileShare<T> S;
   tile_share<T>(P, o, S);
   d2v hv[NPT];
+  vec2<T> orr[NPT], od[NPT];
 #pragma unroll
   for (int j = 0; j < NPT; ++j) {
-    const vec2<T> xv = x0[S.ids[j].n];
+    orr[j] = r[S.ids[j].n];
+    od[j] = dv[S.ids[j].n];
     d2v c = mg_prolong<T>(xc, cny, S.ids[j].ix, S.ids[j].iy);
     if (S.fb[j] & 1u) c.x = 0.0;
     if (S.fb[j] & 2u) c.y = 0.0;
-    hv[j] = d2v{(double)xv.x, (double)xv.y} + c;
+    hv[j] = mg_x0<T>(orr[j], od[j]) + c;
   }
-  d2v rv;
-  {
-    const long m = S.rn;
-    const int nny = P.nely + 1;
-    const int jx = (int)(m / nny), jy = (int)(m - (long)jx * nny);
-    const vec2<T> xv = x0[m];
-    d2v c = mg_prolong<T>(xc, cny, jx, jy);
-    const unsigned fb = fixbits(P.fix, m);
-    if (fb & 1u) c.x = 0.0;
-    if (fb & 2u) c.y = 0.0;
-    rv = d2v{(double)xv.x, (double)xv.y} + c;
-  }
+  const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   tile_store<T>(S, hv, rv, hal, esc);
   __syncthreads();
   double ax[NPT], ay[NPT];
   tile_apply<T, KST>(P, hal, esc, ax, ay);
-  const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
-  const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
   GLB vec2<T>* z = gp((vec2<T>*)P.z);
 #pragma unroll
   for (int j = 0; j < NPT; ++j) {
     if (!S.ids[j].act) continue;
-    const long n = S.ids[j].n;
-    const vec2<T> rr = r[n], dd = dv[n];
     vec2<T> zo;
-    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + TOPT_MG_OMEGA * (double)dd.x * ((double)rr.x - ax[j]));
-    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + TOPT_MG_OMEGA * (double)dd.y * ((double)rr.y - ay[j]));
-    z[n] = zo;
+    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + TOPT_MG_OMEGA * (double)od[j].x * ((double)orr[j].x - ax[j]));
+    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + TOPT_MG_OMEGA * (double)od[j].y * ((double)orr[j].y - ay[j]));
+    z[S.ids[j].n] = zo;
   }
 }
 
@@ -652,12 +866,11 @@ namespace topt {
 // Plans whose problems all use multigrid run the level-0 up sweep and slot C
 // as one kernel (one launch less per CG iteration; arithmetic of the
 // separate kernels). Slot B is not fused with the down sweep: that sweep
-// needs r at the tile's halo ring, which neighbouring workgroups of the
-// same launch update, so it would read old or new values depending on
-// workgroup timing.
+// needs r around its nodes, which neighbouring workgroups of the same launch
+// update, so it would read old or new values depending on workgroup timing.
 
 // Level-0 up sweep + slot C: z = x0 + P x_1 + omega D^-1 (r - K x'), z = 0
-// on fixed DOFs, partial r.z.
+// on fixed DOFs, partial r.z (x0 = omega D^-1 r, formed again from r).
 template <typename T, bool KST>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, const int2* pmap, int q) {
   using Tl = Tile<T>;
@@ -668,7 +881,6 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   int lb;
   PD& P = mg_entry(tab, pmap, xcd_remap(blockIdx.x, gridDim.x), lb);
   const TileOrg o = tile_org<T>(P, lb);
-  const GLB vec2<T>* x0 = gp((const vec2<T>*)P.mg[0].x);
   const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, 1));
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
   const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
@@ -682,26 +894,14 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   vec2<T> orr[NPT], od[NPT];
 #pragma unroll
   for (int j = 0; j < NPT; ++j) {
-    const vec2<T> xv = x0[S.ids[j].n];
+    orr[j] = r[S.ids[j].n];
+    od[j] = dv[S.ids[j].n];
     d2v c = mg_prolong<T>(xc, cny, S.ids[j].ix, S.ids[j].iy);
     if (S.fb[j] & 1u) c.x = 0.0;
     if (S.fb[j] & 2u) c.y = 0.0;
-    hv[j] = d2v{(double)xv.x, (double)xv.y} + c;
-    orr[j] = r[S.ids[j].n];
-    od[j] = dv[S.ids[j].n];
+    hv[j] = mg_x0<T>(orr[j], od[j]) + c;
   }
-  d2v rv;
-  {
-    const long m = S.rn;
-    const int nny = P.nely + 1;
-    const int jx = (int)(m / nny), jy = (int)(m - (long)jx * nny);
-    const vec2<T> xv = x0[m];
-    d2v c = mg_prolong<T>(xc, cny, jx, jy);
-    const unsigned fb = fixbits(P.fix, m);
-    if (fb & 1u) c.x = 0.0;
-    if (fb & 2u) c.y = 0.0;
-    rv = d2v{(double)xv.x, (double)xv.y} + c;
-  }
+  const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   bool upd;
   const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
   if (st.status != ST_RUN) return;
