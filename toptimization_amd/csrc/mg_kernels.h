@@ -298,12 +298,12 @@ __device__ __forceinline__ PD& mg_entry(const ProbDev* tab, const int2* map, int
 __device__ __forceinline__ const void* mg_xup(PD& P, int l) {
   return l == P.mg_nlev - 1 ? P.mg[l].x : P.mg[l].r;
 }
-// The per-node sweep bodies take their node-vector pointers as template
-// parameters: global memory in the per-level kernels, LDS in the tail.
+// The per-node bodies below (restriction, residual, post-smoothing) serve the
+// tail kernel, whose node vectors are in LDS.
 
-// Restriction b_{l+1} = P^T r_l at coarse node N (level 0: fixed DOFs masked).
-template <typename T, typename VR, typename VB>
-__device__ __forceinline__ void mg_restrict_p(PD& P, int l, long N, VR r, VB bout) {
+// Restriction b_{l+1} = P^T r_l at coarse node N (tail levels, l >= 1).
+template <typename T, typename VR>
+__device__ __forceinline__ vec2<T> mg_restrict_p(PD& P, int l, long N, VR r) {
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny;
   const int cny = P.mg[l + 1].ny;
   const int I = (int)(N / (cny + 1)), J = (int)(N - (long)I * (cny + 1));
@@ -315,21 +315,14 @@ __device__ __forceinline__ void mg_restrict_p(PD& P, int l, long N, VR r, VB bou
       const int i = 2 * I + di, j = 2 * J + dj;
       if (i < 0 || i > fnx || j < 0 || j > fny) continue;
       const double w = (di == 0 ? 1.0 : 0.5) * (dj == 0 ? 1.0 : 0.5);
-      const long m = (long)i * (fny + 1) + j;
-      const vec2<T> v = r[m];
-      double vx = (double)v.x, vy = (double)v.y;
-      if (l == 0) {
-        const unsigned fb = fixbits(P.fix, m);
-        if (fb & 1u) vx = 0.0;
-        if (fb & 2u) vy = 0.0;
-      }
-      bx = fma(w, vx, bx);
-      by = fma(w, vy, by);
+      const vec2<T> v = r[(long)i * (fny + 1) + j];
+      bx = fma(w, (double)v.x, bx);
+      by = fma(w, (double)v.y, by);
     }
   vec2<T> o;
   o.x = (T)bx;
   o.y = (T)by;
-  bout[N] = o;
+  return o;
 }
 
 // Interpolated coarse correction (P x_c) at fine node (i, j); xc holds the
@@ -358,11 +351,11 @@ __device__ __forceinline__ d2v mg_prolong(VC xc, int cny, int i, int j) {
 }
 
 // Sum over node (ix, iy)'s elements of its two rows of K_E times x_E, where
-// xv(dx, dy) gives x at the neighbour (ix + dx, iy + dy) (level >= 1).
-template <typename T, typename XF>
-__device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, int ix, int iy, XF xv) {
+// xv(dx, dy) gives x at the neighbour (ix + dx, iy + dy) (level >= 1); K
+// holds the level's element matrices (global memory or an LDS copy).
+template <typename T, typename KP, typename XF>
+__device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, KP K, int ix, int iy, XF xv) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny;
-  const GLB T* K = gp((const T*)P.mg[l].K);
   double ax = 0.0, ay = 0.0;
   // branch-free: elements outside the grid are read at a clamped address and
   // weighted 0, so all the loads can be in flight together
@@ -374,7 +367,7 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, int ix, int iy, XF xv
       const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
       const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
       const int me = mg_me(a, b);
-      const GLB T* k = K + ((long)cx * ny + cy) * 36;
+      const KP k = K + ((long)cx * ny + cy) * 36;
       double tx = 0.0, ty = 0.0;
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
@@ -391,30 +384,29 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, int ix, int iy, XF xv
   return d2v{ax, ay};
 }
 
-// Down sweep at node n of level l >= 1: x = omega D^-1 b (from zero),
-// r = b - K x.
-template <typename T, typename VB, typename VX>
-__device__ __forceinline__ void mg_down_p(PD& P, int l, long n, VB b, VX x, VX r) {
+// Pre-smoothed iterate x = omega D^-1 b (from zero) at node n of level l.
+template <typename T>
+__device__ __forceinline__ vec2<T> mg_jac(PD& P, int l, long n, vec2<T> bb) {
+  const vec2<T> dd = gp((const vec2<T>*)P.mg[l].dinv)[n];
+  vec2<T> o;
+  o.x = (T)(TOPT_MG_OMEGA * (double)dd.x * (double)bb.x);
+  o.y = (T)(TOPT_MG_OMEGA * (double)dd.y * (double)bb.y);
+  return o;
+}
+// Residual r = b - K x at node n of level l (tail: x, b, r in LDS).
+template <typename T, typename KP>
+__device__ __forceinline__ void mg_resid_p(PD& P, int l, KP K, long n, const vec2<T>* b, const vec2<T>* x,
+                                           vec2<T>* r) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-  const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
-  d2v xs[3][3];
-#pragma unroll
-  for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      const long m = nbr_node(ix, iy, dx - 1, dy - 1, nx, ny);   // clamped: meets void only
-      const vec2<T> bb = b[m], dd = dv[m];
-      xs[dx][dy] = d2v{TOPT_MG_OMEGA * (double)dd.x * (double)bb.x, TOPT_MG_OMEGA * (double)dd.y * (double)bb.y};
-    }
-  const d2v ax = mg_apply_rows<T>(P, l, ix, iy, [&](int dx, int dy) { return xs[dx + 1][dy + 1]; });
+  const d2v ax = mg_apply_rows<T>(P, l, K, ix, iy, [&](int dx, int dy) {
+    const vec2<T> v = x[nbr_node(ix, iy, dx, dy, nx, ny)];   // clamped: meets void only
+    return d2v{(double)v.x, (double)v.y};
+  });
   const vec2<T> bn = b[n];
-  vec2<T> xo, ro;
-  xo.x = (T)xs[1][1].x;
-  xo.y = (T)xs[1][1].y;
+  vec2<T> ro;
   ro.x = (T)((double)bn.x - ax.x);
   ro.y = (T)((double)bn.y - ax.y);
-  x[n] = xo;
   r[n] = ro;
 }
 
@@ -599,31 +591,22 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const ProbDev*
                                  gp((const vec2<T>*)P.pre), ek, gp((vec2<T>*)P.mg[1].b));
 }
 
-// Up sweep at node n of level l >= 1 (not the coarsest) inside the tail:
-// x' = x + P x_{l+1}, out = x' + omega D^-1 (b - K x').
-template <typename T, typename VX, typename VC, typename VB, typename VO>
-__device__ __forceinline__ void mg_up_p(PD& P, int l, long n, VX x, VC xc, VB b, VO out) {
+// Post-smoothing at node n of level l (tail): out = x' + omega D^-1 (b - K x'),
+// x' = x + P x_{l+1} already formed (in LDS).
+template <typename T, typename KP>
+__device__ __forceinline__ void mg_smooth_p(PD& P, int l, KP K, long n, const vec2<T>* xp, const vec2<T>* b,
+                                            vec2<T>* out) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
-  const int cny = P.mg[l + 1].ny;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-  const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
-  d2v xs[3][3];
-#pragma unroll
-  for (int dx = 0; dx < 3; ++dx)
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      int jx = ix + dx - 1, jy = iy + dy - 1;
-      jx = jx < 0 ? 0 : (jx > nx ? nx : jx);
-      jy = jy < 0 ? 0 : (jy > ny ? ny : jy);
-      const vec2<T> v = x[(long)jx * nny + jy];
-      xs[dx][dy] = d2v{(double)v.x, (double)v.y} + mg_prolong<T>(xc, cny, jx, jy);
-    }
-  const d2v ax = mg_apply_rows<T>(P, l, ix, iy, [&](int dx, int dy) { return xs[dx + 1][dy + 1]; });
-  const vec2<T> bn = b[n];
-  const vec2<T> dd = dv[n];
+  const d2v ax = mg_apply_rows<T>(P, l, K, ix, iy, [&](int dx, int dy) {
+    const vec2<T> v = xp[nbr_node(ix, iy, dx, dy, nx, ny)];
+    return d2v{(double)v.x, (double)v.y};
+  });
+  const vec2<T> bn = b[n], xn = xp[n];
+  const vec2<T> dd = gp((const vec2<T>*)P.mg[l].dinv)[n];
   vec2<T> o;
-  o.x = (T)(xs[1][1].x + TOPT_MG_OMEGA * (double)dd.x * ((double)bn.x - ax.x));
-  o.y = (T)(xs[1][1].y + TOPT_MG_OMEGA * (double)dd.y * ((double)bn.y - ax.y));
+  o.x = (T)((double)xn.x + TOPT_MG_OMEGA * (double)dd.x * ((double)bn.x - ax.x));
+  o.y = (T)((double)xn.y + TOPT_MG_OMEGA * (double)dd.y * ((double)bn.y - ax.y));
   out[n] = o;
 }
 
@@ -712,53 +695,109 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
 // Levels lt..L (the small ones) in one workgroup: down sweeps and
 // restrictions to the coarsest level, its dense solve, up sweeps back to lt.
 // The levels' x, b and r live in LDS, so a phase hands its results to the
-// next through LDS and a workgroup barrier; element matrices, 1/diag and
-// the coarse inverse are read from global memory. The kernel starts from
-// b_lt (written by the fused sweep launched before it), and level lt's
-// corrected iterate goes back to global memory (mg_xup) for the up sweep
-// after it.
+// next through LDS and a workgroup barrier. The element matrices of levels
+// lt..L-1 are copied to LDS at the start (coalesced), as many levels as fit
+// from the coarsest up; a level that does not fit is read from global
+// memory (one CU reading 64 scattered entries per node is what bounds a
+// sweep otherwise). 1/diag and the coarse inverse are read from global
+// memory. The kernel starts from b_lt (written by the fused sweep launched
+// before it), and level lt's corrected iterate goes back to global memory
+// (mg_xup) for the up sweep after it.
 // One workgroup per problem.
 #define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
 #define TOPT_MG_TAIL_BLOCK 1024  // each phase is one round trip)
 #define TOPT_MG_TAIL_SUM 2560    // nodes of all tail levels together (LDS: 3 vectors each)
+// element matrices held in LDS: 680 (FP32) or 136 (FP64) elements, what the
+// 160 KB of LDS leaves beside the three vectors
+template <typename T>
+struct TailK {
+  static constexpr int N = (sizeof(T) == 4 ? 680 : 136) * 36;
+};
 template <typename T>
 __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
   __shared__ vec2<T> sx[TOPT_MG_TAIL_SUM], sb[TOPT_MG_TAIL_SUM], sr[TOPT_MG_TAIL_SUM];
-  __shared__ double dpart[TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE][TOPT_MG_DENSE];
+  __shared__ __align__(16) T skb[TailK<T>::N];
+  __shared__ int off[TOPT_MG_MAXLEV], koff[TOPT_MG_MAXLEV];   // level offsets in the vectors and skb
+  constexpr int G = TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE;
   static_assert(TOPT_MG_TAIL_BLOCK % TOPT_MG_DENSE == 0, "tail block: whole groups of coarse rows");
+  static_assert(sizeof(sr) >= sizeof(double) * G * TOPT_MG_DENSE, "dense partial sums fit in sr");
   int lb;
   PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
   const int lt = P.mg_lt;
   const int L = P.mg_nlev - 1;
   auto nodes = [&](int l) { return (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
-  int off[TOPT_MG_MAXLEV];
-  {
+  if (threadIdx.x == 0) {
     int o = 0;
     for (int l = lt; l <= L; ++l) {
       off[l] = o;
       o += (int)nodes(l);
     }
+    int k = 0;
+    for (int l = L - 1; l >= lt; --l) {
+      const int ne = P.mg[l].nx * P.mg[l].ny * 36;
+      koff[l] = k + ne <= TailK<T>::N ? k : -1;
+      if (koff[l] >= 0) k += ne;
+    }
   }
-  // b of level lt, formed by the fused sweep of level lt-1
-  for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK)
-    sb[off[lt] + n] = gp((const vec2<T>*)P.mg[lt].b)[n];
   __syncthreads();
+  // b of level lt, formed by the fused sweep of level lt-1; the element
+  // matrices that fit, in 16-byte pieces (a level's matrices start 256-byte
+  // aligned and span 36 ne entries, a whole number of pieces), each thread's
+  // loads issued before its stores
+  for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) {
+    const vec2<T> bb = gp((const vec2<T>*)P.mg[lt].b)[n];
+    sb[off[lt] + n] = bb;
+    if (lt < L) sx[off[lt] + n] = mg_jac<T>(P, lt, n, bb);
+  }
   for (int l = lt; l < L; ++l) {
-    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
-      mg_down_p<T>(P, l, n, (const vec2<T>*)(sb + off[l]), (vec2<T>*)(sx + off[l]), (vec2<T>*)(sr + off[l]));
+    if (koff[l] < 0) continue;
+    typedef unsigned Q __attribute__((ext_vector_type(4)));
+    constexpr int S = (TailK<T>::N * (int)sizeof(T) / 16 + TOPT_MG_TAIL_BLOCK - 1) / TOPT_MG_TAIL_BLOCK;
+    const GLB Q* src = gp((const Q*)P.mg[l].K);
+    Q* dst = (Q*)(skb + koff[l]);
+    const int n16 = P.mg[l].nx * P.mg[l].ny * 36 * (int)sizeof(T) / 16;
+    Q v[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int u = (int)threadIdx.x + k * TOPT_MG_TAIL_BLOCK;
+      v[k] = src[u < n16 ? u : n16 - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int u = (int)threadIdx.x + k * TOPT_MG_TAIL_BLOCK;
+      if (u < n16) dst[u] = v[k];
+    }
+  }
+  __syncthreads();
+  // down: r_l = b_l - K x_l (x_l = omega D^-1 b_l formed with b_l), then
+  // b_{l+1} = P^T r_l and x_{l+1} in one phase
+  for (int l = lt; l < L; ++l) {
+    const vec2<T>* b = (const vec2<T>*)(sb + off[l]);
+    const vec2<T>* x = (const vec2<T>*)(sx + off[l]);
+    vec2<T>* r = (vec2<T>*)(sr + off[l]);
+    if (koff[l] >= 0) {
+      const T* K = skb + koff[l];
+      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_resid_p<T>(P, l, K, n, b, x, r);
+    } else {
+      const GLB T* K = gp((const T*)P.mg[l].K);
+      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_resid_p<T>(P, l, K, n, b, x, r);
+    }
     __syncthreads();
-    for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK)
-      mg_restrict_p<T>(P, l, n, (const vec2<T>*)(sr + off[l]), (vec2<T>*)(sb + off[l + 1]));
+    for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK) {
+      const vec2<T> bc = mg_restrict_p<T>(P, l, n, (const vec2<T>*)(sr + off[l]));
+      sb[off[l + 1] + n] = bc;
+      if (l + 1 < L) sx[off[l + 1] + n] = mg_jac<T>(P, l + 1, n, bc);
+    }
     __syncthreads();
   }
   {
     // x = A^-1 b; A is symmetric, so row `row` is read as column `row`
-    // (consecutive threads, consecutive addresses). Thread group g (of
-    // TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE) takes columns j = g mod groups; its
-    // loads are independent and issued together, then the groups' partial
-    // sums meet in LDS.
-    constexpr int G = TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE;
+    // (consecutive threads, consecutive addresses). Thread group g takes
+    // columns j = g mod G, with every load issued unconditionally (rows past
+    // N read row N-1 and weigh 0), then the groups' partial sums meet in LDS
+    // (in sr, free between the restrictions and the up sweeps).
+    double* dpart = (double*)sr;
     const int N = P.mg_ncd;
     const T* b = (const T*)(sb + off[L]);
     const int row = threadIdx.x % TOPT_MG_DENSE, g = threadIdx.x / TOPT_MG_DENSE;
@@ -768,24 +807,45 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
 #pragma unroll
       for (int k = 0; k < TOPT_MG_DENSE / G; ++k) {
         const int j = g + k * G;
-        if (j < N) acc = fma(A[(long)j * N], (double)b[j], acc);
+        const int jc = j < N ? j : N - 1;
+        const double a = A[(long)jc * N];
+        acc = fma(j < N ? a : 0.0, (double)b[jc], acc);
       }
-      dpart[g][row] = acc;
+      dpart[g * TOPT_MG_DENSE + row] = acc;
     }
     __syncthreads();
     if (g == 0 && row < N) {
-      double acc = dpart[0][row];
+      double acc = dpart[row];
 #pragma unroll
-      for (int h = 1; h < G; ++h) acc += dpart[h][row];
+      for (int h = 1; h < G; ++h) acc += dpart[h * TOPT_MG_DENSE + row];
       ((T*)(sx + off[L]))[row] = (T)acc;
     }
   }
   __syncthreads();
+  // up: x' = x + P x_{l+1} in place, then the post-smoothing into r
   for (int l = L - 1; l >= lt; --l) {
     const vec2<T>* xc = (const vec2<T>*)(l + 1 == L ? sx + off[L] : sr + off[l + 1]);
-    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
-      mg_up_p<T>(P, l, n, (const vec2<T>*)(sx + off[l]), xc, (const vec2<T>*)(sb + off[l]),
-                 (vec2<T>*)(sr + off[l]));
+    vec2<T>* x = (vec2<T>*)(sx + off[l]);
+    const int nny = P.mg[l].ny + 1, cny = P.mg[l + 1].ny;
+    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) {
+      const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
+      const vec2<T> v = x[n];
+      const d2v c = mg_prolong<T>(xc, cny, ix, iy);
+      vec2<T> o;
+      o.x = (T)((double)v.x + c.x);
+      o.y = (T)((double)v.y + c.y);
+      x[n] = o;
+    }
+    __syncthreads();
+    const vec2<T>* b = (const vec2<T>*)(sb + off[l]);
+    vec2<T>* o = (vec2<T>*)(sr + off[l]);
+    if (koff[l] >= 0) {
+      const T* K = skb + koff[l];
+      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_smooth_p<T>(P, l, K, n, x, b, o);
+    } else {
+      const GLB T* K = gp((const T*)P.mg[l].K);
+      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_smooth_p<T>(P, l, K, n, x, b, o);
+    }
     __syncthreads();
   }
   {
