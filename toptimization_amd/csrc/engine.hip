@@ -129,15 +129,18 @@ int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
   }
   int64_t o = 0;
   auto take = [&](int64_t bytes) { const int64_t at = o; o += align256(bytes); return at; };
+  // level 0 borrows b and 1/diag from the PCG (r, Jacobi diagonal); x is
+  // stored at the coarsest level only (the sweeps form the pre-smoothed
+  // iterate from b), r at levels 1 .. L-1 (the corrected iterate)
+  const int L = M.nlev - 1;
   for (int l = 0; l < M.nlev; ++l) {
     const int64_t nn = int64_t(M.nx[l] + 1) * (M.ny[l] + 1), ne = int64_t(M.nx[l]) * M.ny[l];
     M.K[l] = l > 0 ? take(ne * 36 * ts) : -1;
-    M.x[l] = take(nn * 2 * ts);
-    M.r[l] = take(nn * 2 * ts);
+    M.x[l] = l == L ? take(nn * 2 * ts) : -1;
+    M.r[l] = l > 0 && l < L ? take(nn * 2 * ts) : -1;
     M.b[l] = l > 0 ? take(nn * 2 * ts) : -1;
     M.d[l] = l > 0 ? take(nn * 2 * ts) : -1;
   }
-  const int L = M.nlev - 1;
   M.ncd = 2 * (M.nx[L] + 1) * (M.ny[L] + 1);
   M.inv = take(int64_t(M.ncd) * M.ncd * 8);
   M.total = o;
@@ -327,8 +330,8 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
       V.nx = M.nx[l];
       V.ny = M.ny[l];
       V.K = l > 0 ? (void*)(m + M.K[l]) : nullptr;
-      V.x = m + M.x[l];
-      V.r = m + M.r[l];
+      V.x = M.x[l] >= 0 ? (void*)(m + M.x[l]) : nullptr;
+      V.r = M.r[l] >= 0 ? (void*)(m + M.r[l]) : nullptr;
       V.b = l > 0 ? (void*)(m + M.b[l]) : P.r;
       V.dinv = l > 0 ? (void*)(m + M.d[l]) : P.pre;
     }

@@ -47,6 +47,11 @@ __host__ __device__ constexpr double mg_w(int cx, int cy, int a, int k) {
 }
 // Local index of node (ix, iy) in the element at (ix - 1 + a, iy - 1 + b).
 __host__ __device__ constexpr int mg_me(int a, int b) { return (a == 1) ? ((b == 1) ? 0 : 3) : ((b == 1) ? 1 : 2); }
+// Element matrices of a level with ny element rows are stored entry-major
+// per element column: packed entry t (sym8) of element (ex, ey) at
+// mg_kel(ex, ey, ny) + t * ny, so threads on neighbouring elements of a
+// column read neighbouring words.
+__host__ __device__ inline long mg_kel(int ex, int ey, int ny) { return (long)ex * 36 * ny + ey; }
 
 // ---------------------------------------------------------------- setup
 // Coarse element matrices of level l+1 from level l (FINE: l = 0, children
@@ -86,9 +91,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, 
 #pragma unroll
           for (int j = i; j < 8; ++j) kf[sym8(i, j)] = s * P.Ke64[i * 8 + j] * m[i] * m[j];
       } else {
-        const GLB T* K = gp((const T*)P.mg[l].K) + ((long)ex * fny + ey) * 36;
+        const GLB T* K = gp((const T*)P.mg[l].K) + mg_kel(ex, ey, fny);
 #pragma unroll
-        for (int t = 0; t < 36; ++t) kf[t] = (double)K[t];
+        for (int t = 0; t < 36; ++t) kf[t] = (double)K[(long)t * fny];
       }
       // kc += P_c^T kf P_c, one coarse column (kp, dp) at a time
 #pragma unroll
@@ -123,9 +128,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, 
             }
         }
     }
-  GLB T* out = gp((T*)P.mg[l + 1].K) + E * 36;
+  GLB T* out = gp((T*)P.mg[l + 1].K) + mg_kel(I, J, cny);
 #pragma unroll
-  for (int t = 0; t < 36; ++t) out[t] = (T)kc[t];
+  for (int t = 0; t < 36; ++t) out[(long)t * cny] = (T)kc[t];
 }
 
 // 1/diag of the assembled level-l operator (l >= 1); 0 on decoupled DOFs.
@@ -146,9 +151,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
       const int ex = ix - 1 + a, ey = iy - 1 + b;
       if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
       const int me = mg_me(a, b);
-      const GLB T* k = K + ((long)ex * ny + ey) * 36;
-      dx += (double)k[sym8(2 * me, 2 * me)];
-      dy += (double)k[sym8(2 * me + 1, 2 * me + 1)];
+      const GLB T* k = K + mg_kel(ex, ey, ny);
+      dx += (double)k[(long)sym8(2 * me, 2 * me) * ny];
+      dy += (double)k[(long)sym8(2 * me + 1, 2 * me + 1) * ny];
     }
   vec2<T> d;
   d.x = dx > 0.0 ? (T)(1.0 / dx) : T(0);
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
         for (int ey = max(ey0, 0); ey <= min(ey1, ny - 1); ++ey) {
           const int li = (ix - ex) == 1 ? ((iy - ey) == 1 ? 2 : 1) : ((iy - ey) == 1 ? 3 : 0);
           const int lj = (jx - ex) == 1 ? ((jy - ey) == 1 ? 2 : 1) : ((jy - ey) == 1 ? 3 : 0);
-          v += (double)K[((long)ex * ny + ey) * 36 + sym8(2 * li + di, 2 * lj + dj)];
+          v += (double)K[mg_kel(ex, ey, ny) + (long)sym8(2 * li + di, 2 * lj + dj) * ny];
         }
     }
     A[idx] = v;
@@ -326,28 +331,19 @@ __device__ __forceinline__ vec2<T> mg_restrict_p(PD& P, int l, long N, VR r) {
 }
 
 // Interpolated coarse correction (P x_c) at fine node (i, j); xc holds the
-// coarse level's iterate, cny its element rows.
+// coarse level's iterate, cny its element rows. Branch-free (odd and even
+// nodes share a wavefront): the four reads are issued together, and on an
+// even row or column the second of a pair repeats the first, so the pair's
+// average is that value exactly and the result is bitwise the bilinear
+// interpolation of the 1, 2 or 4 coarse values.
 template <typename T, typename VC>
 __device__ __forceinline__ d2v mg_prolong(VC xc, int cny, int i, int j) {
-  const int I = i >> 1, J = j >> 1;
-  const bool ox = i & 1, oy = j & 1;
-  const long c = (long)I * (cny + 1) + J;
-  const vec2<T> a = xc[c];
-  d2v v = d2v{(double)a.x, (double)a.y};
-  if (ox) {
-    const vec2<T> b = xc[c + cny + 1];
-    v = 0.5 * (v + d2v{(double)b.x, (double)b.y});
-  }
-  if (oy) {
-    const vec2<T> e = xc[c + 1];
-    d2v w = d2v{(double)e.x, (double)e.y};
-    if (ox) {
-      const vec2<T> f = xc[c + cny + 2];
-      w = 0.5 * (w + d2v{(double)f.x, (double)f.y});
-    }
-    v = 0.5 * (v + w);
-  }
-  return v;
+  const long c = (long)(i >> 1) * (cny + 1) + (j >> 1);
+  const long dx = (i & 1) ? cny + 1 : 0, dy = j & 1;
+  const vec2<T> a = xc[c], b = xc[c + dx], e = xc[c + dy], f = xc[c + dx + dy];
+  const d2v v = 0.5 * (d2v{(double)a.x, (double)a.y} + d2v{(double)b.x, (double)b.y});
+  const d2v w = 0.5 * (d2v{(double)e.x, (double)e.y} + d2v{(double)f.x, (double)f.y});
+  return 0.5 * (v + w);
 }
 
 // Sum over node (ix, iy)'s elements of its two rows of K_E times x_E, where
@@ -367,16 +363,16 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, KP K, int ix, int iy,
       const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
       const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
       const int me = mg_me(a, b);
-      const KP k = K + ((long)cx * ny + cy) * 36;
+      const KP k = K + mg_kel(cx, cy, ny);
       double tx = 0.0, ty = 0.0;
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
         // element corner kk is neighbour (a - 1 + lx, b - 1 + ly) of the node
         const d2v xv_ = xv(a - 1 + mg_lx(kk), b - 1 + mg_ly(kk));
-        tx = fma((double)k[sym8(2 * me, 2 * kk)], xv_.x, tx);
-        tx = fma((double)k[sym8(2 * me, 2 * kk + 1)], xv_.y, tx);
-        ty = fma((double)k[sym8(2 * me + 1, 2 * kk)], xv_.x, ty);
-        ty = fma((double)k[sym8(2 * me + 1, 2 * kk + 1)], xv_.y, ty);
+        tx = fma((double)k[(long)sym8(2 * me, 2 * kk) * ny], xv_.x, tx);
+        tx = fma((double)k[(long)sym8(2 * me, 2 * kk + 1) * ny], xv_.y, tx);
+        ty = fma((double)k[(long)sym8(2 * me + 1, 2 * kk) * ny], xv_.x, ty);
+        ty = fma((double)k[(long)sym8(2 * me + 1, 2 * kk + 1) * ny], xv_.y, ty);
       }
       ax += ok ? tx : 0.0;
       ay += ok ? ty : 0.0;
@@ -446,12 +442,12 @@ struct MgElems {
   __device__ __forceinline__ R fetch(int ex, int ey, int me) const {
     const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
     const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
-    const GLB T* k = K + ((long)cx * ny + cy) * 36;
+    const GLB T* k = K + mg_kel(cx, cy, ny);
     R r;
 #pragma unroll
     for (int d = 0; d < 2; ++d)
 #pragma unroll
-      for (int c = 0; c < 8; ++c) r.k[d * 8 + c] = k[sym8(2 * me + d, c)];
+      for (int c = 0; c < 8; ++c) r.k[d * 8 + c] = k[(long)sym8(2 * me + d, c) * ny];
     r.f = ok ? 1.0 : 0.0;
     return r;
   }

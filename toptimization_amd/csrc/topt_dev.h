@@ -68,10 +68,10 @@ template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? TOP
 struct MgLevel {
   int nx, ny;     // elements
   void* K;        // levels >= 1: Galerkin element matrices, 36 T each (packed upper
-                  // triangle, element-major: one element's entries share a cache line)
-  void* x;        // pre-smoothed iterate
+                  // triangle), entry-major per element column (mg_kel)
+  void* x;        // coarsest level: the coarse solve's result (null elsewhere)
   void* b;        // right-hand side (level 0: the PCG residual r)
-  void* r;        // residual; after restriction it receives the post-smoothed iterate
+  void* r;        // levels 1 .. L-1: the corrected, post-smoothed iterate (null elsewhere)
   void* dinv;     // 1 / diag (level 0: the Jacobi data in pre); 0 on decoupled DOFs
 };
 
