@@ -380,10 +380,9 @@ __device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, KP K, int ix, int iy,
   return d2v{ax, ay};
 }
 
-// Pre-smoothed iterate x = omega D^-1 b (from zero) at node n of level l.
+// Pre-smoothed iterate x = omega D^-1 b (from zero) at a node with 1/diag dd.
 template <typename T>
-__device__ __forceinline__ vec2<T> mg_jac(PD& P, int l, long n, vec2<T> bb) {
-  const vec2<T> dd = gp((const vec2<T>*)P.mg[l].dinv)[n];
+__device__ __forceinline__ vec2<T> mg_jac(vec2<T> dd, vec2<T> bb) {
   vec2<T> o;
   o.x = (T)(TOPT_MG_OMEGA * (double)dd.x * (double)bb.x);
   o.y = (T)(TOPT_MG_OMEGA * (double)dd.y * (double)bb.y);
@@ -415,13 +414,17 @@ __device__ __forceinline__ void mg_resid_p(PD& P, int l, KP K, long n, const vec
 // (2 DR_CX + 1) x (2 DR_CY + 1) fine nodes their restrictions gather (one
 // per thread; the element rows each thread needs are fetched before the
 // barrier), then the restrictions from LDS (level 0: fixed DOFs masked).
-// Fine nodes on a tile's border are formed by both neighbouring tiles (255
-// residuals per tile against its 4 x 56 = 224 fine nodes). x_l is not
+// Fine nodes on a tile's border are formed by both neighbouring tiles (245
+// residuals per tile against its 4 x 48 = 192 fine nodes). x_l is not
 // stored: the up sweep forms it again from b_l.
+#ifndef TOPT_MG_DR_CX
+#define TOPT_MG_DR_CX 2    // 2 x 24 coarse nodes: long runs along the memory order
+#define TOPT_MG_DR_CY 24   // (7 x 8, 3 x 16 and 1 x 31 measured slower)
+#endif
 struct DrTile {
-  static constexpr int CX = 7, CY = 8;                     // coarse nodes
-  static constexpr int RX = 2 * CX + 1, RY = 2 * CY + 1;   // fine residuals (15 x 17)
-  static constexpr int XX = RX + 2, XY = RY + 2;           // fine iterates (17 x 19)
+  static constexpr int CX = TOPT_MG_DR_CX, CY = TOPT_MG_DR_CY;   // coarse nodes
+  static constexpr int RX = 2 * CX + 1, RY = 2 * CY + 1;         // fine residuals (5 x 49)
+  static constexpr int XX = RX + 2, XY = RY + 2;                 // fine iterates (7 x 51)
 };
 static_assert(DrTile::RX * DrTile::RY <= TOPT_BLOCK, "one fine residual per thread");
 // Workgroups of the fused sweep for a coarse level of cnx x cny elements.
@@ -588,10 +591,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const ProbDev*
 }
 
 // Post-smoothing at node n of level l (tail): out = x' + omega D^-1 (b - K x'),
-// x' = x + P x_{l+1} already formed (in LDS).
+// x' = x + P x_{l+1} already formed (in LDS), dd = 1/diag at n.
 template <typename T, typename KP>
 __device__ __forceinline__ void mg_smooth_p(PD& P, int l, KP K, long n, const vec2<T>* xp, const vec2<T>* b,
-                                            vec2<T>* out) {
+                                            vec2<T> dd, vec2<T>* out) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
   const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
   const d2v ax = mg_apply_rows<T>(P, l, K, ix, iy, [&](int dx, int dy) {
@@ -599,7 +602,6 @@ __device__ __forceinline__ void mg_smooth_p(PD& P, int l, KP K, long n, const ve
     return d2v{(double)v.x, (double)v.y};
   });
   const vec2<T> bn = b[n], xn = xp[n];
-  const vec2<T> dd = gp((const vec2<T>*)P.mg[l].dinv)[n];
   vec2<T> o;
   o.x = (T)((double)xn.x + TOPT_MG_OMEGA * (double)dd.x * ((double)bn.x - ax.x));
   o.y = (T)((double)xn.y + TOPT_MG_OMEGA * (double)dd.y * ((double)bn.y - ax.y));
@@ -612,8 +614,12 @@ __device__ __forceinline__ void mg_smooth_p(PD& P, int l, KP K, long n, const ve
 // thread's node fetched before the barrier, then out = x' + omega D^-1
 // (b - K x'). The corrected iterate goes to r (the fused down sweep stores
 // no residual).
+#ifndef TOPT_MG_UP_TX
+#define TOPT_MG_UP_TX 4    // one node column per wavefront (8 x 32 and 16 x 16
+#define TOPT_MG_UP_TY 64   // measured slower)
+#endif
 struct UpTile {
-  static constexpr int TX = 8, TY = 32;              // nodes (TY along the memory order)
+  static constexpr int TX = TOPT_MG_UP_TX, TY = TOPT_MG_UP_TY;   // nodes (TY along the memory order)
   static constexpr int PX = TX + 2, PY = TY + 2;     // patch with its ring
 };
 static_assert(UpTile::TX * UpTile::TY == TOPT_BLOCK, "one node per thread");
@@ -700,14 +706,19 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
 // before it), and level lt's corrected iterate goes back to global memory
 // (mg_xup) for the up sweep after it.
 // One workgroup per problem.
+#ifndef TOPT_MG_TAIL
 #define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
-#define TOPT_MG_TAIL_BLOCK 1024  // each phase is one round trip)
+#endif                           // each phase is one round trip)
+#define TOPT_MG_TAIL_BLOCK 1024
+#ifndef TOPT_MG_TAIL_SUM
 #define TOPT_MG_TAIL_SUM 2560    // nodes of all tail levels together (LDS: 3 vectors each)
-// element matrices held in LDS: 680 (FP32) or 136 (FP64) elements, what the
-// 160 KB of LDS leaves beside the three vectors
+#endif
+// Element matrices held in LDS: whole elements in what the 160 KB of LDS
+// leaves beside the three vectors and the level offsets (FP32: 707
+// elements, FP64: 140).
 template <typename T>
 struct TailK {
-  static constexpr int N = (sizeof(T) == 4 ? 680 : 136) * 36;
+  static constexpr int N = (163840 - 3 * TOPT_MG_TAIL_SUM * 2 * (int)sizeof(T) - 512) / (36 * (int)sizeof(T)) * 36;
 };
 template <typename T>
 __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
@@ -744,7 +755,7 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) {
     const vec2<T> bb = gp((const vec2<T>*)P.mg[lt].b)[n];
     sb[off[lt] + n] = bb;
-    if (lt < L) sx[off[lt] + n] = mg_jac<T>(P, lt, n, bb);
+    if (lt < L) sx[off[lt] + n] = mg_jac<T>(gp((const vec2<T>*)P.mg[lt].dinv)[n], bb);
   }
   for (int l = lt; l < L; ++l) {
     if (koff[l] < 0) continue;
@@ -768,10 +779,17 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   __syncthreads();
   // down: r_l = b_l - K x_l (x_l = omega D^-1 b_l formed with b_l), then
   // b_{l+1} = P^T r_l and x_{l+1} in one phase
+  // (a tail level has at most one node per thread: with FP32 vectors each
+  // thread fetches the 1/diag its next phase needs before the barrier in
+  // between; FP64 has no registers to spare for it)
+  constexpr bool PF = sizeof(T) == 4;
   for (int l = lt; l < L; ++l) {
     const vec2<T>* b = (const vec2<T>*)(sb + off[l]);
     const vec2<T>* x = (const vec2<T>*)(sx + off[l]);
     vec2<T>* r = (vec2<T>*)(sr + off[l]);
+    const GLB vec2<T>* dnx = gp((const vec2<T>*)P.mg[l + 1].dinv);
+    vec2<T> dnext = vec2<T>{T(0), T(0)};
+    if (PF && l + 1 < L && threadIdx.x < nodes(l + 1)) dnext = dnx[threadIdx.x];
     if (koff[l] >= 0) {
       const T* K = skb + koff[l];
       for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_resid_p<T>(P, l, K, n, b, x, r);
@@ -783,7 +801,7 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK) {
       const vec2<T> bc = mg_restrict_p<T>(P, l, n, (const vec2<T>*)(sr + off[l]));
       sb[off[l + 1] + n] = bc;
-      if (l + 1 < L) sx[off[l + 1] + n] = mg_jac<T>(P, l + 1, n, bc);
+      if (l + 1 < L) sx[off[l + 1] + n] = mg_jac<T>(PF && n < TOPT_MG_TAIL_BLOCK ? dnext : dnx[n], bc);
     }
     __syncthreads();
   }
@@ -823,6 +841,9 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     const vec2<T>* xc = (const vec2<T>*)(l + 1 == L ? sx + off[L] : sr + off[l + 1]);
     vec2<T>* x = (vec2<T>*)(sx + off[l]);
     const int nny = P.mg[l].ny + 1, cny = P.mg[l + 1].ny;
+    const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
+    vec2<T> dcur = vec2<T>{T(0), T(0)};
+    if (PF && threadIdx.x < nodes(l)) dcur = dv[threadIdx.x];
     for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) {
       const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
       const vec2<T> v = x[n];
@@ -837,10 +858,12 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     vec2<T>* o = (vec2<T>*)(sr + off[l]);
     if (koff[l] >= 0) {
       const T* K = skb + koff[l];
-      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_smooth_p<T>(P, l, K, n, x, b, o);
+      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
+        mg_smooth_p<T>(P, l, K, n, x, b, PF && n < TOPT_MG_TAIL_BLOCK ? dcur : dv[n], o);
     } else {
       const GLB T* K = gp((const T*)P.mg[l].K);
-      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_smooth_p<T>(P, l, K, n, x, b, o);
+      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
+        mg_smooth_p<T>(P, l, K, n, x, b, PF && n < TOPT_MG_TAIL_BLOCK ? dcur : dv[n], o);
     }
     __syncthreads();
   }
