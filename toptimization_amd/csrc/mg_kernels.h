@@ -165,10 +165,14 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
 // The nodes are numbered along the longer side of the grid (permutation
 // pos), which makes the matrix banded with half-bandwidth w = 2 (m + 1) + 1
 // DOFs, m = nodes across the shorter side (<= 8 when N <= 128, so w <= 19).
-// Steps: assemble A (decoupled DOFs get a unit diagonal); banded Cholesky
-// A = L L^T in place (diagonal kept apart); copy the band of L to a compact
-// array; solve L L^T x = e_c for every column c (one thread per column, the
-// columns overwrite A); store A^-1 in the natural DOF order.
+// Steps: stage the level's element matrices in LDS (in the band array, free
+// until the factorization ends); assemble the lower band of A (decoupled
+// DOFs get a unit diagonal); banded Cholesky A = L L^T in place, one barrier
+// per column (column k is scaled by 1 / l_kk only when the band of L is
+// copied out, so a step reads nothing it writes); then every column c of
+// A^-1 by row-oriented substitutions L y = e_c, L^T x = y (one thread per
+// column, each step a dot product over the band whose reads are issued
+// together); store A^-1 in the natural DOF order.
 // Modified pivots: a pivot a_kk becomes max(|a_kk|, tau d_k), d_k the
 // assembled diagonal. With FP32 level matrices the rounding of the Galerkin
 // chain (relative 6e-8 per stored entry) exceeds the energy of near-null
@@ -182,6 +186,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const ProbDev* tab, const int2* map,
                                                                          ProbDev single) {
+  constexpr int BW = TOPT_MG_BAND, NB = TOPT_MG_SETUP_BLOCK;
   int lb;
   PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
   const int L = P.mg_nlev - 1;
@@ -191,94 +196,151 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
   const int m = tr ? nnx : nny;
   const int w = min(2 * (m + 1) + 1, N - 1);
   __shared__ double A[TOPT_MG_DENSE * TOPT_MG_DENSE];
-  __shared__ double Lb[TOPT_MG_DENSE * (TOPT_MG_BAND + 1)];
-  __shared__ double Ld[TOPT_MG_DENSE];
-  __shared__ double D0[TOPT_MG_DENSE];
+  __shared__ double Lb[TOPT_MG_DENSE * (BW + 1)];   // first: the element matrices
+  __shared__ double Ld[TOPT_MG_DENSE], D0[TOPT_MG_DENSE];
+  __shared__ int px[TOPT_MG_DENSE / 2], py[TOPT_MG_DENSE / 2];
+  __shared__ short2 tri[BW * (BW + 1) / 2];   // (ri, rj), rj <= ri: the trailing band of a column
+  static_assert(36 * 49 <= TOPT_MG_DENSE * (TOPT_MG_BAND + 1), "coarsest element matrices fit the band array");
   const double tau = sizeof(T) == 4 ? 1e-3 : 1e-12;
-  // permuted node p -> grid (ix, iy)
-  auto node_of = [&](int p, int& ix, int& iy) {
-    if (tr) { iy = p / nnx; ix = p - iy * nnx; }
-    else { ix = p / nny; iy = p - ix * nny; }
-  };
-  const GLB T* K = gp((const T*)P.mg[L].K);
-  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
-    const int i = idx / N, j = idx - (idx / N) * N;
+  const int t = (int)threadIdx.x;
+  // element matrices (<= 49 elements: <= 64 nodes) and node coordinates of
+  // the permuted numbering
+  {
+    constexpr int S = (36 * 49 + NB - 1) / NB;   // loads per thread, issued before the stores
+    const GLB T* K = gp((const T*)P.mg[L].K);
+    const int nk = nx * ny * 36;
+    T v[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) v[k] = K[min(t + k * NB, nk - 1)];
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+      if (t + k * NB < nk) Lb[t + k * NB] = (double)v[k];
+    for (int p = t; p < nnx * nny; p += NB) {
+      if (tr) { py[p] = p / nnx; px[p] = p - (p / nnx) * nnx; }
+      else { px[p] = p / nny; py[p] = p - (p / nny) * nny; }
+    }
+    for (int u = t; u < BW * (BW + 1) / 2; u += NB) {
+      int ri = 0;
+      while ((ri + 1) * (ri + 2) / 2 <= u) ++ri;
+      tri[u] = make_short2((short)ri, (short)(u - ri * (ri + 1) / 2));
+    }
+  }
+  __syncthreads();
+  // lower band of A: entries (i, i - d), d = 0..w
+  for (int u = t; u < N * (w + 1); u += NB) {
+    const int i = u / (w + 1), j = i - (u - (u / (w + 1)) * (w + 1));
+    if (j < 0) continue;
+    const int ix = px[i >> 1], iy = py[i >> 1], jx = px[j >> 1], jy = py[j >> 1];
     const int di = i & 1, dj = j & 1;
-    int ix, iy, jx, jy;
-    node_of(i >> 1, ix, iy);
-    node_of(j >> 1, jx, jy);
     double v = 0.0;
     if (abs(ix - jx) <= 1 && abs(iy - jy) <= 1) {
-      const int ex0 = max(ix, jx) - 1, ex1 = min(ix, jx);
-      const int ey0 = max(iy, jy) - 1, ey1 = min(iy, jy);
-      for (int ex = max(ex0, 0); ex <= min(ex1, nx - 1); ++ex)
-        for (int ey = max(ey0, 0); ey <= min(ey1, ny - 1); ++ey) {
+      for (int ex = max(max(ix, jx) - 1, 0); ex <= min(min(ix, jx), nx - 1); ++ex)
+        for (int ey = max(max(iy, jy) - 1, 0); ey <= min(min(iy, jy), ny - 1); ++ey) {
           const int li = (ix - ex) == 1 ? ((iy - ey) == 1 ? 2 : 1) : ((iy - ey) == 1 ? 3 : 0);
           const int lj = (jx - ex) == 1 ? ((jy - ey) == 1 ? 2 : 1) : ((jy - ey) == 1 ? 3 : 0);
-          v += (double)K[mg_kel(ex, ey, ny) + (long)sym8(2 * li + di, 2 * lj + dj) * ny];
+          v += Lb[mg_kel(ex, ey, ny) + (long)sym8(2 * li + di, 2 * lj + dj) * ny];
         }
     }
-    A[idx] = v;
+    if (i == j && !(v > 0.0)) v = 1.0;
+    A[i * N + j] = v;
+    if (i == j) D0[i] = v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < N; i += TOPT_MG_SETUP_BLOCK) {
-    if (!(A[i * N + i] > 0.0)) A[i * N + i] = 1.0;
-    D0[i] = A[i * N + i];
-  }
-  __syncthreads();
-  // banded Cholesky: column k scaled by 1 / l_kk, then the trailing band updated
+  // banded Cholesky, column k: pivot, then the trailing band (k < j <= i <=
+  // k + w) loses a_ik a_jk / a_kk; column k itself is left unscaled
   for (int k = 0; k < N; ++k) {
-    const int kmax = min(N - 1, k + w);
-    const double lkk = sqrt(fmax(fabs(A[k * N + k]), tau * D0[k]));
-    if (threadIdx.x == 0) Ld[k] = lkk;
-    for (int i = k + 1 + threadIdx.x; i <= kmax; i += TOPT_MG_SETUP_BLOCK) A[i * N + k] /= lkk;
-    __syncthreads();
-    const int nr = kmax - k;   // rows below the diagonal in the band
-    for (int t = threadIdx.x; t < nr * (nr + 1) / 2; t += TOPT_MG_SETUP_BLOCK) {
-      // t -> (i, j), k < j <= i <= kmax
-      int ri = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-      while ((ri + 1) * (ri + 2) / 2 <= t) ++ri;
-      while (ri * (ri + 1) / 2 > t) --ri;
-      const int rj = t - ri * (ri + 1) / 2;
-      const int i = k + 1 + ri, j = k + 1 + rj;
-      A[i * N + j] = fma(-A[i * N + k], A[j * N + k], A[i * N + j]);
+    const double akk = fmax(fabs(A[k * N + k]), tau * D0[k]);
+    const double inv = 1.0 / akk;
+    if (t == 0) Ld[k] = sqrt(akk);
+    const int nr = min(N - 1, k + w) - k;   // rows below the diagonal in the band
+    for (int u = t; u < nr * (nr + 1) / 2; u += NB) {
+      const short2 q = tri[u];
+      const int i = k + 1 + q.x, j = k + 1 + q.y;
+      A[i * N + j] = fma(-A[i * N + k] * inv, A[j * N + k], A[i * N + j]);
     }
     __syncthreads();
   }
-  // compact band of L: Lb[i][d] = L(i, i - d), d = 1..w
-  for (int idx = threadIdx.x; idx < N * w; idx += TOPT_MG_SETUP_BLOCK) {
-    const int i = idx / w, d = idx - (idx / w) * w + 1;
-    Lb[i * (TOPT_MG_BAND + 1) + d] = i - d >= 0 ? A[i * N + (i - d)] : 0.0;
+  // band of L (Lb[i][d] = L(i, i - d), 0 for d > w or i - d < 0) and 1 / l_jj
+  // (in D0)
+  for (int u = t; u < N * BW; u += NB) {
+    const int i = u / BW, d = u - (u / BW) * BW + 1;
+    Lb[i * (BW + 1) + d] = d <= w && i - d >= 0 ? A[i * N + (i - d)] / Ld[i - d] : 0.0;
   }
+  for (int i = t; i < N; i += NB) D0[i] = 1.0 / Ld[i];
   __syncthreads();
-  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK)
-    A[idx] = (idx / N) == (idx - (idx / N) * N) ? 1.0 : 0.0;
-  __syncthreads();
-  // column c of the inverse: forward L y = e_c, then backward L^T x = y
-  const int c = threadIdx.x;
+  // column c of the inverse: y_j = (e_c - sum_d L(j, j - d) y_{j-d}) / l_jj,
+  // then x_j = (y_j - sum_d L(j + d, j) x_{j+d}) / l_jj; every lane runs all
+  // N steps (y_j = 0 for j < c comes out of the recurrence). The last BW
+  // values of a recurrence stay in registers: a ring of BW + 1 slots,
+  // indexed statically by unrolling the steps by BW + 1, so a step reads
+  // only the band of L (the same address for all lanes); the terms of
+  // values older than the previous one are summed first, and the previous
+  // value enters last, so consecutive steps overlap.
+  const int c = t;
   if (c < N) {
-    for (int j = c; j < N; ++j) {
-      const double yj = A[j * N + c] / Ld[j];
-      A[j * N + c] = yj;
-      for (int i = j + 1; i <= min(N - 1, j + w); ++i)
-        A[i * N + c] = fma(-Lb[i * (TOPT_MG_BAND + 1) + (i - j)], yj, A[i * N + c]);
+    constexpr int R = BW + 1;
+    double ring[R];
+    // forward step j (ring slot r = j mod R)
+    auto fwd = [&](int r, int j) {
+      const double* lj = Lb + j * (BW + 1);
+      double a0 = j == c ? 1.0 : 0.0, a1 = 0.0;
+#pragma unroll
+      for (int d = BW; d >= 2; --d) {
+        // L(j, j - d) is 0 for d > w and for j - d < 0 (ring slot still 0)
+        if (d & 1) a1 = fma(-lj[d], ring[(r - d + R) % R], a1);
+        else a0 = fma(-lj[d], ring[(r - d + R) % R], a0);
+      }
+      const double y = fma(-lj[1], ring[(r - 1 + R) % R], a0 + a1) * D0[j];
+      ring[r] = y;
+      A[j * N + c] = y;
+    };
+    // backward step k, row j = N - 1 - k (x_{j+d} sits d steps back)
+    auto bwd = [&](int r, int k) {
+      const int j = N - 1 - k;
+      double a0 = A[j * N + c], a1 = 0.0;
+#pragma unroll
+      for (int d = BW; d >= 2; --d) {
+        // L(j + d, j): 0 for d > w; rows past N - 1 meet a ring slot still 0
+        const int jj = j + d < N ? j + d : N - 1;
+        if (d & 1) a1 = fma(-Lb[jj * (BW + 1) + d], ring[(r - d + R) % R], a1);
+        else a0 = fma(-Lb[jj * (BW + 1) + d], ring[(r - d + R) % R], a0);
+      }
+      const int j1 = j + 1 < N ? j + 1 : N - 1;
+      const double x = fma(-Lb[j1 * (BW + 1) + 1], ring[(r - 1 + R) % R], a0 + a1) * D0[j];
+      ring[r] = x;
+      A[j * N + c] = x;
+    };
+    // whole blocks of R steps carry no bounds test, so a block's reads can
+    // be issued ahead of its recurrence; then the partial block
+#pragma unroll
+    for (int q = 0; q < R; ++q) ring[q] = 0.0;
+    int j0 = 0;
+    for (; j0 + R <= N; j0 += R) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) fwd(r, j0 + r);
     }
-    for (int j = N - 1; j >= 0; --j) {
-      const double xj = A[j * N + c] / Ld[j];
-      A[j * N + c] = xj;
-      for (int i = max(0, j - w); i < j; ++i)
-        A[i * N + c] = fma(-Lb[j * (TOPT_MG_BAND + 1) + (j - i)], xj, A[i * N + c]);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (j0 + r < N) fwd(r, j0 + r);
+#pragma unroll
+    for (int q = 0; q < R; ++q) ring[q] = 0.0;
+    int k0 = 0;
+    for (; k0 + R <= N; k0 += R) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) bwd(r, k0 + r);
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (k0 + r < N) bwd(r, k0 + r);
   }
   __syncthreads();
-  // back to the natural DOF order (node ix * nny + iy)
-  for (int idx = threadIdx.x; idx < N * N; idx += TOPT_MG_SETUP_BLOCK) {
-    const int i = idx / N, j = idx - (idx / N) * N;
-    int ix, iy, jx, jy;
-    node_of(i >> 1, ix, iy);
-    node_of(j >> 1, jx, jy);
-    const int ni = 2 * (ix * nny + iy) + (i & 1), nj = 2 * (jx * nny + jy) + (j & 1);
-    gp(P.mg_inv)[(long)ni * N + nj] = A[idx];
+  // natural DOF order (node ix * nny + iy): row i by all threads together
+  if (c < N) {
+    const int nc = 2 * (px[c >> 1] * nny + py[c >> 1]) + (c & 1);
+    for (int i = 0; i < N; ++i) {
+      const int ni = 2 * (px[i >> 1] * nny + py[i >> 1]) + (i & 1);
+      gp(P.mg_inv)[(long)ni * N + nc] = A[i * N + c];
+    }
   }
 }
 
