@@ -310,7 +310,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--probe", action="store_true",
-                    help="profiling driver: one OC iteration, 100 slot-A launches, op-level matvecs")
+                    help="profiling driver: three multigrid OC iterations, one block-Jacobi OC iteration, "
+                         "100 slot-A launches, op-level matvecs, copy calibration")
     ap.add_argument("--batch", action="store_true",
                     help="64 x 128x64 cantilevers: one batched plan vs one at a time (own JSON line)")
     ap.add_argument("--serving", action="store_true",
