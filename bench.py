@@ -218,8 +218,9 @@ def mbb_bench(log, iters=100):
     """300x100 half MBB beam (left edge x-roller, bottom-right y-roller, unit
     load at the top-left corner), volfrac 0.5, rmin 4, p 3, 100 OC iterations
     from rho = 0.5: the mixed FP32/FP64 path with 2x2 block Jacobi (rtol 1e-6)
-    against the plain FP64 path (same preconditioner) and against mixed Jacobi
-    (CG count of the two preconditioners)."""
+    against the plain FP64 path (same preconditioner), against mixed Jacobi
+    (CG count of the two preconditioners) and against mixed multigrid (the
+    drop-in preconditioner of the bridge line)."""
     import numpy as np
     import torch
     from toptimization_amd.engine import DesignState, Plan
@@ -236,7 +237,7 @@ def mbb_bench(log, iters=100):
 
     out = {}
     for key, prec, pc in (("mixed", "mixed", "block_jacobi_2x2"), ("fp64", "fp64", "block_jacobi_2x2"),
-                          ("mixed_jacobi", "mixed", "jacobi")):
+                          ("mixed_jacobi", "mixed", "jacobi"), ("mixed_multigrid", "mixed", "multigrid")):
         st = DesignState(mbb(prec, pc))
         plan = Plan([st])
         torch.cuda.synchronize()
@@ -261,6 +262,10 @@ def mbb_bench(log, iters=100):
             "mixed_over_fp64": round(out["fp64"]["seconds"] / m["seconds"], 3),
             "jacobi_cg_total": out["mixed_jacobi"]["n_cg"],
             "jacobi_oc_its_per_s": round(iters / out["mixed_jacobi"]["seconds"], 2),
+            "multigrid_oc_its_per_s": round(iters / out["mixed_multigrid"]["seconds"], 2),
+            "multigrid_cg_total": out["mixed_multigrid"]["n_cg"],
+            "multigrid_compliance_rel_err_final": float(abs(out["mixed_multigrid"]["c"][-1] - out["fp64"]["c"][-1])
+                                                        / abs(out["fp64"]["c"][-1])),
             "compliance_rel_err_final": float(rel[-1]), "compliance_rel_err_max": float(rel.max()),
             "pcg_gbs_algorithmic": round(b_it * m["n_cg"] / m["seconds"] / 1e9, 1),
             "dtype": "f32 vectors + f64 dots/refresh/design fields", "data": "synthetic"}

@@ -606,7 +606,9 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
 }
 
 #define TOPT_GRAPH_CHUNK 32
+#ifndef TOPT_MG_CHUNK
 #define TOPT_MG_CHUNK 2   // CG iterations per chunk of a multigrid solve
+#endif
 
 // The slot launches of one full chunk (count iterations), captured into a
 // graph. Every full chunk starts at an even sequence number and an even
