@@ -12,7 +12,7 @@
 // interpolation from the coarse element's corners to child c's corners and
 // K_c = s_c Ke (fixed rows and columns zeroed) at level 0. The coarsest level
 // (<= TOPT_MG_DENSE DOFs) is solved with a dense inverse formed once per
-// solve. Smoother: damped Jacobi (omega 0.6), one sweep before and one after
+// solve. Smoother: damped Jacobi (omega 0.65), one sweep before and one after
 // the coarse correction, so M is symmetric positive definite and plain PCG
 // applies. The iteration count depends little on the grid size (a 300x100
 // bridge after 20 OC iterations: 69 iterations against 1,738 with Jacobi).
@@ -26,10 +26,13 @@ namespace topt {
 // masking, and every Galerkin agglomerate of them) has lambda_max(D_e^-1
 // K_e) = 2.889 for the unit-square Q4 with nu = 0.3, so lambda_max(D^-1 K)
 // <= 2.889 on every level whatever the densities, and a damped Jacobi step
-// converges in the energy norm for omega < 2 / 2.889 = 0.69. 0.6 keeps the
-// V-cycle SPD with margin (0.7 and 0.8 save 4-8 % of the iterations on
-// sample designs but leave the guaranteed range; 0.9 diverges).
-#define TOPT_MG_OMEGA 0.6
+// converges in the energy norm for omega < 2 / 2.889 = 0.69. 0.65 keeps the
+// V-cycle SPD with margin (omega lambda_max <= 1.88; against 0.6 it saves
+// 2 % of the iterations of a 1200x400 bridge over 30 OC iterations; 0.7 and
+// 0.8 leave the guaranteed range, 0.9 diverges).
+#ifndef TOPT_MG_OMEGA
+#define TOPT_MG_OMEGA 0.65
+#endif
 
 // Packed index of entry (i, j) of a symmetric 8x8 matrix (upper triangle,
 // row major: row i holds columns i..7).
