@@ -343,12 +343,15 @@ size_t filter_shmem(int R) {
   return size_t(TOPT_FILT_TX + 2 * R) * size_t(TOPT_FILT_TY + 2 * R) * sizeof(double);
 }
 
+// out: pinned host memory mapped into the device (read by the host after the
+// event recorded behind this kernel)
 __global__ void k_gather_status(const ProbDev* tab, int n, int parity, int* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const CgState* st = tab[i].cg + parity;
   out[2 * i] = st->status;
   out[2 * i + 1] = st->it;
+  __threadfence_system();
 }
 
 __global__ void k_gather_stats(const ProbDev* tab, const int* idx, int n, topt_iter_stats* out) {
@@ -444,7 +447,10 @@ struct topt_plan {
   int nblk[TOPT_NGRID] = {0, 0, 0, 0, 0};
   int64_t q = 0;          // PCG slot sequence number of the next launch
   int64_t na = 0;         // slot-A launches so far in this solve (p buffer parity)
-  int* dstat = nullptr;
+  // (status, iteration) per problem after a chunk: pinned host slots mapped
+  // into the device address space (dstat: their device addresses), which
+  // k_gather_status writes directly (no copy operation behind each chunk)
+  int* dstat[2] = {nullptr, nullptr};
   int* hstat[2] = {nullptr, nullptr};
   topt_iter_stats* dstats = nullptr;
   topt_iter_stats* hstats = nullptr;
@@ -658,9 +664,8 @@ int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n,
-                     (int)(pl->q & 1), pl->dstat);
+                     (int)(pl->q & 1), pl->dstat[slot]);
   LAUNCH_TRY();
-  HIP_TRY(hipMemcpyAsync(pl->hstat[slot], pl->dstat, sizeof(int) * 2 * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipEventRecord(pl->ev[slot], s));
   return TOPT_OK;
 }
@@ -1172,11 +1177,12 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
     if (hipHostMalloc(&pl->sub_hs[k], sizeof(topt_iter_stats) * nprob, hipHostMallocDefault) != hipSuccess)
       return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc subset stats"));
   }
-  if (hipMalloc(&pl->dstat, sizeof(int) * 2 * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc status"));
   if (hipMalloc(&pl->dstats, sizeof(topt_iter_stats) * nprob) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc stats"));
   for (int k = 0; k < 2; ++k) {
-    if (hipHostMalloc(&pl->hstat[k], sizeof(int) * 2 * nprob, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(&pl->hstat[k], sizeof(int) * 2 * nprob, hipHostMallocMapped) != hipSuccess)
       return cleanup(fail(TOPT_ENOMEM, "hipHostMalloc status"));
+    if (hipHostGetDevicePointer((void**)&pl->dstat[k], pl->hstat[k], 0) != hipSuccess)
+      return cleanup(fail(TOPT_EHIP, "hipHostGetDevicePointer status"));
     if (hipEventCreateWithFlags(&pl->ev[k], hipEventDisableTiming) != hipSuccess)
       return cleanup(fail(TOPT_EHIP, "hipEventCreate"));
   }
@@ -1243,7 +1249,6 @@ int topt_plan_destroy(topt_plan* pl) {
   if (pl == nullptr) return TOPT_OK;
   (void)hipFree(pl->dtab);
   for (int g = 0; g < TOPT_NGRID; ++g) (void)hipFree(pl->dmap[g]);
-  (void)hipFree(pl->dstat);
   (void)hipFree(pl->dstats);
   for (int k = 0; k < 2; ++k) {
     if (pl->hstat[k]) (void)hipHostFree(pl->hstat[k]);
