@@ -345,11 +345,27 @@ __device__ bool tree_reduce(double (&v)[NV], double* part, unsigned* ctr, int lb
     __syncthreads();
     if (!s_go) return false;
     acquire_partials();
-    for (int k = wv; k < NV; k += nw) {
-      const bool mx = (maxmask >> k) & 1u;
-      double x = lane < cnt ? collect(&lvl[(first + lane) * NV + k]) : (mx ? -1.0e308 : 0.0);
-      x = mx ? wave_max(x) : wave_sum(x);
-      if (lane == 0) tsh[k] = x;
+    {
+      // wave wv folds components wv, wv + nw, ...: all of its loads are
+      // issued first (clamped addresses, the clamped values selected away),
+      // then the folds
+      constexpr int KW = (NV + nw - 1) / nw;
+      const int ln = lane < cnt ? lane : cnt - 1;
+      double xs[KW];
+#pragma unroll
+      for (int j = 0; j < KW; ++j) {
+        const int k = wv + j * nw < NV ? wv + j * nw : NV - 1;
+        xs[j] = collect(&lvl[(first + ln) * NV + k]);
+      }
+#pragma unroll
+      for (int j = 0; j < KW; ++j) {
+        const int k = wv + j * nw;
+        if (k >= NV) break;
+        const bool mx = (maxmask >> k) & 1u;
+        double x = lane < cnt ? xs[j] : (mx ? -1.0e308 : 0.0);
+        x = mx ? wave_max(x) : wave_sum(x);
+        if (lane == 0) tsh[k] = x;
+      }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
