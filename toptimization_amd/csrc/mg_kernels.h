@@ -14,8 +14,9 @@
 // (<= TOPT_MG_DENSE DOFs) is solved with a dense inverse formed once per
 // solve. Smoother: damped Jacobi (omega 0.65), one sweep before and one after
 // the coarse correction, so M is symmetric positive definite and plain PCG
-// applies. The iteration count depends little on the grid size (a 300x100
-// bridge after 20 OC iterations: 69 iterations against 1,738 with Jacobi).
+// applies. The iteration count depends little on the grid size (with
+// omega 0.6, a 300x100 bridge after 20 OC iterations took 69 iterations
+// against 1,738 with Jacobi; the 1200x400 bridge takes 15-60 per solve).
 #pragma once
 #include "fem_kernels.h"
 
