@@ -52,7 +52,9 @@ enum Grid : int { G_NODE = 0, G_ELEM = 1, G_FILT = 2, G_OC = 3, G_PCG = 4 };
 #define TOPT_NGRID 5
 #define TOPT_FILT_TY 64   // filter tile: 64 elements along y ...
 #define TOPT_FILT_TX 8    // ... by 8 along x, 2 per thread
+#ifndef TOPT_OC_PER_THREAD
 #define TOPT_OC_PER_THREAD 8
+#endif
 #define TOPT_OC_MAXD 6    // bisection levels evaluated per OC pass (<= 6)
 // nodes per thread in the PCG slot kernels (FP32 / FP64 vectors)
 #ifndef TOPT_NPT_F32
