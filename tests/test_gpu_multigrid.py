@@ -203,3 +203,24 @@ def test_mg_fp32_coarse_solve_near_null_support():
     assert all(s.cg_status == 1 for s in out)
     assert max(s.n_cg for s in out) <= 200
     assert np.isfinite([s.compliance for s in out]).all()
+
+
+@pytest.mark.parametrize("shape", [(10, 4), (7, 3), (2, 1)])
+@pytest.mark.parametrize("precision,tol,utol", [("fp64", 1e-10, 1e-7), ("mixed", 1e-8, 1e-5)])
+def test_mg_two_level_grids_match_oracle(shape, precision, tol, utol):
+    """Grids whose level 1 is already the coarsest (<= 128 DOFs): the fused
+    level-0 sweep writes the coarse right-hand side that the tail solves
+    densely, with no level in between."""
+    nelx, nely = shape
+    print(f"multigrid PCG {precision} {nelx}x{nely} (two levels)", flush=True)
+    prob = mbb(nelx, nely, preconditioner="multigrid", cg_tolerance=tol, max_cg_iterations=20000)
+    rho = rng_density(prob.n_elem, seed=nelx + nely)
+    u, it, status, c = _solve(prob, precision, rho)
+    Ke, edof = O.ke_q4(prob.E, prob.nu), O.edof_q4(nelx, nely)
+    fixed = O.fixed_mask(prob.n_dofs, prob.fixed_dofs)
+    f = O.force_vector(prob.n_dofs, prob.force_dofs, prob.force_values, prob.fixed_dofs)
+    u_ref, _ = O.pcg(f, O.scale(rho, prob.E_min, prob.penalty), Ke, edof, fixed,
+                     max_iter=200000, tol=1e-12, preconditioner="jacobi")
+    assert status == 1
+    assert np.abs(u - u_ref).max() <= utol * np.abs(u_ref).max()
+    assert c == pytest.approx(float(f @ u_ref), rel=10 * utol)

@@ -157,6 +157,29 @@ def test_pcg_fp64_matches_oracle(pc):
     plan.close()
 
 
+@pytest.mark.parametrize("pc", ["jacobi", "block_jacobi", "block_jacobi_2x2", "multigrid"])
+@pytest.mark.parametrize("shape", [(2, 1), (3, 2)])
+def test_pcg_fp64_few_iterations_matches_oracle(pc, shape):
+    """Solves of a few steps, where the last step's u += alpha p is a large part
+    of u: every schedule must apply it at the stop (the three-slot ones,
+    element-block Jacobi and multigrid, decide an FP64 stop in slot C)."""
+    from toptimization_amd.engine import DesignState, Plan
+    prob = mbb(*shape, preconditioner=pc, cg_tolerance=1e-12, max_cg_iterations=20000)
+    Ke, edof, fixed, rho, _, _ = _setup(prob, seed=3)
+    st = DesignState(prob, precision="fp64", rho0=rho)
+    st.rhof.copy_(st.rho)
+    plan = Plan([st])
+    it, status = plan.solve(warm=0, use_filtered=True)
+    f = O.force_vector(prob.n_dofs, prob.force_dofs, prob.force_values, prob.fixed_dofs)
+    u_ref, _ = O.pcg(f, O.scale(rho, prob.E_min, prob.penalty), Ke, edof, fixed,
+                     max_iter=20000, tol=1e-14, preconditioner="jacobi")
+    print(f"{pc} {shape}: {it[0]} CG", flush=True)
+    assert status[0] == 1
+    u = st.u.cpu().numpy()
+    assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
+    plan.close()
+
+
 def test_pcg_mixed_matches_fp64():
     from toptimization_amd.engine import DesignState, Plan
     prob = mbb(60, 20, preconditioner="block_jacobi_2x2", cg_tolerance=1e-8, max_cg_iterations=20000)

@@ -594,10 +594,10 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
       mg_vcycle<T>(pl, (int)(pl->q & 1), true, s);
       if (pl->kstruct)
         hipLaunchKernelGGL((k_slot_c_mg<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
-                           (int)(pl->q++ & 1));
+                           (int)(pl->q++ & 1), pa ^ 1);
       else
         hipLaunchKernelGGL((k_slot_c_mg<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
-                           (int)(pl->q++ & 1));
+                           (int)(pl->q++ & 1), pa ^ 1);
       continue;
     }
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,

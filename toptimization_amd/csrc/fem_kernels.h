@@ -574,8 +574,11 @@ __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)
 // slot A, which holds p at its nodes anyway (slot B then streams only r, Ap
 // and M). The flag upend marks it. Slot A applies it in A_CG mode and when
 // the solve has stopped; in A_REF mode neighbours read u, so the refresh
-// uses u + alpha p on the fly and B_REF applies the update. *upd (out): this
-// kernel must apply u += alpha p at its own nodes (alpha from the state).
+// uses u + alpha p on the fly and B_REF applies the update. In a three-slot
+// schedule (element-block Jacobi, multigrid) an FP64 stop at a B_CG step
+// (convergence, iteration cap) is decided by slot C's advance; slot C then
+// applies the step's deferred update itself (no later slot A will). *upd (out): this kernel must apply u += alpha p at
+// its own nodes (alpha from the state).
 enum SlotKind : int { SK_A = 0, SK_B = 1, SK_C = 2 };
 template <typename T, int K>
 __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevParts& pp, double* sh,
@@ -628,7 +631,8 @@ __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevPa
     *upd = pend && !(run && s.op == OP_B_CG);
     s.upend = (run && s.op == OP_B_CG) ? 1 : 0;
   } else {
-    *upd = false;
+    *upd = pend && !run;
+    if (*upd) s.upend = 0;
   }
   if (lb == 0 && threadIdx.x == 0) P.cg[(q + 1) & 1] = s;
   return s;
@@ -889,6 +893,25 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
 // Slot C (element-block Jacobi and multigrid): z = M^-1 r by gather, partial
 // r.z (fem/preconditioner.py:47-108 apply_and_rz_2d); for multigrid the
 // V-cycle launched between slots B and C has written z, and slot C forms r.z.
+// Slot C at the stop decided by its own advance: the step's deferred
+// u += alpha p at the tile's nodes (p: the buffer the preceding slot A wrote,
+// par).
+template <typename T>
+__device__ __forceinline__ void slot_c_stop_update(PD& P, TileOrg o, double alpha, int par) {
+  const GLB vec2<T>* p = gp((const vec2<T>*)(par ? P.p1 : P.p0));
+  GLB d2v* u = gp((d2v*)P.u64);
+#pragma unroll
+  for (int j = 0; j < Tile<T>::NPT; ++j) {
+    const NodeIdx id = tile_node<T>(P, o, j);
+    if (!id.act) continue;
+    const vec2<T> pj = p[id.n];
+    d2v un = u[id.n];
+    un.x = fma(alpha, (double)pj.x, un.x);
+    un.y = fma(alpha, (double)pj.y, un.y);
+    u[id.n] = un;
+  }
+}
+
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int2* pmap,
                                                        ProbDev single, int q, int par) {
@@ -902,7 +925,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
   parts_fetch(P, q, pp);
   bool upd;
   const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
-  if (st.status != ST_RUN) return;
+  if (st.status != ST_RUN) {
+    if (upd) slot_c_stop_update<T>(P, o, st.alpha, par);
+    return;
+  }
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
   double v = 0.0;
 #pragma unroll

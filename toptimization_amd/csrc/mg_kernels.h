@@ -1017,7 +1017,7 @@ namespace topt {
 // Level-0 up sweep + slot C: z = x0 + P x_1 + omega D^-1 (r - K x'), z = 0
 // on fixed DOFs, partial r.z (x0 = omega D^-1 r, formed again from r).
 template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, const int2* pmap, int q) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, const int2* pmap, int q, int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
   __shared__ d2v hal[Tl::HN];
@@ -1049,7 +1049,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   bool upd;
   const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
-  if (st.status != ST_RUN) return;
+  if (st.status != ST_RUN) {
+    if (upd) slot_c_stop_update<T>(P, o, st.alpha, par);
+    return;
+  }
   tile_store<T>(S, hv, rv, hal, esc);
   __syncthreads();
   double ax[NPT], ay[NPT];
