@@ -23,7 +23,9 @@ def _solve(prob, precision, rho):
     return st.u.cpu().numpy(), it[0], status[0], c
 
 
-@pytest.mark.parametrize("shape", [(30, 12), (61, 23), (96, 32)])
+# (56, 56): coarsest level 7x7 elements, 128 DOFs, half-bandwidth 19 (the
+# largest the coarse factorization takes)
+@pytest.mark.parametrize("shape", [(30, 12), (61, 23), (96, 32), (56, 56)])
 @pytest.mark.parametrize("precision,tol,utol", [("fp64", 1e-10, 1e-7), ("mixed", 1e-8, 1e-5)])
 def test_mg_pcg_matches_oracle(shape, precision, tol, utol):
     nelx, nely = shape
