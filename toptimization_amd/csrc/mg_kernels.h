@@ -800,6 +800,21 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   const int lt = P.mg_lt;
   const int L = P.mg_nlev - 1;
   auto nodes = [&](int l) { return (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
+  // The element matrices that fit in skb are those of levels ls..L-1
+  // (coarsest first; a level that does not fit is followed by larger ones),
+  // level L-1 at offset 0. Every thread knows them, so the first phase (b of
+  // level lt, which starts every vector at offset 0, its 1/diag, and the
+  // copies into skb) needs no offset from thread 0.
+  int ls = L;
+  {
+    int k = 0;
+    while (ls > lt) {
+      const int ne = P.mg[ls - 1].nx * P.mg[ls - 1].ny * 36;
+      if (k + ne > TailK<T>::N) break;
+      k += ne;
+      --ls;
+    }
+  }
   if (threadIdx.x == 0) {
     int o = 0;
     for (int l = lt; l <= L; ++l) {
@@ -808,38 +823,40 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     }
     int k = 0;
     for (int l = L - 1; l >= lt; --l) {
-      const int ne = P.mg[l].nx * P.mg[l].ny * 36;
-      koff[l] = k + ne <= TailK<T>::N ? k : -1;
-      if (koff[l] >= 0) k += ne;
+      koff[l] = l >= ls ? k : -1;
+      if (l >= ls) k += P.mg[l].nx * P.mg[l].ny * 36;
     }
   }
-  __syncthreads();
-  // b of level lt, formed by the fused sweep of level lt-1; the element
-  // matrices that fit, in 16-byte pieces (a level's matrices start 256-byte
-  // aligned and span 36 ne entries, a whole number of pieces), each thread's
-  // loads issued before its stores
+  // b of level lt, formed by the fused sweep of level lt-1, and x_lt
   for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) {
     const vec2<T> bb = gp((const vec2<T>*)P.mg[lt].b)[n];
-    sb[off[lt] + n] = bb;
-    if (lt < L) sx[off[lt] + n] = mg_jac<T>(gp((const vec2<T>*)P.mg[lt].dinv)[n], bb);
+    sb[n] = bb;
+    if (lt < L) sx[n] = mg_jac<T>(gp((const vec2<T>*)P.mg[lt].dinv)[n], bb);
   }
-  for (int l = lt; l < L; ++l) {
-    if (koff[l] < 0) continue;
-    typedef unsigned Q __attribute__((ext_vector_type(4)));
-    constexpr int S = (TailK<T>::N * (int)sizeof(T) / 16 + TOPT_MG_TAIL_BLOCK - 1) / TOPT_MG_TAIL_BLOCK;
-    const GLB Q* src = gp((const Q*)P.mg[l].K);
-    Q* dst = (Q*)(skb + koff[l]);
-    const int n16 = P.mg[l].nx * P.mg[l].ny * 36 * (int)sizeof(T) / 16;
-    Q v[S];
+  // the element matrices that fit, in 16-byte pieces (a level's matrices
+  // start 256-byte aligned and span 36 ne entries, a whole number of pieces),
+  // each thread's loads issued before its stores
+  {
+    int k = 0;
+    for (int l = L - 1; l >= ls; --l) {
+      typedef unsigned Q __attribute__((ext_vector_type(4)));
+      constexpr int S = (TailK<T>::N * (int)sizeof(T) / 16 + TOPT_MG_TAIL_BLOCK - 1) / TOPT_MG_TAIL_BLOCK;
+      const GLB Q* src = gp((const Q*)P.mg[l].K);
+      Q* dst = (Q*)(skb + k);
+      const int ne36 = P.mg[l].nx * P.mg[l].ny * 36;
+      const int n16 = ne36 * (int)sizeof(T) / 16;
+      Q v[S];
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-      const int u = (int)threadIdx.x + k * TOPT_MG_TAIL_BLOCK;
-      v[k] = src[u < n16 ? u : n16 - 1];
-    }
+      for (int q = 0; q < S; ++q) {
+        const int u = (int)threadIdx.x + q * TOPT_MG_TAIL_BLOCK;
+        v[q] = src[u < n16 ? u : n16 - 1];
+      }
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-      const int u = (int)threadIdx.x + k * TOPT_MG_TAIL_BLOCK;
-      if (u < n16) dst[u] = v[k];
+      for (int q = 0; q < S; ++q) {
+        const int u = (int)threadIdx.x + q * TOPT_MG_TAIL_BLOCK;
+        if (u < n16) dst[u] = v[q];
+      }
+      k += ne36;
     }
   }
   __syncthreads();
