@@ -354,8 +354,11 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
 // enqueues whole chunks of iterations).
 __device__ __forceinline__ bool mg_stopped(PD& P, int par) {
   // the level-0 sweeps use the PCG tile map, which may hold problems of
-  // the plan with another three-slot preconditioner
-  return P.precond != PC_MG || cp((const CgState*)P.cg)[par].status != ST_RUN;
+  // the plan with another three-slot preconditioner. A slot B that confirmed
+  // a stop on the true residual (B_REF with a pending status) is followed by
+  // the slot C that ends the solve: its V-cycle would go unused.
+  const CST CgState& c = cp((const CgState*)P.cg)[par];
+  return P.precond != PC_MG || c.status != ST_RUN || (c.op == OP_B_REF && c.pending != 0);
 }
 // The V-cycle kernels cover a batch through (problem, local block) maps: one
 // map per level and kind, holding only the problems that take that step.
