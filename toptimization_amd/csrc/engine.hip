@@ -62,8 +62,10 @@ Geom geom(int nelx, int nely, int dtype = TOPT_F64) {
   g.nb[G_FILT] = (int)(cdiv(nelx, TOPT_FILT_TX) * cdiv(nely, TOPT_FILT_TY));
   g.nb[G_OC] = (int)cdiv(g.ne, int64_t(TOPT_BLOCK) * TOPT_OC_PER_THREAD);
   g.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float>(nelx, nely) : pcg_tiles<double>(nelx, nely));
-  g.ntree = (int)std::max<int64_t>(g.nb[G_NODE], std::max(pcg_tiles<float>(nelx, nely),
-                                                          pcg_tiles<double>(nelx, nely)));
+  // partials of the widest reduction: node blocks, PCG tiles, op-level matvec tiles
+  g.ntree = (int)std::max<int64_t>(
+      std::max<int64_t>(g.nb[G_NODE], std::max(pcg_tiles<float>(nelx, nely), pcg_tiles<double>(nelx, nely))),
+      std::max(pcg_tiles<float, NptMv<float>::v>(nelx, nely), pcg_tiles<double, NptMv<double>::v>(nelx, nely)));
   return g;
 }
 
@@ -1425,7 +1427,8 @@ int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, cons
   P.s = (void*)s;
   P.fix = fix;
   // one block per tile; the tree over those blocks (sized by geom().ntree)
-  P.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float>(P.nelx, P.nely) : pcg_tiles<double>(P.nelx, P.nely));
+  P.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float, NptMv<float>::v>(P.nelx, P.nely)
+                                         : pcg_tiles<double, NptMv<double>::v>(P.nelx, P.nely));
   const dim3 grid(P.nb[G_PCG]), blk(TOPT_BLOCK);
   if (dtype == TOPT_F32) {
     if (P.kstruct)

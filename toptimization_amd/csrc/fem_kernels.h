@@ -207,15 +207,16 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_cg_prepare(const ProbDev* tab, c
 // for the p buffer the state selects.
 //
 // Work decomposition: one block = one tile of TY = 64 node rows (one per
-// lane, contiguous in memory) by TX = 4 Npt node columns (wave w owns
-// columns w, w + 4, ...). The stencil kernels stage the tile's node halo
+// lane, contiguous in memory) by TX = 4 N node columns (wave w owns
+// columns w, w + 4, ...; N = Npt for the PCG kernels, NptMv for the
+// op-level matvec). The stencil kernels stage the tile's node halo
 // ((TX + 2) x 66 values) and element halo ((TX + 1) x 65 scales) in LDS:
 // every value is fetched from L2/HBM once per tile instead of once per
 // neighbour, and all of a thread's fetches are in flight together.
 
-template <typename T>
+template <typename T, int N = Npt<T>::v>
 struct Tile {
-  static constexpr int NPT = Npt<T>::v;
+  static constexpr int NPT = N;
   static constexpr int NW = TOPT_BLOCK / TOPT_WAVE;        // waves per block
   static constexpr int TY = TOPT_WAVE, TX = NW * NPT;      // tile: rows x columns
   static constexpr int HY = TY + 2, HN = (TX + 2) * HY;    // node halo
@@ -225,9 +226,9 @@ struct Tile {
 };
 
 // Tiles of a grid (the slot kernels' block count per problem).
-template <typename T>
+template <typename T, int N = Npt<T>::v>
 __host__ __device__ inline long pcg_tiles(int nelx, int nely) {
-  return (long)((nely + TOPT_WAVE) / TOPT_WAVE) * ((nelx + Tile<T>::TX) / Tile<T>::TX);
+  return (long)((nely + TOPT_WAVE) / TOPT_WAVE) * ((nelx + Tile<T, N>::TX) / Tile<T, N>::TX);
 }
 
 struct TileOrg {
@@ -235,19 +236,19 @@ struct TileOrg {
 };
 
 // Tiles are numbered along y first (the memory order of the nodes).
-template <typename T>
+template <typename T, int N = Npt<T>::v>
 __device__ __forceinline__ TileOrg tile_org(PD& P, int lb) {
   const int nty = (P.nely + TOPT_WAVE) / TOPT_WAVE;
   const int tx = lb / nty;
-  return TileOrg{tx * Tile<T>::TX, (lb - tx * nty) * TOPT_WAVE};
+  return TileOrg{tx * Tile<T, N>::TX, (lb - tx * nty) * TOPT_WAVE};
 }
 
 // Tile node j of this thread; nodes outside the grid are clamped onto it and
 // marked inactive (they store nothing and add nothing).
-template <typename T>
+template <typename T, int N = Npt<T>::v>
 __device__ __forceinline__ NodeIdx tile_node(PD& P, TileOrg o, int j) {
   NodeIdx id;
-  const int ix = o.x0 + (int)(threadIdx.x >> 6) + Tile<T>::NW * j;
+  const int ix = o.x0 + (int)(threadIdx.x >> 6) + Tile<T, N>::NW * j;
   const int iy = o.y0 + (int)(threadIdx.x & 63);
   id.act = ix <= P.nelx && iy <= P.nely;
   id.ix = ix <= P.nelx ? ix : P.nelx;
@@ -263,25 +264,25 @@ __device__ __forceinline__ NodeIdx tile_node(PD& P, TileOrg o, int j) {
 // scales likewise: own entries at (w + 4j + 1, lane + 1) and a ring of
 // Ring::NE entries (column 0 and row 0). Every load is issued before the
 // first wait, with simple index arithmetic.
-template <typename T>
+template <typename T, int N>
 struct Ring {
-  static constexpr int NN = 2 * Tile<T>::HY + 2 * Tile<T>::TX;   // node ring entries
-  static constexpr int NE = Tile<T>::EY + Tile<T>::TX;           // element ring entries
+  static constexpr int NN = 2 * Tile<T, N>::HY + 2 * Tile<T, N>::TX;   // node ring entries
+  static constexpr int NE = Tile<T, N>::EY + Tile<T, N>::TX;           // element ring entries
 };
 
 // Ring node entry t (< Ring::NN) -> halo position (cx, cy).
-template <typename T>
+template <typename T, int N>
 __device__ __forceinline__ void ring_node_pos(int t, int& cx, int& cy) {
-  constexpr int HY = Tile<T>::HY, TX = Tile<T>::TX;
+  constexpr int HY = Tile<T, N>::HY, TX = Tile<T, N>::TX;
   if (t < HY) { cx = 0; cy = t; }
   else if (t < 2 * HY) { cx = TX + 1; cy = t - HY; }
   else if (t < 2 * HY + TX) { cx = t - 2 * HY + 1; cy = 0; }
   else { cx = t - 2 * HY - TX + 1; cy = HY - 1; }
 }
 // Ring element entry t (< Ring::NE) -> element-tile position (cx, cy).
-template <typename T>
+template <typename T, int N>
 __device__ __forceinline__ void ring_elem_pos(int t, int& cx, int& cy) {
-  constexpr int EY = Tile<T>::EY;
+  constexpr int EY = Tile<T, N>::EY;
   if (t < EY) { cx = 0; cy = t; }
   else { cx = t - EY + 1; cy = 0; }
 }
@@ -306,11 +307,11 @@ __device__ __forceinline__ T halo_scale_at(PD& P, const GLB T* s, TileOrg o, int
 
 // State-independent geometry of a thread's share of a tile: its nodes, their
 // fixed bits, the ring node, and the element scales (own + ring), fetched.
-template <typename T>
+template <typename T, int N = Npt<T>::v>
 struct TileShare {
-  NodeIdx ids[Tile<T>::NPT];
-  unsigned fb[Tile<T>::NPT];
-  T se[Tile<T>::NPT];
+  NodeIdx ids[N];
+  unsigned fb[N];
+  T se[N];
   long rn;          // ring node (clamped; valid for every thread)
   int rcx, rcy;     // its halo position
   int ecx, ecy;     // ring element position
@@ -319,9 +320,9 @@ struct TileShare {
 };
 // INNER: the tile and its halo lie inside the grid (no clamping, every
 // node active); see tile_inner.
-template <typename T, bool INNER>
-__device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T>& S) {
-  using Tl = Tile<T>;
+template <typename T, int N, bool INNER>
+__device__ __forceinline__ void tile_share_at(PD& P, TileOrg o, TileShare<T, N>& S) {
+  using Tl = Tile<T, N>;
   const GLB T* s = gp((const T*)P.s);
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int nny = P.nely + 1;
@@ -335,15 +336,15 @@ __device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T>& S) {
       S.ids[j].n = (long)ix * nny + iy;
       S.se[j] = s[(long)ix * P.nely + iy];
     } else {
-      S.ids[j] = tile_node<T>(P, o, j);
+      S.ids[j] = tile_node<T, N>(P, o, j);
       S.se[j] = halo_scale_at<T>(P, s, o, w + Tl::NW * j + 1, lane + 1);
     }
     S.fb[j] = fixbits(P.fix, S.ids[j].n);
   }
-  S.hasr = (int)threadIdx.x < Ring<T>::NN;
-  ring_node_pos<T>(S.hasr ? (int)threadIdx.x : 0, S.rcx, S.rcy);
-  S.hase = (int)threadIdx.x < Ring<T>::NE;
-  ring_elem_pos<T>(S.hase ? (int)threadIdx.x : 0, S.ecx, S.ecy);
+  S.hasr = (int)threadIdx.x < Ring<T, N>::NN;
+  ring_node_pos<T, N>(S.hasr ? (int)threadIdx.x : 0, S.rcx, S.rcy);
+  S.hase = (int)threadIdx.x < Ring<T, N>::NE;
+  ring_elem_pos<T, N>(S.hase ? (int)threadIdx.x : 0, S.ecx, S.ecy);
   if constexpr (INNER) {
     S.rn = (long)(o.x0 - 1 + S.rcx) * nny + (o.y0 - 1 + S.rcy);
     S.rse = s[(long)(o.x0 - 1 + S.ecx) * P.nely + (o.y0 - 1 + S.ecy)];
@@ -353,21 +354,22 @@ __device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T>& S) {
   }
 }
 // True when every halo node and element of the tile lies inside the grid.
-template <typename T>
+template <typename T, int N>
 __device__ __forceinline__ bool tile_inner(PD& P, TileOrg o) {
-  return o.x0 >= 1 && o.y0 >= 1 && o.x0 + Tile<T>::TX <= P.nelx && o.y0 + Tile<T>::TY <= P.nely;
+  return o.x0 >= 1 && o.y0 >= 1 && o.x0 + Tile<T, N>::TX <= P.nelx && o.y0 + Tile<T, N>::TY <= P.nely;
 }
-template <typename T>
-__device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T>& S) {
-  if (tile_inner<T>(P, o)) tile_share<T, true>(P, o, S);
-  else tile_share<T, false>(P, o, S);
+// N is deduced from S.
+template <typename T, int N>
+__device__ __forceinline__ void tile_share(PD& P, TileOrg o, TileShare<T, N>& S) {
+  if (tile_inner<T, N>(P, o)) tile_share_at<T, N, true>(P, o, S);
+  else tile_share_at<T, N, false>(P, o, S);
 }
 
 // Write the tile's halo values (own + ring, FP64) and element scales to LDS.
-template <typename T>
-__device__ __forceinline__ void tile_store(const TileShare<T>& S, const d2v (&own)[Tile<T>::NPT], d2v ring,
+template <typename T, int N>
+__device__ __forceinline__ void tile_store(const TileShare<T, N>& S, const d2v (&own)[N], d2v ring,
                                            d2v* hal, double* esc) {
-  using Tl = Tile<T>;
+  using Tl = Tile<T, N>;
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
 #pragma unroll
   for (int j = 0; j < Tl::NPT; ++j) {
@@ -383,10 +385,10 @@ __device__ __forceinline__ void tile_store(const TileShare<T>& S, const d2v (&ow
 // scales held in FP64): FP64 Ke, FP64 accumulation. Element-outer order: the
 // two Ke rows of our local node in element (a, b) (16 scalars in SGPRs) serve
 // all NPT nodes before the next element's rows are loaded.
-template <typename T>
+template <typename T, int N>
 __device__ __forceinline__ void tile_stencil(PD& P, const d2v* hal, const double* esc,
-                                             double (&ax)[Tile<T>::NPT], double (&ay)[Tile<T>::NPT]) {
-  using Tl = Tile<T>;
+                                             double (&ax)[N], double (&ay)[N]) {
+  using Tl = Tile<T, N>;
   constexpr int CX[4] = {0, 1, 1, 0};
   constexpr int CY[4] = {0, 0, 1, 1};
   const int w = (int)(threadIdx.x >> 6), row = (int)(threadIdx.x & 63);
@@ -422,10 +424,10 @@ __device__ __forceinline__ void tile_stencil(PD& P, const d2v* hal, const double
 
 // K(s) q at the thread's NPT tile nodes, structured Ke (six values in
 // scalar registers), node-outer: each neighbour value is read from LDS once.
-template <typename T>
+template <typename T, int N>
 __device__ __forceinline__ void tile_stencil_s(PD& P, const d2v* hal, const double* esc,
-                                               double (&ax)[Tile<T>::NPT], double (&ay)[Tile<T>::NPT]) {
-  using Tl = Tile<T>;
+                                               double (&ax)[N], double (&ay)[N]) {
+  using Tl = Tile<T, N>;
   const int w = (int)(threadIdx.x >> 6), row = (int)(threadIdx.x & 63);
   double k[6];
 #pragma unroll
@@ -447,11 +449,12 @@ __device__ __forceinline__ void tile_stencil_s(PD& P, const d2v* hal, const doub
   }
 }
 
-template <typename T, bool KST>
-__device__ __forceinline__ void tile_apply(PD& P, const d2v* hal, const double* esc,
-                                           double (&ax)[Tile<T>::NPT], double (&ay)[Tile<T>::NPT]) {
-  if constexpr (KST) tile_stencil_s<T>(P, hal, esc, ax, ay);
-  else tile_stencil<T>(P, hal, esc, ax, ay);
+// N is deduced from the result arrays.
+template <typename T, bool KST, int N>
+__device__ __forceinline__ void tile_apply(PD& P, const d2v* hal, const double* esc, double (&ax)[N],
+                                           double (&ay)[N]) {
+  if constexpr (KST) tile_stencil_s<T, N>(P, hal, esc, ax, ay);
+  else tile_stencil<T, N>(P, hal, esc, ax, ay);
 }
 
 // Previous slot's partials, fetched into registers before the state is read.
@@ -992,21 +995,22 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_sens(const ProbDev* tab, const i
 
 // Op-level K(s) p with the reference BC rule (fixed rows -> identity), and
 // optionally p.Ap into out[0] (fem/kernels.py:143-176, fused_kernels.py:21-56).
-// Same tiles and LDS staging as PCG slot A; P.nb[G_PCG] = number of tiles.
+// The LDS staging of PCG slot A on tiles NptMv<T> nodes per thread wide;
+// P.nb[G_PCG] = number of those tiles.
 template <typename T, bool KST>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* __restrict__ pin,
                                                        T* __restrict__ Ap, double* out) {
-  using Tl = Tile<T>;
-  constexpr int NPT = Tl::NPT;
+  constexpr int NPT = NptMv<T>::v;
+  using Tl = Tile<T, NPT>;
   PD& P = get_prob<false>(nullptr, nullptr, single, 0);
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   __shared__ double sh[16];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const TileOrg o = tile_org<T>(P, lb);
+  const TileOrg o = tile_org<T, NPT>(P, lb);
   const GLB vec2<T>* p = gp((const vec2<T>*)pin);
-  TileShare<T> S;
-  tile_share<T>(P, o, S);
+  TileShare<T, NPT> S;
+  tile_share<T, NPT>(P, o, S);
   vec2<T> op[NPT];
 #pragma unroll
   for (int j = 0; j < NPT; ++j) op[j] = p[S.ids[j].n];
@@ -1014,10 +1018,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* 
   d2v hv[NPT];
 #pragma unroll
   for (int j = 0; j < NPT; ++j) hv[j] = d2v{(double)op[j].x, (double)op[j].y};
-  tile_store<T>(S, hv, d2v{(double)rp.x, (double)rp.y}, hal, esc);
+  tile_store<T, NPT>(S, hv, d2v{(double)rp.x, (double)rp.y}, hal, esc);
   __syncthreads();
   double ax[NPT], ay[NPT];
-  tile_apply<T, KST>(P, hal, esc, ax, ay);
+  tile_apply<T, KST, NPT>(P, hal, esc, ax, ay);
   double v[1] = {0.0};
   GLB vec2<T>* ap = gp((vec2<T>*)Ap);
   vec2<T> ov[NPT];

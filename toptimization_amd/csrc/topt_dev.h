@@ -56,11 +56,23 @@ enum Grid : int { G_NODE = 0, G_ELEM = 1, G_FILT = 2, G_OC = 3, G_PCG = 4 };
 #define TOPT_OC_PER_THREAD 8
 #endif
 #define TOPT_OC_MAXD 6    // bisection levels evaluated per OC pass (<= 6)
-// nodes per thread in the PCG slot kernels (FP32 / FP64 vectors)
+// nodes per thread in the PCG slot kernels (FP32 / FP64 vectors). Two in
+// FP32 as in FP64: 8-column tiles give small grids twice the workgroups (the
+// 300x100 MBB runs 76 per slot launch instead of 38, 8 % faster) and leave
+// the 1200x400 bridge within 1 %.
 #ifndef TOPT_NPT_F32
-#define TOPT_NPT_F32 4
+#define TOPT_NPT_F32 2
 #endif
 template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? TOPT_NPT_F32 : 2; };
+// nodes per thread in the op-level matvec (FP32 / FP64): at 8192x4096 the
+// 16-column FP32 tiles re-read less halo than 8-column ones
+#ifndef TOPT_NPT_MV_F32
+#define TOPT_NPT_MV_F32 4
+#endif
+#ifndef TOPT_NPT_MV_F64
+#define TOPT_NPT_MV_F64 2
+#endif
+template <typename T> struct NptMv { static constexpr int v = sizeof(T) == 4 ? TOPT_NPT_MV_F32 : TOPT_NPT_MV_F64; };
 
 // One level of the geometric multigrid hierarchy (PC_MG). Level 0 is the
 // design grid; level l+1 has ceil(nx/2) x ceil(ny/2) elements, each the
