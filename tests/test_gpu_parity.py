@@ -32,7 +32,9 @@ def _cuda(a, dtype=torch.float64):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
 
 
-@pytest.mark.parametrize("shape", [(8, 4), (60, 20), (33, 17)])
+# (70, 130): three tiles of node rows and interior tiles (halo inside the
+# grid, the unclamped staging path) at both tile widths
+@pytest.mark.parametrize("shape", [(8, 4), (60, 20), (33, 17), (70, 130)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_matvec_matches_oracle(shape, dtype):
     from toptimization_amd import ops
