@@ -182,6 +182,28 @@ def test_pcg_fp64_few_iterations_matches_oracle(pc, shape):
     plan.close()
 
 
+@pytest.mark.parametrize("pc", ["jacobi", "block_jacobi", "block_jacobi_2x2"])
+def test_pcg_fp64_more_than_2048_tiles_matches_oracle(pc):
+    """A 16400x8 grid has 2051 PCG tiles, so every slot sums the previous
+    slot's block partials in its loop beyond the 2048 register-held ones.
+    Eight CG steps (the iteration cap) against the oracle's eight."""
+    from toptimization_amd.engine import DesignState, Plan
+    prob = mbb(16400, 8, preconditioner=pc, cg_tolerance=1e-14, max_cg_iterations=8)
+    Ke, edof, fixed, rho, _, _ = _setup(prob, seed=4)
+    st = DesignState(prob, precision="fp64", rho0=rho)
+    st.rhof.copy_(st.rho)
+    plan = Plan([st])
+    it, status = plan.solve(warm=0, use_filtered=True)
+    f = O.force_vector(prob.n_dofs, prob.force_dofs, prob.force_values, prob.fixed_dofs)
+    u_ref, n_ref = O.pcg(f, O.scale(rho, prob.E_min, prob.penalty), Ke, edof, fixed,
+                         max_iter=8, tol=1e-14, preconditioner=pc)
+    print(f"{pc}: {it[0]} CG, status {status[0]}", flush=True)
+    assert it[0] == n_ref == 8 and status[0] == 3   # stopped by the cap
+    u = st.u.cpu().numpy()
+    assert np.abs(u - u_ref).max() <= 1e-10 * np.abs(u_ref).max()
+    plan.close()
+
+
 def test_pcg_mixed_matches_fp64():
     from toptimization_amd.engine import DesignState, Plan
     prob = mbb(60, 20, preconditioner="block_jacobi_2x2", cg_tolerance=1e-8, max_cg_iterations=20000)
