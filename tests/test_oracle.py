@@ -254,3 +254,87 @@ def test_simp_loop_small_cantilever_behaves():
     assert h["c"][5] < h["c"][0]
     assert abs(h["vol"][-1] - 0.5) < 5e-3
     assert all(n > 0 for n in h["n_cg"])
+
+
+# ---------------------------------------------------------------------------
+# OC candidate rule by hand (optimizer/oc.py:40-47): values worked out on
+# paper, so a dropped move limit, eta = 1 instead of 1/2, a lower bound of 0
+# instead of 0.001, or a missing max(., 0) fails.
+# ---------------------------------------------------------------------------
+
+
+def test_oc_candidate_hand_computed():
+    rho = np.array([0.5, 0.5, 0.5, 0.5, 0.5, 0.05, 0.95, 0.5, 0.5, 0.002])
+    dc = np.array([-4.0, -1.0, -0.25, -1.21, -0.81, -1e-8, -4.0, 0.3, -2.0, -1.0])
+    dv = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0])
+    got = O.oc_candidate(rho, dc, dv, 1.0, 0.2)
+    want = [0.7,     # 0.5*sqrt(4) = 1.0, clipped to rho + move
+            0.5,     # 0.5*sqrt(1)
+            0.3,     # 0.5*sqrt(0.25) = 0.25, clipped to rho - move
+            0.55,    # 0.5*sqrt(1.21): eta = 1/2 (eta = 1 would give 0.605)
+            0.45,    # 0.5*sqrt(0.81) (eta = 1: 0.405)
+            0.001,   # 0.05*1e-4, clipped to rho_min = 0.001 (not 0, not rho - move < 0)
+            1.0,     # 0.95*2 = 1.9, clipped to min(1, rho + move) = 1
+            0.3,     # dc > 0: B = max(-dc/lam, 0) = 0 -> 0, clipped to rho - move
+            0.5,     # dv = 2: B = 2 / (1*2) = 1
+            0.002]   # 0.002*1 inside [max(0.001, -0.198), 0.202]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+    # lambda = 0: B = -dc / 1e-20 is huge -> upper bound
+    np.testing.assert_allclose(O.oc_candidate(rho[:3], dc[:3], dv[:3], 0.0, 0.2), [0.7, 0.7, 0.7], atol=0)
+    # the move limit holds for any lambda
+    rng = np.random.default_rng(11)
+    r = rng.uniform(0.001, 1.0, 400)
+    d = -rng.uniform(0.0, 5.0, 400)
+    for lam in (1e-6, 0.3, 1.0, 7.0, 1e6):
+        c = O.oc_candidate(r, d, np.ones(400), lam, 0.2)
+        assert np.all(np.abs(c - r) <= 0.2 + 1e-15)
+        assert np.all(c >= 0.001) and np.all(c <= 1.0)
+
+
+@pytest.mark.parametrize("lam_star,steps", [(100.0, 36), (1.0, 43), (0.1, 46), (0.01, 49), (1e-3, 50)])
+def test_oc_bisection_closed_form(lam_star, steps):
+    """Uniform rho = 0.5, dc = -lam_star, volfrac 0.5: V(lam) = clip(0.5
+    sqrt(lam_star / lam), 0.3, 0.7), so the root is lam = lam_star. The step
+    counts follow from the bracket [0, 1e9] halved until (hi - lo) / (hi +
+    lo) < 1e-4, with the 50-step cap (worked out by hand: 36 steps for a
+    root at 100, 43 at 1, 46 at 0.1, 49 at 0.01, the cap below ~4.4e-3)."""
+    n = 64
+    rho = np.full(n, 0.5)
+    dc = np.full(n, -lam_star)
+    new, change, trace = O.oc_update(rho, dc, np.ones(n), 0.5, 0.2, return_trace=True)
+    assert len(trace) == steps
+    lam = trace[-1][0]
+    if steps < 50:
+        assert abs(lam - lam_star) / lam_star < 2e-4
+        np.testing.assert_allclose(new, 0.5 * np.sqrt(lam_star / lam), rtol=1e-15)
+        assert abs(new.mean() - 0.5) < 1e-4
+    assert change == pytest.approx(np.abs(new - 0.5).max(), abs=0)
+    assert trace[0][0] == 5e8 and trace[1][0] == 2.5e8   # first midpoints of [0, 1e9]
+
+
+# ---------------------------------------------------------------------------
+# The C restatement (oracle/csrc, checker of the large-grid GPU tests) held to
+# the NumPy restatement on whole SIMP loops, both filter modes.
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("case", ["cantilever60x20", "mbb40x16"])
+@pytest.mark.parametrize("filter_type", ["density", "sensitivity"])
+def test_c_oracle_matches_numpy_oracle(case, filter_type):
+    from oracle import c_oracle
+    from helpers import oracle_args
+    c_oracle.set_threads(4)
+    kw = dict(volume_fraction=0.5, filter_radius=1.5 if case.startswith("cant") else 2.0,
+              filter_type=filter_type, max_iterations=20, tolerance=0.0, cg_tolerance=1e-10,
+              max_cg_iterations=20000, preconditioner="jacobi")
+    prob = cantilever(60, 20, **kw) if case.startswith("cant") else mbb(40, 16, **kw)
+    a, k = oracle_args(prob)
+    ref = O.simp_run(*a, **k)
+    got = c_oracle.simp(prob, "f64")
+    assert got["iterations"] == 20 == len(ref["c"])
+    c_ref, c_got = np.array(ref["c"]), np.array(got["c"])
+    assert np.max(np.abs(c_got - c_ref) / np.abs(c_ref)) <= 1e-10
+    assert np.max(np.abs(got["rho"] - ref["rho"])) <= 1e-8
+    np.testing.assert_allclose(got["vol"], ref["vol"], rtol=1e-10)
+    # same PCG iteration counts, allowing a one-step shift at a stopping boundary
+    assert np.max(np.abs(np.array(got["n_cg"]) - np.array(ref["n_cg"]))) <= 1
