@@ -108,3 +108,24 @@ def test_gradient_central_differences_bridge_1200x400(precision):
         dc_fd = -np.sum((s(rhof[sup] + h * d) - s(rhof[sup] - h * d)) * uKu) / (2 * h)
         assert g[j] == pytest.approx(dc_fd, rel=1e-6), j
     model.close()
+
+
+def test_gradient_keeps_the_shape_of_rho():
+    """rho as (nelx, nely), the shape simp.run returns, and in float32: the
+    gradient comes back in rho's shape and dtype and equals the flat one."""
+    from toptimization_amd.autograd import TopologyModel
+    prob = cantilever(24, 10, filter_radius=1.5, cg_tolerance=1e-10, max_cg_iterations=50000)
+    model = TopologyModel(prob, precision="fp64")
+    flat = torch.as_tensor(rng_density(prob.n_elem, 23, 0.2, 1.0), device="cuda")
+    r1 = flat.clone().requires_grad_(True)
+    model.compliance(r1).backward()
+    r2 = flat.clone().reshape(prob.nelx, prob.nely).requires_grad_(True)
+    model.compliance(r2).backward()
+    assert r2.grad.shape == (prob.nelx, prob.nely)
+    np.testing.assert_allclose(r2.grad.reshape(-1).cpu().numpy(), r1.grad.cpu().numpy(), rtol=1e-9)
+    r3 = flat.clone().float().reshape(prob.nelx, prob.nely).requires_grad_(True)
+    (2.0 * model.compliance(r3)).backward()
+    assert r3.grad.dtype == torch.float32 and r3.grad.shape == (prob.nelx, prob.nely)
+    np.testing.assert_allclose(r3.grad.reshape(-1).double().cpu().numpy(), 2.0 * r1.grad.cpu().numpy(),
+                               rtol=1e-5)
+    model.close()

@@ -75,3 +75,31 @@ def test_cli_serve_jsonl(tmp_path):
         assert rho.shape == (d["nelx"], d["nely"])
         assert d["iterations"] == 6
         assert abs(rho.mean() - 0.5) < 1e-3
+
+
+@pytest.mark.parametrize("mode", ["async", "lockstep"])
+def test_stream_mixing_plan_classes(mode):
+    """Requests naming their own preconditioner (three-slot multigrid,
+    two-slot Jacobi and 2x2 block Jacobi) and filter type in one stream: the
+    server runs one plan per class, and every design equals its solo run."""
+    from toptimization_amd.engine import DesignState, Plan
+    from toptimization_amd.serving import DesignServer, plan_class, synthetic_stream
+    reqs = synthetic_stream(n=6, rate=100.0, seed=4, nelx_range=(32, 80), max_iter=8, tol=0.01,
+                            cg_tolerance=1e-6, max_cg_iterations=20000, precision="mixed")
+    pcs = ["multigrid", "jacobi", "block_jacobi_2x2", "multigrid", "jacobi", "multigrid"]
+    for r, pc in zip(reqs, pcs):
+        r.problem.preconditioner = pc
+    reqs[3].problem.filter_type = "sensitivity"
+    assert len({plan_class(r.problem) for r in reqs}) == 3
+    server = DesignServer(max_active=4, hbm_budget=1 << 30, precision="mixed", keep_rho=True, mode=mode)
+    stats = server.run(reqs, clock="instant")
+    assert stats.summary()["designs"] == 6
+    for res in stats.results:
+        prob = reqs[res.rid].problem
+        st = DesignState(prob, precision="mixed")
+        plan = Plan([st])
+        for _ in range(res.iterations):
+            out = plan.iteration(sensitivity_filter=prob.filter_type == "sensitivity")[0]
+        plan.close()
+        assert (out.compliance, out.volume, out.change) == (res.compliance, res.volume, res.change), res.rid
+        np.testing.assert_array_equal(st.rho.cpu().numpy(), res.rho)

@@ -60,3 +60,20 @@ def test_synthetic_stream_is_seeded():
     assert [(r.problem.nelx, r.problem.nely, r.arrival) for r in a] == \
         [(r.problem.nelx, r.problem.nely, r.arrival) for r in b]
     assert all(r.problem.nelx % 16 == 0 and 64 <= r.problem.nelx <= 768 for r in a)
+
+
+def test_plan_class_groups_by_slot_count_and_filter():
+    from toptimization_amd.serving import plan_class
+    kw = dict(fixed=[{"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x", "y"]}],
+              loads=[{"nodes": {"type": "point", "location": [20, 5]}, "force": [0.0, -1.0]}])
+    keys = {}
+    for pc in ("none", "jacobi", "block_jacobi", "block_jacobi_2x2", "multigrid"):
+        for ft in ("density", "sensitivity"):
+            keys[pc, ft] = plan_class(make_problem(20, 10, preconditioner=pc, filter_type=ft, **kw))
+    assert keys["jacobi", "density"] == keys["none", "density"] == keys["block_jacobi_2x2", "density"]
+    assert keys["multigrid", "density"] == keys["block_jacobi", "density"] != keys["jacobi", "density"]
+    assert keys["jacobi", "density"] != keys["jacobi", "sensitivity"]
+    assert len(set(keys.values())) == 4
+    # a server-wide preconditioner overrides the request's
+    p = make_problem(20, 10, preconditioner="jacobi", **kw)
+    assert plan_class(p, "multigrid") == keys["multigrid", "density"]

@@ -79,6 +79,7 @@ class ComplianceFunction(torch.autograd.Function):
     def forward(ctx, rho: torch.Tensor, model: TopologyModel) -> torch.Tensor:
         c, dc_phys = model.forward_raw(rho)
         ctx.model = model
+        ctx.shape, ctx.dtype = rho.shape, rho.dtype
         ctx.save_for_backward(dc_phys)
         return torch.tensor(c, dtype=torch.float64, device=rho.device)
 
@@ -86,4 +87,6 @@ class ComplianceFunction(torch.autograd.Function):
     def backward(ctx, grad_c: torch.Tensor):
         (dc_phys,) = ctx.saved_tensors
         g = ctx.model.chain_rule(dc_phys)
-        return (grad_c * g).reshape(ctx.model.problem.n_elem), None
+        # the gradient has rho's shape and dtype (flat, or (nelx, nely) as
+        # simp.run returns it)
+        return (grad_c * g).reshape(ctx.shape).to(ctx.dtype), None

@@ -32,6 +32,9 @@ KE_ALPHA_REG = 0.0
 NELX = 0
 NELY = 0
 KE_NP = None
+# incremented by every allocate(): device states built from the previous
+# fields (fem/solver.py's cache) are discarded when it changes
+GENERATION = 0
 
 
 def allocate(n_dofs: int, n_elem: int, dpe: int, max_neighbors: int, Ke_np: np.ndarray,
@@ -41,6 +44,7 @@ def allocate(n_dofs: int, n_elem: int, dpe: int, max_neighbors: int, Ke_np: np.n
     global u, f, r, p, Ap, z, diag, u_prev, rho, rho_new, dc, dc_filt, dv, rho_filt
     global Ke, Ke_inv, edof, is_fixed, diag_count, filt_neighbors, filt_weights, filt_n_nb
     global _dot_result, _max_result, FIXMASK, N_DOFS, N_ELEM, DPE, KE_ALPHA_REG, NELX, NELY, KE_NP
+    global GENERATION
     if dpe != 8:
         raise NotImplementedError("the GPU path covers 2-D Q4 problems (8 DOFs per element) only")
     dev = require_gpu()
@@ -70,6 +74,7 @@ def allocate(n_dofs: int, n_elem: int, dpe: int, max_neighbors: int, Ke_np: np.n
     np.add.at(f_np, np.asarray(force_dofs, dtype=np.int64), np.asarray(force_values, dtype=np.float64))
     f_np[fixed] = 0.0
     f.copy_(torch.from_numpy(f_np))
+    GENERATION += 1
 
 
 def grid() -> tuple[int, int]:

@@ -5,7 +5,7 @@ from __future__ import annotations
 
 from .. import ops
 from . import fields as F
-from .kernels import _dim2, _fixmask, _grid
+from .kernels import _dim2, _fixmask, _grid, _scale
 
 
 def matvec_and_dot(p_in, Ap_out, rho, Ke, edof, is_fixed, E_min: float, penalty: float,
@@ -13,7 +13,7 @@ def matvec_and_dot(p_in, Ap_out, rho, Ke, edof, is_fixed, E_min: float, penalty:
     """Ap = K(rho) p; returns p.Ap (``fused_kernels.py:21-56``)."""
     _dim2(dim)
     g = _grid(Ke, E_min, penalty)
-    s = ops.scale(g, rho, p_in.dtype)
+    s = _scale(g, rho, p_in.dtype)
     _, dot = ops.matvec(g, p_in, s, _fixmask(is_fixed), Ap=Ap_out, with_dot=True)
     return float(dot.item())
 

@@ -35,6 +35,19 @@ def _grid(Ke, E_min: float, penalty: float, edof=None):
     return g
 
 
+_s_bufs: dict = {}
+
+
+def _scale(g, rho, dtype):
+    """SIMP scales of rho in a buffer reused across calls (one launch, no
+    allocation after the first call for a given length and dtype)."""
+    key = (rho.numel(), dtype, rho.device)
+    buf = _s_bufs.get(key)
+    if buf is None:
+        buf = _s_bufs[key] = torch.empty(rho.numel(), dtype=dtype, device=rho.device)
+    return ops.scale(g, rho, dtype, out=buf)
+
+
 def _fixmask(is_fixed):
     if is_fixed is F.is_fixed and F.FIXMASK is not None:
         return F.FIXMASK
@@ -81,7 +94,7 @@ def compute_diagonal(diag, rho, Ke, edof, is_fixed, E_min: float, penalty: float
     """diag K(rho), fixed -> 1, clamp 1e-20 (``kernels.py:76-140``)."""
     _dim2(dim)
     g = _grid(Ke, E_min, penalty)
-    s = ops.scale(g, rho, diag.dtype)
+    s = _scale(g, rho, diag.dtype)
     diag.copy_(ops.diagonal(g, s, _fixmask(is_fixed)))
 
 
@@ -90,7 +103,7 @@ def compute_matvec(p_in, Ap_out, rho, Ke, edof, is_fixed, E_min: float, penalty:
     """Ap = K(rho) p with identity rows at fixed DOFs (``kernels.py:143-231``)."""
     _dim2(dim)
     g = _grid(Ke, E_min, penalty)
-    s = ops.scale(g, rho, p_in.dtype)
+    s = _scale(g, rho, p_in.dtype)
     ops.matvec(g, p_in, s, _fixmask(is_fixed), Ap=Ap_out)
 
 

@@ -9,7 +9,7 @@ import torch
 
 from .. import ops
 from . import fields as F
-from .kernels import _dim2, _fixmask, _grid
+from .kernels import _dim2, _fixmask, _grid, _scale
 
 
 def setup(edof, diag_count, n_elem: int, dim: int) -> None:
@@ -29,7 +29,7 @@ def apply_and_rz(r, z, rho, Ke_inv, edof, diag_count, is_fixed, dot_result, E_mi
     Ke inverse is formed from fields.Ke inside libtopt (``fields.py:139-143``)."""
     _dim2(dim)
     g = _grid(F.Ke, E_min, penalty)
-    s = ops.scale(g, rho, r.dtype)
+    s = _scale(g, rho, r.dtype)
     zz, rz, rr = ops.elemblock_apply(g, r, s, _fixmask(is_fixed))
     z.copy_(zz)
     dot_result.view(-1)[0] = rr

@@ -16,19 +16,36 @@ from pathlib import Path
 
 import numpy as np
 
+from .. import ops
 from ..engine import DesignState, Plan
 from ..problem import PRECONDITIONERS, Problem
 from . import fields as F
 
+# device states of the current fields, keyed by the solver settings; all
+# of them are dropped when fields.allocate() runs again, and at most
+# _MAX_CACHED are kept (oldest evicted)
 _cache: dict = {}
+_cache_gen = -1
+_MAX_CACHED = 4
+
+
+def _drop(entry) -> None:
+    entry[1].close()
 
 
 def _state_for(E_min, penalty, max_iter, tol, preconditioner, precision):
-    key = (id(F.u), F.NELX, F.NELY, float(E_min), float(penalty), int(max_iter), float(tol),
-           preconditioner, precision)
+    global _cache_gen
+    if _cache_gen != F.GENERATION:
+        for entry in _cache.values():
+            _drop(entry)
+        _cache.clear()
+        _cache_gen = F.GENERATION
+    key = (float(E_min), float(penalty), int(max_iter), float(tol), preconditioner, precision)
     hit = _cache.get(key)
     if hit is not None:
-        return hit
+        return key, hit
+    while len(_cache) >= _MAX_CACHED:
+        _drop(_cache.pop(next(iter(_cache))))
     nelx, nely = F.grid()
     fixed = np.flatnonzero(F.is_fixed.cpu().numpy())
     f_np = F.f.double().cpu().numpy()
@@ -41,7 +58,7 @@ def _state_for(E_min, penalty, max_iter, tol, preconditioner, precision):
                    output_format="png", precision=precision)
     st = DesignState(prob, Ke=F.KE_NP, f=f_np)
     hit = _cache[key] = (st, Plan([st]))
-    return hit
+    return key, hit
 
 
 def solve(E_min: float, penalty: float, dim: int, max_iter: int = 2000, tol: float = 1e-8,
@@ -51,8 +68,17 @@ def solve(E_min: float, penalty: float, dim: int, max_iter: int = 2000, tol: flo
         raise NotImplementedError("3-D H8 solves are not on the GPU path")
     if preconditioner not in PRECONDITIONERS:
         raise ValueError(f"unknown preconditioner {preconditioner!r}")
-    st, plan = _state_for(E_min, penalty, max_iter, tol, preconditioner, precision)
+    key, (st, plan) = _state_for(E_min, penalty, max_iter, tol, preconditioner, precision)
     st.f.copy_(F.f)
+    # |f|^2 sets the stopping tolerance and the zero-load test; the caller
+    # may have changed f since the last call (solver.py:58-64 recomputes it
+    # every time), and a plan holds the value it was created with
+    fn2 = float(ops.blas1("dot", st.f.numel(), x=st.f, y=st.f, reduce=True)[0].item())
+    if fn2 != st.desc.fnorm2:
+        st.desc.fnorm2 = fn2
+        plan.close()
+        plan = Plan([st])
+        _cache[key] = (st, plan)
     st.rhof.copy_(F.rho)
     if warm_start:
         st.u.copy_(F.u_prev)

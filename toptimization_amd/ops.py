@@ -64,8 +64,11 @@ def fix_mask(n_dofs: int, fixed_dofs, device=None) -> torch.Tensor:
     return torch.from_numpy(fixed_bitmask(n_dofs, fixed_dofs).view(np.int32)).to(device or "cuda")
 
 
-def scale(g: TGrid, rho: torch.Tensor, dtype=torch.float64) -> torch.Tensor:
-    s = torch.empty(rho.numel(), dtype=dtype, device=rho.device)
+def scale(g: TGrid, rho: torch.Tensor, dtype=torch.float64, out: torch.Tensor | None = None) -> torch.Tensor:
+    """s_e = E_min + rho_e^p (1 - E_min) in dtype, into `out` when given."""
+    if out is not None and (out.dtype != dtype or out.numel() != rho.numel()):
+        raise ValueError("out must have rho's length and the requested dtype")
+    s = torch.empty(rho.numel(), dtype=dtype, device=rho.device) if out is None else out
     check(_lib.load().topt_scale(C.byref(g), _DT[dtype], _p(rho), _p(s), _s()), "topt_scale")
     return s
 

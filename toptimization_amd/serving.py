@@ -121,6 +121,19 @@ class ServerStats:
                 "engine_steps": self.steps, "max_active": self.max_active}
 
 
+# preconditioners whose PCG iteration has three slots (A, B, C); a plan
+# holds either these or the two-slot ones (engine.hip: topt_plan_create)
+THREE_SLOT = ("block_jacobi", "multigrid")
+
+
+def plan_class(problem: Problem, preconditioner: str | None = None) -> tuple[bool, bool]:
+    """Key of the designs that can share one plan: (three-slot PCG,
+    sensitivity filter). Designs of different classes are served by
+    separate plans, advanced in turn."""
+    pc = preconditioner or problem.preconditioner
+    return (pc in THREE_SLOT, problem.filter_type == "sensitivity")
+
+
 class DesignServer:
     """Admission by slot count and HBM budget.
 
@@ -130,13 +143,19 @@ class DesignServer:
     keep solving, and a finished design leaves at once.
     mode='lockstep': one SIMP iteration of every active design per engine
     step (each step waits for the slowest solve of the batch).
-    Both give bitwise the same designs."""
+    Both give bitwise the same designs.
+
+    Requests may name any preconditioner and filter type. Designs share a
+    plan within their class (``plan_class``); a stream that mixes classes
+    runs one plan per class, advanced in turn, with the same results."""
 
     def __init__(self, max_active: int = 64, hbm_budget: int = 32 << 30, precision: str = "mixed",
                  preconditioner: str | None = None, keep_rho: bool = False, mode: str = "async",
                  admit_chunks: int = 8):
         if mode not in ("async", "lockstep"):
             raise ValueError("mode must be 'async' or 'lockstep'")
+        if preconditioner is not None and preconditioner not in _lib.PRECONDITIONER_IDS:
+            raise ValueError(f"unknown preconditioner {preconditioner!r}")
         self.max_active = int(max_active)
         self.hbm_budget = int(hbm_budget)
         self.precision = precision
@@ -145,157 +164,112 @@ class DesignServer:
         self.mode = mode
         self.admit_chunks = int(admit_chunks)   # chunks per call while requests wait
 
-    def run(self, requests: list[Request], clock: str = "wall", progress=None) -> ServerStats:
-        """Serve the stream. clock='wall': arrivals are real time offsets from
-        the start; clock='instant': every request is available at t=0.
-        progress(msg) is called whenever a design finishes."""
-        if self.mode == "async":
-            return self._run_async(requests, clock, progress)
-        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        active: list[_Active] = []
-        stats = ServerStats()
-        plan: Plan | None = None
-        used = 0
-        t0 = time.perf_counter()
-        now = lambda: time.perf_counter() - t0  # noqa: E731
-        while pending or active:
-            t = now()
-            changed = False
-            # admission: arrived requests, in order, while slots and memory allow
-            while pending and len(active) < self.max_active:
-                r = pending[0]
-                if clock == "wall" and r.arrival > t:
-                    break
-                need = footprint_bytes(r.problem, self.precision, self.preconditioner)
-                if active and used + need > self.hbm_budget:
-                    break
-                pending.pop(0)
-                st = DesignState(r.problem, precision=self.precision, preconditioner=self.preconditioner)
-                active.append(_Active(r, st, t))
-                used += need
-                changed = True
-            if not active:
-                if pending and clock == "wall":
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
-                continue
-            if changed or plan is None:
-                if plan is not None:
-                    plan.close()
-                plan = Plan([a.state for a in active])
-            stats.max_active = max(stats.max_active, len(active))
-            sens = [a.req.problem.filter_type == "sensitivity" for a in active]
-            if any(sens) and not all(sens):
-                raise NotImplementedError("a batch mixes density and sensitivity filtering")
-            out = plan.iteration(sensitivity_filter=sens[0])
-            stats.steps += 1
-            t = now()
-            if progress is not None and stats.steps % 25 == 1:
-                progress(f"step {stats.steps} at {t:.1f} s: {len(active)} active, max CG "
-                         f"{max(o.n_cg for o in out)}, {len(stats.results)} finished, {len(pending)} waiting")
-            keep = []
-            for a, s in zip(active, out):
-                a.it += 1
-                a.n_cg += s.n_cg
-                pr = a.req.problem
-                if a.it >= pr.max_iter or (a.it > 5 and s.change < pr.tol):
-                    stats.results.append(Result(
-                        a.req.rid, a.req.arrival, a.admitted, t, a.it, s.compliance, s.volume, s.change,
-                        a.n_cg, a.state.rho.cpu().numpy() if self.keep_rho else None))
-                    used -= footprint_bytes(pr, self.precision, self.preconditioner)
-                    changed = True
-                    if progress is not None:
-                        r = stats.results[-1]
-                        progress(f"design {r.rid} ({pr.nelx}x{pr.nely}) done after {r.iterations} it, "
-                                 f"latency {r.latency:.2f} s, {len(stats.results)} finished, "
-                                 f"{len(pending)} waiting")
-                else:
-                    keep.append(a)
-            if len(keep) != len(active):
-                active = keep
-                if plan is not None:
-                    plan.close()
-                plan = None
-        if plan is not None:
-            plan.close()
-        stats.makespan = now()
-        stats.results.sort(key=lambda r: r.rid)
-        return stats
-
-
-    def _admit(self, pending, active, used, t, clock):
-        while pending and len(active) < self.max_active:
+    def _admit(self, pending, groups, used, t, clock) -> tuple[int, set]:
+        """Admit arrived requests in order while slots and memory allow;
+        returns the new memory use and the classes whose membership grew."""
+        touched = set()
+        n_active = sum(len(g) for g in groups.values())
+        while pending and n_active < self.max_active:
             r = pending[0]
             if clock == "wall" and r.arrival > t:
                 break
             need = footprint_bytes(r.problem, self.precision, self.preconditioner)
-            if active and used + need > self.hbm_budget:
+            if n_active and used + need > self.hbm_budget:
                 break
             pending.pop(0)
             st = DesignState(r.problem, precision=self.precision, preconditioner=self.preconditioner)
-            active.append(_Active(r, st, t))
+            key = plan_class(r.problem, self.preconditioner)
+            groups.setdefault(key, []).append(_Active(r, st, t))
+            touched.add(key)
             used += need
-        return used
+            n_active += 1
+        return used, touched
 
-    def _run_async(self, requests, clock, progress) -> ServerStats:
+    def run(self, requests: list[Request], clock: str = "wall", progress=None) -> ServerStats:
+        """Serve the stream. clock='wall': arrivals are real time offsets from
+        the start; clock='instant': every request is available at t=0.
+        progress(msg) is called whenever a design finishes."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        active: list[_Active] = []
+        groups: dict[tuple, list[_Active]] = {}
+        plans: dict[tuple, Plan] = {}
         stats = ServerStats()
-        plan: Plan | None = None
         used = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
-        while pending or active:
-            n_before = len(active)
-            used = self._admit(pending, active, used, now(), clock)
-            if not active:
-                if pending and clock == "wall":
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
-                continue
-            if plan is None or len(active) != n_before:
-                if plan is not None:
-                    plan.close()
-                plan = Plan([a.state for a in active])
-            stats.max_active = max(stats.max_active, len(active))
-            sens = [a.req.problem.filter_type == "sensitivity" for a in active]
-            if any(sens) and not all(sens):
-                raise NotImplementedError("a batch mixes density and sensitivity filtering")
-            # while requests wait for a slot that is free, return often to admit them
-            waiting = bool(pending) and len(active) < self.max_active
-            phase = [a.phase for a in active]
-            out, done_now, cg_now = plan.advance(phase, sensitivity_filter=sens[0],
-                                                 max_chunks=self.admit_chunks if waiting else 256)
-            stats.steps += 1
-            t = now()
-            if progress is not None and stats.steps % 25 == 1:
-                progress(f"call {stats.steps} at {t:.1f} s: {len(active)} active, "
-                         f"{len(stats.results)} finished, {len(pending)} waiting")
-            keep = []
-            for a, ph, s, dn, cg in zip(active, phase, out, done_now, cg_now):
-                a.phase = ph
-                a.it += dn
-                a.n_cg += cg
-                if dn:
-                    a.last = s
-                if ph == 3:
-                    s = a.last
-                    pr = a.req.problem
-                    stats.results.append(Result(
-                        a.req.rid, a.req.arrival, a.admitted, t, a.it, s.compliance, s.volume, s.change,
-                        a.n_cg, a.state.rho.cpu().numpy() if self.keep_rho else None))
-                    used -= footprint_bytes(pr, self.precision, self.preconditioner)
-                    if progress is not None:
-                        r = stats.results[-1]
-                        progress(f"design {r.rid} ({pr.nelx}x{pr.nely}) done after {r.iterations} it, "
-                                 f"latency {r.latency:.2f} s, {len(stats.results)} finished, "
-                                 f"{len(pending)} waiting")
-                else:
-                    keep.append(a)
-            if len(keep) != len(active):
-                active = keep
-                plan.close()
-                plan = None
-        if plan is not None:
-            plan.close()
+
+        def finish(a: _Active, s, t):
+            pr = a.req.problem
+            stats.results.append(Result(
+                a.req.rid, a.req.arrival, a.admitted, t, a.it, s.compliance, s.volume, s.change,
+                a.n_cg, a.state.rho.cpu().numpy() if self.keep_rho else None))
+            if progress is not None:
+                r = stats.results[-1]
+                progress(f"design {r.rid} ({pr.nelx}x{pr.nely}) done after {r.iterations} it, "
+                         f"latency {r.latency:.2f} s, {len(stats.results)} finished, "
+                         f"{len(pending)} waiting")
+            return footprint_bytes(pr, self.precision, self.preconditioner)
+
+        try:
+            while pending or any(groups.values()):
+                used, touched = self._admit(pending, groups, used, now(), clock)
+                for key in touched:
+                    if key in plans:
+                        plans.pop(key).close()
+                live = [k for k, g in groups.items() if g]
+                if not live:
+                    if pending and clock == "wall":
+                        time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                    continue
+                n_active = sum(len(groups[k]) for k in live)
+                stats.max_active = max(stats.max_active, n_active)
+                # while requests wait for a free slot, or other classes wait
+                # for their turn, return often
+                short = (bool(pending) and n_active < self.max_active) or len(live) > 1
+                for key in live:
+                    members = groups[key]
+                    if key not in plans:
+                        plans[key] = Plan([a.state for a in members])
+                    plan = plans[key]
+                    if self.mode == "async":
+                        phase = [a.phase for a in members]
+                        out, done_now, cg_now = plan.advance(
+                            phase, sensitivity_filter=key[1],
+                            max_chunks=self.admit_chunks if short else 256)
+                        t = now()
+                        keep = []
+                        for a, ph, s, dn, cg in zip(members, phase, out, done_now, cg_now):
+                            a.phase = ph
+                            a.it += dn
+                            a.n_cg += cg
+                            if dn:
+                                a.last = s
+                            if ph == 3:
+                                used -= finish(a, a.last, t)
+                            else:
+                                keep.append(a)
+                    else:
+                        out = plan.iteration(sensitivity_filter=key[1])
+                        t = now()
+                        keep = []
+                        for a, s in zip(members, out):
+                            a.it += 1
+                            a.n_cg += s.n_cg
+                            pr = a.req.problem
+                            if a.it >= pr.max_iter or (a.it > 5 and s.change < pr.tol):
+                                used -= finish(a, s, t)
+                            else:
+                                keep.append(a)
+                    if len(keep) != len(members):
+                        groups[key] = keep
+                        plans.pop(key).close()
+                stats.steps += 1
+                if progress is not None and stats.steps % 25 == 1:
+                    progress(f"step {stats.steps} at {now():.1f} s: "
+                             f"{sum(len(g) for g in groups.values())} active, "
+                             f"{len(stats.results)} finished, {len(pending)} waiting")
+        finally:
+            for p in plans.values():
+                p.close()
         stats.makespan = now()
         stats.results.sort(key=lambda r: r.rid)
         return stats
