@@ -211,6 +211,19 @@ int topt_filter_tables(int32_t nelx, int32_t nely, double rmin, int32_t width, i
  * out[4] (device): change, volume, last lambda, steps. */
 int topt_oc_update(int32_t nelx, int32_t nely, double* rho, const double* dcf, const double* dv,
                    double volfrac, double move, double* out, void* ws, void* stream);
+/* Allocation-free variant of topt_oc_update (no reference counterpart): for
+ * callers that cannot allocate device memory or pass a workspace, such as
+ * code under HIP graph capture and tight serving loops. Same arguments
+ * without ws, same results bit for bit. Its scratch (OC state, reduction
+ * counters, partial sums) is a static device array of
+ * TOPT_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; grids
+ * needing more (beyond about 32 million elements) are rejected with
+ * TOPT_EINVAL. Makes no hipMalloc/hipFree/hipMemset call; the state is
+ * cleared by a kernel at every call. All calls on one GPU share the array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+#define TOPT_NOALLOC_SCRATCH_BYTES (4 << 20)
+int topt_oc_update_noalloc(int32_t nelx, int32_t nely, double* rho, const double* dcf, const double* dv,
+                           double volfrac, double move, double* out, void* stream);
 /* mean of an element field (fem/kernels.py:330-336) into out[0] (device). */
 int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* ws, void* stream);
 

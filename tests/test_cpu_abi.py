@@ -50,3 +50,16 @@ def test_gpu_entry_points_reject_null_without_touching_the_gpu():
     assert lib.topt_plan_create(None, 0, 1, None) < 0
     assert lib.topt_matvec(None, 1, None, None, None, None, None, None, None) < 0
     assert lib.topt_oc_update(4, 4, None, None, None, 0.5, 0.2, None, None, None) < 0
+
+
+def test_noalloc_oc_rejects_oversize_grid_on_the_host():
+    """The size check runs before any device call (no GPU needed)."""
+    from toptimization_amd import _lib
+    lib = _lib.load()
+    dummy = C.c_void_p(256)
+    # 8192 x 8192 elements need ~8.5 MB of scratch, more than the 4 MiB array
+    assert lib.topt_oc_update_noalloc(8192, 8192, dummy, dummy, None, 0.5, 0.2, dummy, None) == -1
+    assert b"static" in lib.topt_last_error()
+    assert lib.topt_oc_update_noalloc(4, 4, None, None, None, 0.5, 0.2, None, None) == -1
+    assert lib.topt_oc_update_noalloc(4, 4, dummy, dummy, dummy, 0.5, 0.2, dummy, None) == -1
+    assert b"dv must be NULL" in lib.topt_last_error()

@@ -21,6 +21,7 @@ ST_RUN, ST_CONV, ST_BREAK, ST_MAXIT, ST_ZERO, ST_DONE = 0, 1, 2, 3, 4, 5
 BUF_NAMES = ["u", "f", "fix", "rho", "rhof", "dc", "dcf", "Hs", "x", "r", "z", "p0", "p1", "Ap",
              "pre", "pre2", "s", "s64", "taps", "cg", "oc", "part", "opart", "ctr", "mg"]
 BUF = {n: i for i, n in enumerate(BUF_NAMES)}
+NOALLOC_SCRATCH_BYTES = 4 << 20   # TOPT_NOALLOC_SCRATCH_BYTES
 
 
 class TProblem(C.Structure):
@@ -71,6 +72,7 @@ _SIGS = {
     "topt_filter": (C.c_int, [_i32, _i32, _d, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "topt_filter_tables": (C.c_int, [_i32, _i32, _d, _i32, _vp, _vp, _vp, _vp, _vp]),
     "topt_oc_update": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp]),
+    "topt_oc_update_noalloc": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     "topt_volume": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _vp]),
     "topt_blas1_workspace_bytes": (_i64, []),
     "topt_blas1": (C.c_int, [_i32, _i32, _i64, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

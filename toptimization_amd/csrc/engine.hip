@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,12 @@
 #include "opt_kernels.h"
 
 using namespace topt;
+
+// Scratch of topt_oc_update_noalloc (OC state, reduction counters, partial
+// sums): device memory carried by the code object, one copy per GPU, held
+// until the process ends. Host code never uses this name as an address: the
+// per-device address comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[TOPT_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -414,6 +421,63 @@ __global__ void k_oc_out(const OcState* o, double* out) {
   out[1] = o->vol;
   out[2] = o->last_mid;
   out[3] = (double)o->steps;
+}
+
+// Zero n 32-bit words (used instead of hipMemsetAsync where a caller may
+// capture the launches into a graph).
+__global__ void k_zero_words(unsigned* p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Per-device address of `scratch`, looked up once per device with that
+// device current.
+#define TOPT_MAX_DEVICES 64
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[TOPT_MAX_DEVICES] = {};
+
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= TOPT_MAX_DEVICES) return fail(TOPT_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) return fail(TOPT_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TOPT_OK;
+}
+
+// Layout of the OC bisection's scratch inside one buffer: OcState, the OC
+// tree's arrival counters (these two are zeroed per call), then the partial
+// sums (31 bisection volumes per block at each tree level).
+struct OcScratch {
+  int64_t ctr_off, part_off, zero_words, total;
+};
+
+OcScratch oc_scratch(int nb_oc) {
+  OcScratch o;
+  o.ctr_off = align256(sizeof(OcState));
+  o.part_off = align256(o.ctr_off + (tree_ctrs(nb_oc) + 1) * TOPT_CTR_STRIDE * 4);
+  o.zero_words = o.part_off / 4;
+  o.total = o.part_off + align256(tree_parts(nb_oc) * 32 * 8);
+  return o;
+}
+
+// The OC update's launches (optimizer/oc.py:52-93): state reset, ten
+// depth-5 bisection passes (the reference's <= 50 steps), commit, results.
+int launch_oc_update(ProbDev& P, double* out, hipStream_t st) {
+  constexpr int D = 5;
+  hipLaunchKernelGGL((k_oc_reset<false>), dim3(1), dim3(64), 0, st, nullptr, nullptr, 1, P);
+  for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
+    hipLaunchKernelGGL((k_oc_eval<D, false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
+  hipLaunchKernelGGL((k_oc_apply<false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
+  LAUNCH_TRY();
+  hipLaunchKernelGGL(k_oc_out, dim3(1), dim3(1), 0, st, P.oc, out);
+  LAUNCH_TRY();
+  return TOPT_OK;
 }
 
 }  // namespace
@@ -1586,15 +1650,44 @@ int topt_oc_update(int32_t nelx, int32_t nely, double* rho, const double* dcf, c
     // subtree pass; reject instead of silently assuming dv = 1
     return fail(TOPT_EINVAL, "topt_oc_update: dv must be NULL (dv = 1, fem/kernels.py:270)");
   }
-  constexpr int D = 5;
-  hipLaunchKernelGGL((k_oc_reset<false>), dim3(1), dim3(64), 0, st, nullptr, nullptr, 1, P);
-  for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
-    hipLaunchKernelGGL((k_oc_eval<D, false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
-  hipLaunchKernelGGL((k_oc_apply<false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
+  return launch_oc_update(P, out, st);
+}
+
+int topt_oc_update_noalloc(int32_t nelx, int32_t nely, double* rho, const double* dcf, const double* dv,
+                           double volfrac, double move, double* out, void* stream) {
+  if (rho == nullptr || dcf == nullptr || out == nullptr) return fail(TOPT_EINVAL, "NULL argument");
+  if (dv != nullptr) return fail(TOPT_EINVAL, "topt_oc_update_noalloc: dv must be NULL (dv = 1, fem/kernels.py:270)");
+  if (nelx < 1 || nely < 1) return fail(TOPT_EINVAL, "grid must have nelx, nely >= 1");
+  const Geom g = geom(nelx, nely);
+  const OcScratch o = oc_scratch(g.nb[G_OC]);
+  if (o.total > TOPT_NOALLOC_SCRATCH_BYTES)
+    return fail(TOPT_EINVAL, "topt_oc_update_noalloc: the grid needs " + std::to_string(o.total) +
+                                 " bytes of scratch, more than the static " +
+                                 std::to_string((long long)TOPT_NOALLOC_SCRATCH_BYTES) +
+                                 " (at most about 32 million elements); use topt_oc_update");
+  void* base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  ProbDev P;
+  std::memset(&P, 0, sizeof(P));
+  P.nelx = nelx;
+  P.nely = nely;
+  for (int k = 0; k < TOPT_NGRID; ++k) P.nb[k] = g.nb[k];
+  char* w = (char*)base;
+  P.oc = (OcState*)w;
+  P.ctr[G_OC] = (unsigned*)(w + o.ctr_off);
+  P.opart = (double*)(w + o.part_off);
+  P.rho = rho;
+  P.dcf = (double*)dcf;
+  P.dv = nullptr;
+  P.volfrac = volfrac;
+  P.move = move;
+  // the state and the counters start at zero (a fill kernel, no memset node)
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(64, cdiv(o.zero_words, TOPT_BLOCK))),
+                     dim3(TOPT_BLOCK), 0, st, (unsigned*)w, (long)o.zero_words);
   LAUNCH_TRY();
-  hipLaunchKernelGGL(k_oc_out, dim3(1), dim3(1), 0, st, P.oc, out);
-  LAUNCH_TRY();
-  return TOPT_OK;
+  return launch_oc_update(P, out, st);
 }
 
 int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* ws, void* stream) {
