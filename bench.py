@@ -4,13 +4,16 @@ A step is one OC iteration of the SIMP loop (density filter -> PCG ->
 sensitivity -> dc filter -> OC bisection) over the 1200x400 Q4 bridge:
 bottom corners pinned, total unit load spread over the top row, volfrac 0.3,
 rmin 6, p 3, PCG rtol 1e-6 with FP32 vectors, FP64 dots and FP64 residual
-refresh, 2x2 block Jacobi. Inputs live in HBM before the timed region.
+refresh, geometric multigrid V(1,1) preconditioner. Inputs live in HBM
+before the timed region.
 
-Also reported (same JSON line): PCG iterations/s, matvec GDOF/s, the plain
-FP64 GPU path on the same iterations, the compliance agreement of the two
-paths on one density, the roofline of the dominant kernel (PCG slot A:
-p = z + beta p over the tile halo, Ap = K p, p.Ap) timed with HIP events,
-and the C
+Also reported (same JSON line): PCG iterations/s, the device time of the
+iteration's phases (HIP events), the roofline of the multigrid CG iteration
+(13 launches: PCG slots A, B, C and the V-cycle; algorithmic bytes over
+device time per CG iteration, HBM traffic from the committed rocprofv3
+summary), slot A alone, the whole 200-iteration run, matvec GDOF/s, the
+plain FP64 GPU path on the same iterations, the 2x2 block-Jacobi path, the
+compliance agreement of the two precisions on one density, and the C
 restatement of the reference algorithm (FP32, Jacobi, as the reference) on
 the host cores as the CPU baseline.
 
@@ -31,14 +34,67 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per slot-A launch at 1200x400 (FP32 CG step with the deferred
-# solution update), measured with rocprofv3 --pmc FETCH_SIZE and --pmc
-# WRITE_SIZE over `bench.py --probe`, each scaled by the counter's reading for
-# a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0):
-# profiles/pcg_slots_bridge1200x400_mixed.md (FETCH 8759.4 KiB x 2.0 + WRITE
-# 15072.6 KiB, median of the 100 replayed launches).
-SLOT_A_TRAFFIC_BYTES = 33_372_090
 NELX, NELY = 1200, 400
+# rocprofv3 summaries this bench takes its measured HBM traffic from: each
+# holds a line "traffic_bytes_per_<unit>: N" (FETCH_SIZE and WRITE_SIZE of
+# the unit's launches, scaled by the copy calibration of the same runs)
+PROFILE_MG = os.path.join(ROOT, "profiles", "multigrid_bridge1200x400_mixed.md")
+PROFILE_SLOTS = os.path.join(ROOT, "profiles", "pcg_slots_bridge1200x400_mixed.md")
+
+
+def profile_traffic(path, unit):
+    """Measured HBM bytes per `unit` from a committed rocprofv3 summary, or
+    None when the summary or its line is absent."""
+    import re
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(rf"traffic_bytes_per_{unit}:\s*([0-9]+)", text)
+    return int(m.group(1)) if m else None
+
+
+def mg_levels(nelx, nely, dense=128):
+    """Multigrid hierarchy: 2x2 agglomeration with ceil halving down to a
+    coarsest level of at most `dense` DOFs."""
+    lv = [(nelx, nely)]
+    while True:
+        nx, ny = (lv[-1][0] + 1) // 2, (lv[-1][1] + 1) // 2
+        lv.append((nx, ny))
+        if 2 * (nx + 1) * (ny + 1) <= dense:
+            return lv
+
+
+def mg_cg_bytes(nelx, nely, ts=4, tail=1024, tail_sum=2560):
+    """Algorithmic bytes of one multigrid PCG iteration of the mixed path
+    (vectors in ts-byte words, two per node; the FP64 solution u; level
+    element matrices of 36 words; the FP64 coarsest inverse):
+      slot A        read z, p, u; write p, Ap, u; read s
+      slot B        read r, Ap; write r
+      down 0        read r, 1/diag, s; write b_1
+      down l, up l  (1 <= l < tail level) read b_l, 1/diag_l, K_l, [x_{l+1}];
+                    write b_{l+1} / x_l
+      tail          read b, 1/diag, K of its levels and the coarsest inverse;
+                    write x of its first level
+      up 0 + slot C read r, 1/diag, x_1, s; write z"""
+    lv = mg_levels(nelx, nely)
+    L = len(lv) - 1
+    nn = [(x + 1) * (y + 1) for x, y in lv]
+    ne = [x * y for x, y in lv]
+    lt = 1
+    while lt < L and (nn[lt] > tail or sum(nn[lt:]) > tail_sum):
+        lt += 1
+    v = 2 * ts
+    b = 4 * v * nn[0] + 32 * nn[0] + ts * ne[0]          # slot A
+    b += 3 * v * nn[0]                                      # slot B
+    b += 2 * v * nn[0] + ts * ne[0] + v * nn[1]             # level-0 down sweep + restriction
+    for l in range(1, lt):
+        b += 2 * v * nn[l] + 36 * ts * ne[l] + v * nn[l + 1]   # down sweep + restriction
+        b += 3 * v * nn[l] + 36 * ts * ne[l] + v * nn[l + 1]   # up sweep
+    b += v * nn[lt] + sum(v * nn[l] + 36 * ts * ne[l] for l in range(lt, L)) + 8 * (2 * nn[L]) ** 2 \
+        + v * nn[lt]                                        # tail
+    b += 3 * v * nn[0] + ts * ne[0] + v * nn[1]             # level-0 up sweep + slot C
+    return b
 
 
 def bridge(nelx, nely, precision, preconditioner="block_jacobi_2x2", tol=1e-6):
@@ -74,7 +130,41 @@ def run_path(precision, warmup, steps, log, preconditioner=MAIN_PC):
     ncg = sum(s.n_cg for s in stats)
     log(f"  [{tag}] timed {steps} it: {dt:.3f} s ({steps / dt:.2f} OC it/s), {ncg} CG iterations, "
         f"last c={stats[-1].compliance:.6e}")
-    return {"seconds": dt, "n_cg": ncg, "stats": stats, "state": st, "plan": plan, "problem": prob}
+    # the same iterations again with the phases timed by HIP events (the
+    # events add a host sync per iteration, so this pass is not the headline)
+    plan.set_timing(True)
+    stats_t = [plan.iteration()[0] for _ in range(steps)]
+    tm = plan.timing()
+    plan.set_timing(False)
+    tm["n_cg"] = sum(s.n_cg for s in stats_t)
+    log(f"  [{tag}] device time per OC iteration: setup {tm['setup'] / steps:.3f} ms, PCG "
+        f"{tm['pcg'] / steps:.3f} ms ({1e3 * tm['pcg'] / max(tm['n_cg'], 1):.1f} us per CG iteration), "
+        f"post {tm['post'] / steps:.3f} ms")
+    return {"seconds": dt, "n_cg": ncg, "stats": stats, "state": st, "plan": plan, "problem": prob,
+            "timing": tm}
+
+
+def full_run(log, iters=200):
+    """Config 2 as stated: all 200 OC iterations of the bridge from rho = 0.3
+    (mixed, multigrid), wall time of the whole loop."""
+    import torch
+    from toptimization_amd.engine import DesignState, Plan
+    prob = bridge(NELX, NELY, "mixed", preconditioner=MAIN_PC)
+    st = DesignState(prob)
+    plan = Plan([st])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ncg = 0
+    for k in range(iters):
+        s = plan.iteration()[0]
+        ncg += s.n_cg
+        if k % 50 == 49:
+            log(f"  [full run] {k + 1} OC iterations, c={s.compliance:.6e}")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.close()
+    log(f"  [full run] {iters} OC iterations in {dt:.3f} s ({iters / dt:.1f} OC it/s), {ncg} CG iterations")
+    return {"seconds": dt, "n_cg": ncg, "iterations": iters, "final_compliance": s.compliance}
 
 
 def same_rho_compliance(rho, log, preconditioner=MAIN_PC):
@@ -165,7 +255,7 @@ def cpu_baseline(log, budget_s=25.0):
 
 def batch_bench(log, n=64, iters=100, seq_sample=None, preconditioner=MAIN_PC):
     """64 independent 128x64 cantilevers (seeded loads, volfrac U[0.3, 0.6],
-    rmin 2.5, mixed FP32/FP64 PCG rtol 1e-6, 2x2 block Jacobi) advanced
+    rmin 2.5, mixed FP32/FP64 PCG rtol 1e-6, multigrid by default) advanced
     together by one plan (one launch per kernel for all of them) against the
     same problems solved one at a time. Every problem's iterates are the same
     either way (per-problem reductions in a fixed order)."""
@@ -380,6 +470,7 @@ def main() -> int:
     ms_a, bytes_a, gbs_a = kernel_roofline(mixed["plan"], nn, ne, log)
     rho_probe = mixed["state"].rho.cpu().numpy()
     mixed["plan"].close()
+    full = full_run(log)
     fp64 = run_path("fp64", args.warmup, args.steps, log)
     fp64["plan"].close()
     # the same workload with the nodal 2x2 block-Jacobi preconditioner (a
@@ -394,6 +485,13 @@ def main() -> int:
 
     value = args.steps / mixed["seconds"]
     fp64_value = args.steps / fp64["seconds"]
+    # roofline of the headline's unit of work: one multigrid CG iteration
+    # (13 launches), algorithmic bytes over device time per CG iteration
+    tm = mixed["timing"]
+    us_cg = 1e3 * tm["pcg"] / max(tm["n_cg"], 1)
+    b_cg = mg_cg_bytes(NELX, NELY)
+    gbs_cg = b_cg / (us_cg * 1e-6) / 1e9
+    log(f"  multigrid CG iteration: {b_cg / 1e6:.2f} MB algorithmic in {us_cg:.1f} us -> {gbs_cg:.0f} GB/s")
     line = {
         "metric": "FEM matvec GDOF/s, OC iters/s, serving p50/p99 latency at compliance err <=1e-6",
         "value": round(value, 3),
@@ -428,10 +526,21 @@ def main() -> int:
         "matvec_8192x4096_gbs": round(mvb_gbs, 1),
         "matvec_f64_8192x4096_gdofs": round(mvd_gdofs, 2),
         "matvec_f64_8192x4096_gbs": round(mvd_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_slot_a<float> (PCG slot A)",
-                     "achieved": round(gbs_a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs_a / HBM_PEAK_GBS, 4), "traffic": SLOT_A_TRAFFIC_BYTES,
-                     "bytes_per_launch": bytes_a, "us_per_launch": round(ms_a * 1e3, 3)},
+        "roofline": {"bound": "hbm",
+                     "kernel": "multigrid PCG iteration: slots A, B, C (fused level-0 up sweep) and the "
+                               "V-cycle (down sweeps 0-4, tail, up sweeps 4-1), 13 launches",
+                     "achieved": round(gbs_cg, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs_cg / HBM_PEAK_GBS, 4),
+                     "traffic": profile_traffic(PROFILE_MG, "cg_iteration"),
+                     "bytes_per_cg_iteration": b_cg, "us_per_cg_iteration": round(us_cg, 2),
+                     "timing": "HIP events around the PCG chunks of the timed OC iterations (launch stream)"},
+        "phase_ms_per_oc_iteration": {k: round(tm[k] / args.steps, 4) for k in ("setup", "pcg", "post")},
+        "slot_a_roofline": {"kernel": "k_slot_a<float> (PCG slot A)", "achieved": round(gbs_a, 1),
+                            "frac": round(gbs_a / HBM_PEAK_GBS, 4),
+                            "traffic": profile_traffic(PROFILE_SLOTS, "slot_a_launch"),
+                            "bytes_per_launch": bytes_a, "us_per_launch": round(ms_a * 1e3, 3)},
+        "full_run_200": {"seconds": round(full["seconds"], 3), "oc_its_per_s": round(200 / full["seconds"], 2),
+                         "cg_iterations": full["n_cg"], "final_compliance": full["final_compliance"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -159,8 +159,19 @@ int topt_plan_sensitivity(topt_plan* plan, int32_t use_filtered, void* stream);
 /* OC bisection + commit for every problem (optimizer/oc.py:52-93). */
 int topt_plan_oc(topt_plan* plan, void* stream);
 /* One SIMP iteration for every problem (optimizer/simp.py:141-197):
- * density mode (sens_filter=0) or sensitivity mode (1). Blocks; fills stats[nprob]. */
+ * density mode (sens_filter=0) or sensitivity mode (1). The launches before
+ * the PCG (density filter, solve setup) and after it (sensitivity, dc filter,
+ * OC, statistics) are replayed from two HIP graphs per mode, captured on the
+ * first call. Blocks when stats != NULL (fills stats[nprob]). */
 int topt_plan_iteration(topt_plan* plan, int32_t sens_filter, topt_iter_stats* stats, void* stream);
+/* Benchmark support: while enabled, every topt_plan_iteration call adds the
+ * device time (HIP events on its stream) of its three phases to running sums:
+ * ms3[0] density filter + solve setup, ms3[1] the PCG (all chunks of CG
+ * iterations, including the chunk in flight when the stop is seen), ms3[2]
+ * sensitivity, dc filter, OC bisection and statistics. Enabling resets the
+ * sums. No reference counterpart. */
+int topt_plan_set_timing(topt_plan* plan, int32_t enable);
+int topt_plan_timing(topt_plan* plan, double* ms3, int64_t* calls);
 /* Read back per-problem stats of the last iteration (blocks). */
 int topt_plan_stats(topt_plan* plan, topt_iter_stats* stats, void* stream);
 

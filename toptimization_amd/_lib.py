@@ -61,6 +61,8 @@ _SIGS = {
     "topt_plan_oc": (C.c_int, [_vp, _vp]),
     "topt_plan_iteration": (C.c_int, [_vp, _i32, C.POINTER(TIterStats), _vp]),
     "topt_plan_stats": (C.c_int, [_vp, C.POINTER(TIterStats), _vp]),
+    "topt_plan_set_timing": (C.c_int, [_vp, _i32]),
+    "topt_plan_timing": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(_i64)]),
     "topt_plan_time_slot": (C.c_int, [_vp, _i32, _i32, C.POINTER(C.c_float), _vp]),
     "topt_op_workspace_bytes": (_i64, [_i32, _i32]),
     "topt_scale": (C.c_int, [C.POINTER(TGrid), _i32, _vp, _vp, _vp]),

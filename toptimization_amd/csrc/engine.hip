@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -526,10 +527,22 @@ struct topt_plan {
   // hipGraphLaunch on the caller's stream
   bool use_graph = false;
   hipStream_t cap = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t mgraph = nullptr;       // multigrid plans: one 2-iteration chunk
-  hipGraphExec_t mgexec = nullptr;
+  // one graph per status slot: the chunk's slots followed by its status
+  // gather into dstat[slot]
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t mgraph[2] = {nullptr, nullptr};       // multigrid plans: one 2-iteration chunk
+  hipGraphExec_t mgexec[2] = {nullptr, nullptr};
+  // lockstep SIMP iteration (topt_plan_iteration), per filter mode: the
+  // launches before the PCG (density filter, solve setup) and after it
+  // (sensitivity, dc filter, OC, statistics gather)
+  hipGraph_t igraph[2][2] = {};
+  hipGraphExec_t iexec[2][2] = {};
+  // phase timing of topt_plan_iteration (bench support)
+  bool timing = false;
+  hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double tsum[3] = {0.0, 0.0, 0.0};
+  int64_t tcalls = 0;
   // PCG grid map of (problem, local tile) pairs: the full map, and two
   // compacted copies (alternating, each filled from its own pinned staging
   // buffer) that leave out problems whose solve has ended
@@ -687,17 +700,16 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
 // slot-A count (the ramp-up chunks 4 + 4 + 8 + 16 = 32 iterations leave both
 // even; multigrid chunks are 2 iterations of 3 slots), so its kernel
 // arguments are the same each time and one graph serves the whole solve.
-int chunk_graph(topt_plan* pl, int count, hipGraph_t* graph, hipGraphExec_t* exec) {
+// Capture what `body` enqueues on the plan's private stream into a graph
+// (kernels only) and instantiate it, once.
+int capture_graph(topt_plan* pl, const std::function<int(hipStream_t)>& body, hipGraph_t* graph,
+                  hipGraphExec_t* exec) {
   if (*exec) return TOPT_OK;
   if (!pl->cap) HIP_TRY(hipStreamCreateWithFlags(&pl->cap, hipStreamNonBlocking));
-  const int64_t q0 = pl->q, na0 = pl->na;
   HIP_TRY(hipStreamBeginCapture(pl->cap, hipStreamCaptureModeThreadLocal));
-  const int rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, pl->cap)
-                                       : launch_slots<double>(pl, count, pl->cap);
+  const int rc = body(pl->cap);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(pl->cap, &g);
-  pl->q = q0;     // capturing launched nothing
-  pl->na = na0;
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -708,6 +720,26 @@ int chunk_graph(topt_plan* pl, int count, hipGraph_t* graph, hipGraphExec_t* exe
   return TOPT_OK;
 }
 
+// A full chunk's slot launches and its status gather into dstat[slot]. The
+// gather reads the state parity after the chunk, which a full chunk (even
+// start, even length in slots) leaves even.
+int chunk_graph(topt_plan* pl, int count, int slot, hipGraph_t* graph, hipGraphExec_t* exec) {
+  if (*exec) return TOPT_OK;
+  const int64_t q0 = pl->q, na0 = pl->na;
+  const int rc = capture_graph(pl, [&](hipStream_t cs) {
+    const int r = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, cs)
+                                        : launch_slots<double>(pl, count, cs);
+    if (r) return r;
+    hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, cs, pl->dtab, pl->n,
+                       (int)(pl->q & 1), pl->dstat[slot]);
+    LAUNCH_TRY();
+    return (int)TOPT_OK;
+  }, graph, exec);
+  pl->q = q0;     // capturing launched nothing
+  pl->na = na0;
+  return rc;
+}
+
 int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   int rc;
   const bool whole = (pl->q & 1) == 0 && (pl->na & 1) == 0 && pl->pcur == pl->dpfull;
@@ -716,11 +748,11 @@ int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   const bool mg_graph = pl->mg && count == TOPT_MG_CHUNK && whole;
   if (user_graph || mg_graph) {
     if (user_graph) {
-      if ((rc = chunk_graph(pl, count, &pl->graph, &pl->gexec))) return rc;
-      HIP_TRY(hipGraphLaunch(pl->gexec, s));
+      if ((rc = chunk_graph(pl, count, slot, &pl->graph[slot], &pl->gexec[slot]))) return rc;
+      HIP_TRY(hipGraphLaunch(pl->gexec[slot], s));
     } else {
-      if ((rc = chunk_graph(pl, count, &pl->mgraph, &pl->mgexec))) return rc;
-      HIP_TRY(hipGraphLaunch(pl->mgexec, s));
+      if ((rc = chunk_graph(pl, count, slot, &pl->mgraph[slot], &pl->mgexec[slot]))) return rc;
+      HIP_TRY(hipGraphLaunch(pl->mgexec[slot], s));
     }
     const int per_it = pl->three_slot ? 3 : 2;
     pl->q += (int64_t)per_it * count;
@@ -728,10 +760,10 @@ int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   } else {
     rc = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, s) : launch_slots<double>(pl, count, s);
     if (rc) return rc;
+    hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n,
+                       (int)(pl->q & 1), pl->dstat[slot]);
+    LAUNCH_TRY();
   }
-  hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n,
-                     (int)(pl->q & 1), pl->dstat[slot]);
-  LAUNCH_TRY();
   HIP_TRY(hipEventRecord(pl->ev[slot], s));
   return TOPT_OK;
 }
@@ -780,10 +812,12 @@ int plan_sens(topt_plan* pl, int use_filtered, hipStream_t s) {
   return TOPT_OK;
 }
 
-int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* status, hipStream_t s) {
-  int rc = pl->dtype == TOPT_F32 ? launch_solve_setup<float>(pl, warm, use_filtered, s)
-                                 : launch_solve_setup<double>(pl, warm, use_filtered, s);
-  if (rc) return rc;
+// The PCG chunks of a solve whose setup is enqueued. Returns once the status
+// snapshots show every solve ended; the problems' final states are then
+// settled into cg[0] in stream order (nothing is waited for after the last
+// poll: the chunk still in flight and the settle run while the host goes on).
+int plan_solve_chunks(topt_plan* pl, hipStream_t s) {
+  int rc;
   pl->q = 0;
   pl->na = 0;
   pl->pcur = pl->dpfull;
@@ -836,14 +870,24 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
     issued += chunk;
     wait ^= 1;
   }
-  // drain the chunk still in flight; problems still in the map end at q & 1
-  HIP_TRY(hipEventSynchronize(pl->ev[wait ^ 1]));
+  // problems still in the map end with the chunk in flight, at q & 1 (hpar
+  // is pinned: the copy reads it in stream order, and the host writes it
+  // again only after polling a later chunk)
   for (int i = 0; i < pl->n; ++i)
     if (pl->live[i]) pl->hpar[i] = (int)(pl->q & 1);
   HIP_TRY(hipMemcpyAsync(pl->dpar, pl->hpar, sizeof(int) * pl->n, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_cg_settle, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, pl->n, pl->dpar);
+  LAUNCH_TRY();
   pl->q = 0;
   pl->na = 0;
+  return TOPT_OK;
+}
+
+int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* status, hipStream_t s) {
+  int rc = pl->dtype == TOPT_F32 ? launch_solve_setup<float>(pl, warm, use_filtered, s)
+                                 : launch_solve_setup<double>(pl, warm, use_filtered, s);
+  if (rc) return rc;
+  if ((rc = plan_solve_chunks(pl, s))) return rc;
   hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
@@ -1336,10 +1380,18 @@ int topt_plan_destroy(topt_plan* pl) {
   }
   if (pl->hpar) (void)hipHostFree(pl->hpar);
   (void)hipFree(pl->dpar);
-  if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
-  if (pl->graph) (void)hipGraphDestroy(pl->graph);
-  if (pl->mgexec) (void)hipGraphExecDestroy(pl->mgexec);
-  if (pl->mgraph) (void)hipGraphDestroy(pl->mgraph);
+  for (int k = 0; k < 2; ++k) {
+    if (pl->gexec[k]) (void)hipGraphExecDestroy(pl->gexec[k]);
+    if (pl->graph[k]) (void)hipGraphDestroy(pl->graph[k]);
+    if (pl->mgexec[k]) (void)hipGraphExecDestroy(pl->mgexec[k]);
+    if (pl->mgraph[k]) (void)hipGraphDestroy(pl->mgraph[k]);
+    for (int j = 0; j < 2; ++j) {
+      if (pl->iexec[k][j]) (void)hipGraphExecDestroy(pl->iexec[k][j]);
+      if (pl->igraph[k][j]) (void)hipGraphDestroy(pl->igraph[k][j]);
+    }
+  }
+  for (int k = 0; k < 4; ++k)
+    if (pl->tev[k]) (void)hipEventDestroy(pl->tev[k]);
   for (int l = 0; l < TOPT_MG_MAXLEV; ++l) {
     (void)hipFree(pl->mg_dn[l]);
     (void)hipFree(pl->mg_rs[l]);
@@ -1392,13 +1444,65 @@ int topt_plan_iteration(topt_plan* pl, int32_t sens_filter, topt_iter_stats* sta
   if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
   hipStream_t s = (hipStream_t)stream;
   int rc;
+  const int m = sens_filter ? 1 : 0;
   const int use_f = sens_filter ? 0 : 1;
-  if (!sens_filter && (rc = plan_filter(pl, F_DENSITY, s))) return rc;
-  if ((rc = plan_solve(pl, 2, use_f, nullptr, nullptr, s))) return rc;
-  if ((rc = plan_sens(pl, use_f, s))) return rc;
-  if ((rc = plan_filter(pl, sens_filter ? F_SENS : F_DCAVG, s))) return rc;
-  if ((rc = plan_oc(pl, s))) return rc;
-  if (stats) return gather_stats(pl, stats, s);
+  // before the PCG: density filter and solve setup, one graph per mode
+  rc = capture_graph(pl, [&](hipStream_t cs) {
+    int r;
+    if (!sens_filter && (r = plan_filter(pl, F_DENSITY, cs))) return r;
+    return pl->dtype == TOPT_F32 ? launch_solve_setup<float>(pl, 2, use_f, cs)
+                                 : launch_solve_setup<double>(pl, 2, use_f, cs);
+  }, &pl->igraph[m][0], &pl->iexec[m][0]);
+  if (rc) return rc;
+  // after it: sensitivity, dc filter, OC bisection, statistics gather
+  rc = capture_graph(pl, [&](hipStream_t cs) {
+    int r;
+    if ((r = plan_sens(pl, use_f, cs))) return r;
+    if ((r = plan_filter(pl, sens_filter ? F_SENS : F_DCAVG, cs))) return r;
+    if ((r = plan_oc(pl, cs))) return r;
+    hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, cs, pl->dtab, nullptr, pl->n,
+                       pl->dstats);
+    LAUNCH_TRY();
+    return (int)TOPT_OK;
+  }, &pl->igraph[m][1], &pl->iexec[m][1]);
+  if (rc) return rc;
+  if (pl->timing) HIP_TRY(hipEventRecord(pl->tev[0], s));
+  HIP_TRY(hipGraphLaunch(pl->iexec[m][0], s));
+  if (pl->timing) HIP_TRY(hipEventRecord(pl->tev[1], s));
+  if ((rc = plan_solve_chunks(pl, s))) return rc;
+  if (pl->timing) HIP_TRY(hipEventRecord(pl->tev[2], s));
+  HIP_TRY(hipGraphLaunch(pl->iexec[m][1], s));
+  if (stats)
+    HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
+  if (pl->timing) HIP_TRY(hipEventRecord(pl->tev[3], s));
+  if (!stats && !pl->timing) return TOPT_OK;
+  HIP_TRY(hipStreamSynchronize(s));
+  if (pl->timing) {
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, pl->tev[k], pl->tev[k + 1]));
+      pl->tsum[k] += ms;
+    }
+    pl->tcalls++;
+  }
+  if (stats) std::memcpy(stats, pl->hstats, sizeof(topt_iter_stats) * pl->n);
+  return TOPT_OK;
+}
+
+int topt_plan_set_timing(topt_plan* pl, int32_t enable) {
+  if (pl == nullptr) return fail(TOPT_EINVAL, "plan is NULL");
+  for (int k = 0; k < 4 && enable; ++k)
+    if (!pl->tev[k]) HIP_TRY(hipEventCreate(&pl->tev[k]));
+  pl->timing = enable != 0;
+  pl->tsum[0] = pl->tsum[1] = pl->tsum[2] = 0.0;
+  pl->tcalls = 0;
+  return TOPT_OK;
+}
+
+int topt_plan_timing(topt_plan* pl, double* ms3, int64_t* calls) {
+  if (pl == nullptr || ms3 == nullptr) return fail(TOPT_EINVAL, "invalid timing call");
+  for (int k = 0; k < 3; ++k) ms3[k] = pl->tsum[k];
+  if (calls) *calls = pl->tcalls;
   return TOPT_OK;
 }
 

@@ -243,6 +243,21 @@ class Plan:
                                               self._stream()), "topt_plan_time_slot")
         return ms.value / reps
 
+    def set_timing(self, enable: bool = True) -> None:
+        """Accumulate the device time of each iteration() phase (HIP events on
+        the launch stream); enabling resets the sums (benchmark support)."""
+        check(_lib.load().topt_plan_set_timing(self.handle, int(bool(enable))), "topt_plan_set_timing")
+
+    def timing(self) -> dict:
+        """Summed device milliseconds of the iteration() phases since
+        set_timing(True): 'setup' (density filter + solve setup), 'pcg' (all
+        chunks of CG iterations), 'post' (sensitivity, dc filter, OC,
+        statistics), and the number of iterations timed."""
+        ms = (C.c_double * 3)()
+        calls = C.c_int64()
+        check(_lib.load().topt_plan_timing(self.handle, ms, C.byref(calls)), "topt_plan_timing")
+        return {"setup": ms[0], "pcg": ms[1], "post": ms[2], "iterations": calls.value}
+
     def stats(self) -> list[IterStats]:
         out = (TIterStats * len(self))()
         check(_lib.load().topt_plan_stats(self.handle, out, self._stream()), "topt_plan_stats")
