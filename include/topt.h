@@ -238,6 +238,52 @@ int topt_oc_update_noalloc(int32_t nelx, int32_t nely, double* rho, const double
 /* mean of an element field (fem/kernels.py:330-336) into out[0] (device). */
 int topt_volume(int32_t nelx, int32_t nely, const double* v, double* out, void* ws, void* stream);
 
+/* ---- 3-D H8 path (FP64; trilinear hexahedra on the structured grid) ----
+ * Layout: node n = (ix*(nely+1) + iy)*(nelz+1) + iz with three DOFs per node
+ * interleaved (x, y, z); element e = (ex*nely + ey)*nelz + ez (mesh.py:68-112);
+ * fixed DOFs as a bitmask (bit 3n+c). Ke is a device array of 576 doubles,
+ * the 24x24 H8 stiffness in the reference's local DOF order (material.py:89-171).
+ * The OC update of a 3-D design is topt_oc_update with nely*nelz in place of
+ * nely (it only reads element arrays). */
+typedef struct topt_grid3 {
+  int32_t nelx, nely, nelz, pad;
+  double E_min, penal;
+  const double* Ke;
+} topt_grid3;
+
+/* Workspace bytes of the 3-D calls (reductions, PCG vectors, filter taps). */
+int64_t topt3_workspace_bytes(int32_t nelx, int32_t nely, int32_t nelz);
+/* s_e = E_min + rho_e^p (1 - E_min) (fem/kernels.py:200). */
+int topt3_scale(const topt_grid3* g, const double* rho, double* s, void* stream);
+/* diag K(s), fixed -> 1, clamp 1e-20 (fem/kernels.py:104-129). */
+int topt3_diagonal(const topt_grid3* g, const double* s, const uint32_t* fix, double* diag, void* stream);
+/* Ap = K(s) p, fixed rows identity (fem/kernels.py:179-212); if pAp != NULL
+ * also p.Ap (fem/fused_kernels.py:59-94). */
+int topt3_matvec(const topt_grid3* g, const double* p, double* Ap, const double* s, const uint32_t* fix,
+                 double* pAp, void* ws, void* stream);
+/* Element-block Jacobi z = M r with r.z, r.r into rz_rr[2]
+ * (fem/preconditioner.py:111-156); Ke_inv = inv(Ke + alpha_reg I) (device, 576). */
+int topt3_elemblock_apply(const topt_grid3* g, const double* Ke_inv, double alpha_reg, const double* r, double* z,
+                          const double* s, const uint32_t* fix, double* rz_rr, void* ws, void* stream);
+/* Compliance and dc (fem/kernels.py:275-308); dv filled with 1 if non-NULL. */
+int topt3_sensitivity(const topt_grid3* g, const double* u, const double* rho, double* dc, double* dv,
+                      double* compliance, void* ws, void* stream);
+/* 3-D filter over the reference's neighbourhood (optimizer/filter.py:75-97,
+ * 106-171): modes TOPT_FILTER_DENSITY, _DCAVG, _SENS; Hs is filled. rmin <= 12. */
+int topt3_filter(int32_t nelx, int32_t nely, int32_t nelz, double rmin, int32_t mode, const double* rho,
+                 const double* dc, double* out, double* Hs, void* ws, void* stream);
+/* The reference's padded 3-D tables (optimizer/filter.py:75-97). Blocks. */
+int topt3_filter_tables(int32_t nelx, int32_t nely, int32_t nelz, double rmin, int32_t width, int32_t* nb,
+                        float* wt, int32_t* cnt, void* ws, void* stream);
+/* PCG K(s) u = f (fem/solver.py:27-167) with preconditioner TOPT_PC_NONE,
+ * _JACOBI or _ELEMBLOCK, on the device (alpha, beta and the stopping tests in
+ * device state); warm = 1 starts from u, else from 0; u[fixed] = 0. Blocks;
+ * n_iter follows the reference's count (max_iter on breakdown or the cap, 0
+ * for a zero load, which leaves u = 0). */
+int topt3_solve(const topt_grid3* g, int32_t precond, const double* Ke_inv, double alpha_reg, const double* f,
+                const uint32_t* fix, const double* s, double* u, int32_t warm, double tol, int32_t max_iter,
+                int32_t* n_iter, int32_t* status, void* ws, void* stream);
+
 /* Vector ops of the drop-in API, op codes (buffers by role x, y, z, w, v):
  *  0 FILL      x = a                              fem/kernels.py:19-23
  *  1 COPY      y = x                              fem/kernels.py:26-30

@@ -52,6 +52,58 @@ def edof_q4(nelx: int, nely: int) -> np.ndarray:
     return out
 
 
+H8_NODES = ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))
+
+
+def ke_h8(E: float = 1.0, nu: float = 0.3) -> np.ndarray:
+    """24x24 H8 stiffness of the unit cube by 2x2x2 Gauss quadrature
+    (``material.py:89-171``): isotropic D (Lame lambda, mu), trilinear shape
+    functions written out node by node, Jacobian 1/2 per axis. Pinned by the
+    six rigid-body modes and the constant-strain patch test, not by
+    comparison with the reference's code."""
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    D = np.zeros((6, 6))
+    for i in range(3):
+        for j in range(3):
+            D[i, j] = lam + (2.0 * mu if i == j else 0.0)
+        D[3 + i, 3 + i] = mu
+    g = 1.0 / math.sqrt(3.0)
+    K = np.zeros((24, 24))
+    for xi in (-g, g):
+        for eta in (-g, g):
+            for zeta in (-g, g):
+                B = np.zeros((6, 24))
+                for a, (cx, cy, cz) in enumerate(H8_NODES):
+                    sx, sy, sz = 2 * cx - 1, 2 * cy - 1, 2 * cz - 1
+                    # d/dx = 2 d/dxi on the unit cube
+                    nx = sx * (1 + sy * eta) * (1 + sz * zeta) / 4.0
+                    ny = sy * (1 + sx * xi) * (1 + sz * zeta) / 4.0
+                    nz = sz * (1 + sx * xi) * (1 + sy * eta) / 4.0
+                    c = 3 * a
+                    B[0, c] = nx
+                    B[1, c + 1] = ny
+                    B[2, c + 2] = nz
+                    B[3, c], B[3, c + 1] = ny, nx          # gamma_xy
+                    B[4, c + 1], B[4, c + 2] = nz, ny      # gamma_yz
+                    B[5, c], B[5, c + 2] = nz, nx          # gamma_xz
+                K += B.T @ D @ B / 8.0
+    return K
+
+
+def edof_h8(nelx: int, nely: int, nelz: int) -> np.ndarray:
+    """3-D element -> DOF map, restating the loop of ``mesh.py:68-112``."""
+    out = np.zeros((nelx * nely * nelz, 24), dtype=np.int64)
+    for ex in range(nelx):
+        for ey in range(nely):
+            for ez in range(nelz):
+                e = ex * nely * nelz + ey * nelz + ez
+                for k, (dx, dy, dz) in enumerate(H8_NODES):
+                    n = (ex + dx) * (nely + 1) * (nelz + 1) + (ey + dy) * (nelz + 1) + (ez + dz)
+                    out[e, 3 * k:3 * k + 3] = [3 * n, 3 * n + 1, 3 * n + 2]
+    return out
+
+
 def force_vector(n_dofs: int, force_dofs, force_values, fixed_dofs) -> np.ndarray:
     """``fields.py:151-155``: duplicate loads summed, fixed entries zeroed."""
     f = np.zeros(n_dofs)
@@ -284,6 +336,61 @@ def filter_tables(nelx: int, nely: int, rmin: float):
     return nb, wt, cnt
 
 
+def filter_tables_bruteforce_3d(nelx: int, nely: int, nelz: int, rmin: float):
+    """3-D padded tables exactly as ``optimizer/filter.py:75-97,99-103``
+    (float32 centres, FP64 sqrt, float32 weights; jx outer, jy, jz inner)."""
+    r_ceil = math.ceil(rmin)
+    width = (2 * r_ceil + 1) ** 3
+    n_elem = nelx * nely * nelz
+    nb = np.full((n_elem, width), -1, dtype=np.int32)
+    wt = np.zeros((n_elem, width), dtype=np.float32)
+    cnt = np.zeros(n_elem, dtype=np.int32)
+    for ex in range(nelx):
+        for ey in range(nely):
+            for ez in range(nelz):
+                e = ex * nely * nelz + ey * nelz + ez
+                cx, cy, cz = np.float32(ex + 0.5), np.float32(ey + 0.5), np.float32(ez + 0.5)
+                k = 0
+                for jx in range(max(0, ex - r_ceil), min(nelx, ex + r_ceil + 1)):
+                    for jy in range(max(0, ey - r_ceil), min(nely, ey + r_ceil + 1)):
+                        for jz in range(max(0, ez - r_ceil), min(nelz, ez + r_ceil + 1)):
+                            dist = math.sqrt((cx - (jx + 0.5)) ** 2 + (cy - (jy + 0.5)) ** 2
+                                             + (cz - (jz + 0.5)) ** 2)
+                            w = max(0.0, rmin - dist)
+                            if w > 0.0:
+                                nb[e, k] = jx * nely * nelz + jy * nelz + jz
+                                wt[e, k] = w
+                                k += 1
+                cnt[e] = k
+    return nb, wt, cnt
+
+
+def filter_tables_3d(nelx: int, nely: int, nelz: int, rmin: float):
+    """Vectorised 3-D tables from the stencil in table order; equality with
+    ``filter_tables_bruteforce_3d`` is a test."""
+    r_ceil = math.ceil(rmin)
+    width = (2 * r_ceil + 1) ** 3
+    n_elem = nelx * nely * nelz
+    e = np.arange(n_elem)
+    ex, rem = np.divmod(e, nely * nelz)
+    ey, ez = np.divmod(rem, nelz)
+    nb = np.full((n_elem, width), -1, dtype=np.int32)
+    wt = np.zeros((n_elem, width), dtype=np.float32)
+    cnt = np.zeros(n_elem, dtype=np.int32)
+    for dx in range(-r_ceil, r_ceil + 1):
+        for dy in range(-r_ceil, r_ceil + 1):
+            for dz in range(-r_ceil, r_ceil + 1):
+                w = max(0.0, rmin - math.sqrt(float(dx * dx + dy * dy + dz * dz)))
+                if w <= 0.0:
+                    continue
+                jx, jy, jz = ex + dx, ey + dy, ez + dz
+                ok = (jx >= 0) & (jx < nelx) & (jy >= 0) & (jy < nely) & (jz >= 0) & (jz < nelz)
+                nb[e[ok], cnt[ok]] = ((jx * nely + jy) * nelz + jz)[ok]
+                wt[e[ok], cnt[ok]] = np.float32(w)
+                cnt[ok] += 1
+    return nb, wt, cnt
+
+
 def _gather_sum(vals, nb, wt, cnt):
     """Sequential table-order sums sum_k w_k vals[nb_k] and sum_k w_k (FP64)."""
     num = np.zeros(nb.shape[0])
@@ -383,13 +490,33 @@ def simp_run(nelx, nely, fixed_dofs, force_dofs, force_values, *, E=1.0, nu=0.3,
     Returns a dict with per-iteration lists ``c, vol, change, n_cg`` and the
     final ``rho`` (flat, e = ex*nely+ey) and ``u``.
     """
-    Ke = ke_q4(E, nu)
-    edof = edof_q4(nelx, nely)
     n_dofs = 2 * (nelx + 1) * (nely + 1)
+    return _simp_loop(ke_q4(E, nu), edof_q4(nelx, nely), n_dofs, filter_tables(nelx, nely, rmin),
+                      fixed_dofs, force_dofs, force_values, E_min=E_min, volfrac=volfrac, penal=penal,
+                      filter_type=filter_type, max_iter=max_iter, tol=tol, move=move, max_cg=max_cg,
+                      cg_tol=cg_tol, preconditioner=preconditioner)
+
+
+def simp_run_3d(nelx, nely, nelz, fixed_dofs, force_dofs, force_values, *, E=1.0, nu=0.3, E_min=1e-9,
+                volfrac=0.5, penal=3.0, rmin=1.5, filter_type="density", max_iter=50, tol=0.01,
+                move=0.2, max_cg=2000, cg_tol=1e-8, preconditioner="jacobi"):
+    """The same loop on the 3-D H8 grid (``optimizer/simp.py:99-219`` with
+    dim = 3): H8 Ke, 3-D connectivity and 3-D filter tables; rho flat in
+    e = (ex*nely + ey)*nelz + ez order."""
+    n_dofs = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    return _simp_loop(ke_h8(E, nu), edof_h8(nelx, nely, nelz), n_dofs, filter_tables_3d(nelx, nely, nelz, rmin),
+                      fixed_dofs, force_dofs, force_values, E_min=E_min, volfrac=volfrac, penal=penal,
+                      filter_type=filter_type, max_iter=max_iter, tol=tol, move=move, max_cg=max_cg,
+                      cg_tol=cg_tol, preconditioner=preconditioner)
+
+
+def _simp_loop(Ke, edof, n_dofs, tables, fixed_dofs, force_dofs, force_values, *, E_min, volfrac, penal,
+               filter_type, max_iter, tol, move, max_cg, cg_tol, preconditioner):
+    n_elem = edof.shape[0]
     fixed = fixed_mask(n_dofs, fixed_dofs)
     f = force_vector(n_dofs, force_dofs, force_values, fixed_dofs)
-    nb, wt, cnt = filter_tables(nelx, nely, rmin)
-    rho = np.full(nelx * nely, float(volfrac))
+    nb, wt, cnt = tables
+    rho = np.full(n_elem, float(volfrac))
     u_prev = np.zeros(n_dofs)
     hist = {"c": [], "vol": [], "change": [], "n_cg": []}
     u = u_prev

@@ -15,6 +15,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB_PATH = PKG / "libtopt.so"
+SOURCES = ("engine.hip", "h8.hip")   # 2-D engine and C ABI; 3-D H8 path
 ARCH = os.environ.get("TOPT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -33,9 +34,9 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
-    """Compile csrc/engine.hip into toptimization_amd/libtopt.so (always rebuilds)."""
+    """Compile csrc/*.hip into toptimization_amd/libtopt.so (always rebuilds)."""
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", str(CSRC / "engine.hip"), "-o", str(tmp)]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{ROOT / 'include'}", *[str(CSRC / f) for f in SOURCES], "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"
     msgs = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if noise not in l)

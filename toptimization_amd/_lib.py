@@ -38,6 +38,11 @@ class TGrid(C.Structure):
                 ("penal", C.c_double), ("Ke", C.c_double * 64)]
 
 
+class TGrid3(C.Structure):
+    _fields_ = [("nelx", C.c_int32), ("nely", C.c_int32), ("nelz", C.c_int32), ("pad", C.c_int32),
+                ("E_min", C.c_double), ("penal", C.c_double), ("Ke", C.c_void_p)]
+
+
 class TIterStats(C.Structure):
     _fields_ = [("compliance", C.c_double), ("volume", C.c_double), ("change", C.c_double),
                 ("lam", C.c_double), ("n_cg", C.c_int32), ("cg_status", C.c_int32),
@@ -76,6 +81,16 @@ _SIGS = {
     "topt_oc_update": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp]),
     "topt_oc_update_noalloc": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _d, _d, _vp, _vp]),
     "topt_volume": (C.c_int, [_i32, _i32, _vp, _vp, _vp, _vp]),
+    "topt3_workspace_bytes": (_i64, [_i32, _i32, _i32]),
+    "topt3_scale": (C.c_int, [C.POINTER(TGrid3), _vp, _vp, _vp]),
+    "topt3_diagonal": (C.c_int, [C.POINTER(TGrid3), _vp, _vp, _vp, _vp]),
+    "topt3_matvec": (C.c_int, [C.POINTER(TGrid3), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "topt3_elemblock_apply": (C.c_int, [C.POINTER(TGrid3), _vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "topt3_sensitivity": (C.c_int, [C.POINTER(TGrid3), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "topt3_filter": (C.c_int, [_i32, _i32, _i32, _d, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "topt3_filter_tables": (C.c_int, [_i32, _i32, _i32, _d, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "topt3_solve": (C.c_int, [C.POINTER(TGrid3), _i32, _vp, _d, _vp, _vp, _vp, _vp, _i32, _d, _i32,
+                              C.POINTER(_i32), C.POINTER(_i32), _vp, _vp]),
     "topt_blas1_workspace_bytes": (_i64, []),
     "topt_blas1": (C.c_int, [_i32, _i32, _i64, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }

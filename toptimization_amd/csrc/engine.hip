@@ -21,6 +21,7 @@
 #include "fem_kernels.h"
 #include "mg_kernels.h"
 #include "opt_kernels.h"
+#include "abi_internal.h"
 
 using namespace topt;
 
@@ -482,6 +483,8 @@ int launch_oc_update(ProbDev& P, double* out, hipStream_t st) {
 }
 
 }  // namespace
+
+int topt_abi::set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 // ============================================================== plan
 #define TOPT_SUBSLOTS 4

@@ -32,8 +32,10 @@ def run(problem, backend: str = "auto", live_viz: bool = True,
     """
     if backend not in ("auto", "hip"):
         raise ValueError(f"Unknown backend '{backend}': this build runs on HIP (gfx950) only")
-    if problem.dim != 2:
-        raise NotImplementedError("3-D H8 problems are not on the GPU path")
+    if problem.dim == 3:
+        # the 3-D H8 path: op-level kernels and the device PCG (h8.py)
+        from ..h8 import run_3d
+        return run_3d(problem, verbose=verbose, history=history)
     log = print if verbose else (lambda *a, **k: None)
     log("[toptimization] Backend: hip (gfx950)")
     log(f"[toptimization] Building mesh ({problem.nelx}x{problem.nely})...")

@@ -294,21 +294,25 @@ def load_problem(yaml_path: str | Path) -> Problem:
 
 
 def make_problem(nelx: int, nely: int, *, fixed: list, loads: list, name: str = "request",
-                 **opts) -> Problem:
+                 nelz: int | None = None, **opts) -> Problem:
     """Programmatic constructor used for serving requests and benchmarks.
 
     ``fixed`` / ``loads`` use the YAML list formats of
     ``boundary_conditions.fixed`` and ``loads``; ``opts`` may hold any
     optimization/solver/material key under its YAML name
     (e.g. ``volume_fraction=0.3, filter_radius=6.0, precision='mixed'``).
+    ``nelz`` makes a 3-D problem (``dimension: 3``).
     """
     sections = {"material": ("E", "nu", "E_min"),
                 "optimization": ("volume_fraction", "penalty", "filter_radius", "filter_type",
                                  "max_iterations", "tolerance", "move_limit"),
                 "solver": ("max_cg_iterations", "cg_tolerance", "preconditioner", "precision"),
                 "output": ("directory", "save_interval", "format")}
-    raw: dict = {"problem": {"name": name, "dimension": 2},
-                 "domain": {"nelx": int(nelx), "nely": int(nely)},
+    domain = {"nelx": int(nelx), "nely": int(nely)}
+    if nelz is not None:
+        domain["nelz"] = int(nelz)
+    raw: dict = {"problem": {"name": name, "dimension": 2 if nelz is None else 3},
+                 "domain": domain,
                  "boundary_conditions": {"fixed": fixed}, "loads": loads}
     for sec, keys in sections.items():
         raw[sec] = {k: opts.pop(k) for k in keys if k in opts}
