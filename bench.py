@@ -397,6 +397,58 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="asy
             "sequential_sample": seq_sample, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
 
 
+def h8_bench(log, nelx=96, nely=48, nelz=48, iters=6, reps=50):
+    """3-D H8 path: a 3-D cantilever (left face clamped, unit downward load
+    at the middle of the right face's bottom edge), volfrac 0.3, rmin 1.5,
+    FP64 Jacobi PCG rtol 1e-6 on the device; OC it/s over `iters` OC
+    iterations after one warm-up, CG it/s, and the FP64 3-D matvec (read p,
+    write Ap: 24 B per node each, read s: 8 B per element) with HIP events
+    over back-to-back launches."""
+    import numpy as np
+    import torch
+    from toptimization_amd import h8
+    from toptimization_amd.problem import make_problem
+    prob = make_problem(nelx, nely, nelz=nelz,
+                        fixed=[{"nodes": {"type": "face", "face": "left"}, "dofs": ["x", "y", "z"]}],
+                        loads=[{"nodes": {"type": "point", "location": [nelx, 0, nelz // 2]},
+                                "force": [0.0, -1.0, 0.0]}],
+                        name="cantilever3d", volume_fraction=0.3, filter_radius=1.5, max_iterations=iters + 1,
+                        tolerance=0.0, cg_tolerance=1e-6, max_cg_iterations=50000, preconditioner="jacobi")
+    d = h8.Design3(prob)
+    st = d.iteration()
+    log(f"  [h8 {nelx}x{nely}x{nelz}] warm-up OC iteration: c={st['c']:.6e} n_cg={st['n_cg']}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ncg = 0
+    for _ in range(iters):
+        st = d.iteration()
+        ncg += st["n_cg"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"  [h8] {iters} OC iterations in {dt:.3f} s ({iters / dt:.2f} OC it/s), {ncg} CG iterations "
+        f"({ncg / dt:.0f} CG it/s)")
+    g = d.g
+    p = torch.as_tensor(np.random.default_rng(1).normal(size=g.n_dofs), device="cuda")
+    Ap = torch.empty_like(p)
+    for _ in range(3):
+        h8.matvec(g, p, d.s, d.fix, Ap=Ap)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        h8.matvec(g, p, d.s, d.fix, Ap=Ap)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    b = 48 * g.n_nodes + 8 * g.n_elem
+    log(f"  [h8] matvec {ms * 1e3:.1f} us, {g.n_dofs / (ms * 1e-3) / 1e9:.2f} GDOF/s, "
+        f"{b / (ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
+    return {"metric": f"OC it/s (3-D H8 cantilever {nelx}x{nely}x{nelz}, FP64 Jacobi PCG rtol 1e-6)",
+            "value": round(iters / dt, 3), "unit": "OC it/s", "cg_its_per_s": round(ncg / dt, 1),
+            "cg_per_oc_it": round(ncg / iters, 1), "dofs": g.n_dofs, "elements": g.n_elem,
+            "matvec_us": round(ms * 1e3, 2), "matvec_gdofs": round(g.n_dofs / (ms * 1e-3) / 1e9, 2),
+            "matvec_gbs_algorithmic": round(b / (ms * 1e-3) / 1e9, 1), "dtype": "f64", "data": "synthetic"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,6 +465,8 @@ def main() -> int:
                     help="512-request serving stream: designs/hour, p50/p99 latency (own JSON line)")
     ap.add_argument("--mbb", action="store_true",
                     help="300x100 MBB, 100 OC iterations: mixed vs FP64 vs Jacobi (own JSON line)")
+    ap.add_argument("--h8", action="store_true",
+                    help="3-D H8 path: 96x48x48 cantilever OC it/s and 3-D matvec (own JSON line)")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--serving-mode", choices=["async", "lockstep"], default="async")
@@ -433,6 +487,9 @@ def main() -> int:
         return 0
     if args.mbb:
         print(json.dumps(mbb_bench(log)), flush=True)
+        return 0
+    if args.h8:
+        print(json.dumps(h8_bench(log)), flush=True)
         return 0
     if args.serving:
         print(json.dumps(serving_bench(log, n=args.requests, seq_sample=args.seq_sample, mode=args.serving_mode,

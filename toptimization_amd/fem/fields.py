@@ -7,8 +7,8 @@ rho_filt, Ke, Ke_inv, edof, is_fixed, diag_count, filt_neighbors,
 filt_weights, filt_n_nb, _dot_result, _max_result``) and constants
 (``N_DOFS, N_ELEM, DPE, KE_ALPHA_REG``). Fields are FP64 (the reference
 uses FP32). The grid shape the HIP kernels need is read back from ``edof``
-(``mesh.grid_from_edof``); ``NELX, NELY`` and the fixed-DOF bitmask
-``FIXMASK`` are set in addition.
+(``mesh.grid_from_edof`` for Q4, ``grid_from_edof_3d`` for H8); ``NELX,
+NELY, NELZ, DIM`` and the fixed-DOF bitmask ``FIXMASK`` are set in addition.
 """
 
 from __future__ import annotations
@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from ..engine import fixed_bitmask, require_gpu
-from ..mesh import grid_from_edof
+from ..mesh import grid_from_edof, grid_from_edof_3d
 
 u = f = r = p = Ap = z = diag = u_prev = None
 rho = rho_new = dc = dc_filt = dv = rho_filt = None
@@ -31,6 +31,8 @@ DPE = 0
 KE_ALPHA_REG = 0.0
 NELX = 0
 NELY = 0
+NELZ = 1
+DIM = 2
 KE_NP = None
 # incremented by every allocate(): device states built from the previous
 # fields (fem/solver.py's cache) are discarded when it changes
@@ -44,12 +46,17 @@ def allocate(n_dofs: int, n_elem: int, dpe: int, max_neighbors: int, Ke_np: np.n
     global u, f, r, p, Ap, z, diag, u_prev, rho, rho_new, dc, dc_filt, dv, rho_filt
     global Ke, Ke_inv, edof, is_fixed, diag_count, filt_neighbors, filt_weights, filt_n_nb
     global _dot_result, _max_result, FIXMASK, N_DOFS, N_ELEM, DPE, KE_ALPHA_REG, NELX, NELY, KE_NP
-    global GENERATION
-    if dpe != 8:
-        raise NotImplementedError("the GPU path covers 2-D Q4 problems (8 DOFs per element) only")
+    global GENERATION, NELZ, DIM
+    if dpe not in (8, 24):
+        raise ValueError(f"dpe must be 8 (Q4) or 24 (H8), got {dpe}")
     dev = require_gpu()
     N_DOFS, N_ELEM, DPE = int(n_dofs), int(n_elem), int(dpe)
-    NELX, NELY = grid_from_edof(edof_np, N_ELEM)
+    if dpe == 8:
+        NELX, NELY = grid_from_edof(edof_np, N_ELEM)
+        NELZ, DIM = 1, 2
+    else:
+        NELX, NELY, NELZ = grid_from_edof_3d(edof_np, N_ELEM)
+        DIM = 3
     z64 = lambda n: torch.zeros(n, dtype=torch.float64, device=dev)  # noqa: E731
     u, f, r, p, Ap, z, diag, u_prev = (z64(N_DOFS) for _ in range(8))
     rho, rho_new, dc, dc_filt, dv, rho_filt = (z64(N_ELEM) for _ in range(6))
@@ -78,6 +85,26 @@ def allocate(n_dofs: int, n_elem: int, dpe: int, max_neighbors: int, Ke_np: np.n
 
 
 def grid() -> tuple[int, int]:
+    """(nelx, nely) of the allocated 2-D grid."""
     if NELX == 0:
         raise RuntimeError("fields.allocate() has not been called")
+    if DIM != 2:
+        raise RuntimeError("the allocated fields are 3-D (use grid3())")
     return NELX, NELY
+
+
+def grid3() -> tuple[int, int, int]:
+    """(nelx, nely, nelz) of the allocated 3-D grid."""
+    if NELX == 0:
+        raise RuntimeError("fields.allocate() has not been called")
+    if DIM != 3:
+        raise RuntimeError("the allocated fields are 2-D (use grid())")
+    return NELX, NELY, NELZ
+
+
+def elem_grid() -> tuple[int, int]:
+    """Element array as (columns, rows) for the element-only kernels (OC,
+    volume): (nelx, nely) in 2-D, (nelx, nely*nelz) in 3-D."""
+    if NELX == 0:
+        raise RuntimeError("fields.allocate() has not been called")
+    return NELX, NELY * NELZ

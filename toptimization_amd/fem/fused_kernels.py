@@ -3,15 +3,20 @@
 
 from __future__ import annotations
 
-from .. import ops
+import torch
+
+from .. import h8, ops
 from . import fields as F
-from .kernels import _dim2, _fixmask, _grid, _scale
+from .kernels import _check_dim, _fixmask, _grid, _grid3, _scale
 
 
 def matvec_and_dot(p_in, Ap_out, rho, Ke, edof, is_fixed, E_min: float, penalty: float,
                    n_elem: int, dim: int) -> float:
-    """Ap = K(rho) p; returns p.Ap (``fused_kernels.py:21-56``)."""
-    _dim2(dim)
+    """Ap = K(rho) p; returns p.Ap (``fused_kernels.py:21-94``)."""
+    if _check_dim(dim) == 3:
+        g = _grid3(Ke, E_min, penalty)
+        _, dot = h8.matvec(g, p_in, _scale(g, rho, torch.float64), _fixmask(is_fixed), Ap=Ap_out, with_dot=True)
+        return float(dot.item())
     g = _grid(Ke, E_min, penalty)
     s = _scale(g, rho, p_in.dtype)
     _, dot = ops.matvec(g, p_in, s, _fixmask(is_fixed), Ap=Ap_out, with_dot=True)

@@ -3,8 +3,8 @@
 Drop-in for ``fem/kernels.py`` of the reference: every function takes the
 same arguments (torch CUDA tensors where the reference takes Taichi fields)
 and runs a HIP kernel of libtopt. Scalar-returning functions return Python
-floats (a device->host read, as in the reference). 2-D only; ``dim == 3``
-raises NotImplementedError.
+floats (a device->host read, as in the reference). ``dim == 2`` runs the Q4
+kernels, ``dim == 3`` the H8 ones (``h8.py``).
 """
 
 from __future__ import annotations
@@ -12,13 +12,34 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .. import ops
+from .. import h8, ops
 from . import fields as F
 
 
 def _dim2(dim: int) -> None:
     if dim != 2:
-        raise NotImplementedError("3-D H8 kernels are not on the GPU path")
+        raise ValueError(f"dim must be 2 here, got {dim}")
+
+
+def _check_dim(dim: int) -> int:
+    if dim not in (2, 3) or dim != F.DIM:
+        raise ValueError(f"dim={dim} does not match the allocated fields (dim {F.DIM})")
+    return dim
+
+
+_grid3_cache: dict = {}
+
+
+def _grid3(Ke, E_min: float, penalty: float):
+    """h8.Grid3 for the current 3-D fields (Ke on the device, workspace)."""
+    nelx, nely, nelz = F.grid3()
+    Ke_np = Ke.detach().cpu().numpy() if isinstance(Ke, torch.Tensor) else np.asarray(Ke)
+    key = (F.GENERATION, nelx, nely, nelz, float(E_min), float(penalty), Ke_np.tobytes())
+    g = _grid3_cache.get(key)
+    if g is None:
+        _grid3_cache.clear()
+        g = _grid3_cache[key] = h8.Grid3(nelx, nely, nelz, Ke_np.reshape(24, 24), E_min, penalty)
+    return g
 
 
 _grid_cache: dict = {}
@@ -45,6 +66,8 @@ def _scale(g, rho, dtype):
     buf = _s_bufs.get(key)
     if buf is None:
         buf = _s_bufs[key] = torch.empty(rho.numel(), dtype=dtype, device=rho.device)
+    if isinstance(g, h8.Grid3):
+        return h8.scale(g, rho.double().contiguous(), out=buf)
     return ops.scale(g, rho, dtype, out=buf)
 
 
@@ -92,7 +115,10 @@ def apply_diag_precond(diag, r, z) -> None:
 def compute_diagonal(diag, rho, Ke, edof, is_fixed, E_min: float, penalty: float, n_elem: int,
                      dim: int) -> None:
     """diag K(rho), fixed -> 1, clamp 1e-20 (``kernels.py:76-140``)."""
-    _dim2(dim)
+    if _check_dim(dim) == 3:
+        g = _grid3(Ke, E_min, penalty)
+        diag.copy_(h8.diagonal(g, _scale(g, rho, torch.float64), _fixmask(is_fixed)))
+        return
     g = _grid(Ke, E_min, penalty)
     s = _scale(g, rho, diag.dtype)
     diag.copy_(ops.diagonal(g, s, _fixmask(is_fixed)))
@@ -101,7 +127,10 @@ def compute_diagonal(diag, rho, Ke, edof, is_fixed, E_min: float, penalty: float
 def compute_matvec(p_in, Ap_out, rho, Ke, edof, is_fixed, E_min: float, penalty: float,
                    n_elem: int, dim: int) -> None:
     """Ap = K(rho) p with identity rows at fixed DOFs (``kernels.py:143-231``)."""
-    _dim2(dim)
+    if _check_dim(dim) == 3:
+        g = _grid3(Ke, E_min, penalty)
+        h8.matvec(g, p_in, _scale(g, rho, torch.float64), _fixmask(is_fixed), Ap=Ap_out)
+        return
     g = _grid(Ke, E_min, penalty)
     s = _scale(g, rho, p_in.dtype)
     ops.matvec(g, p_in, s, _fixmask(is_fixed), Ap=Ap_out)
@@ -110,7 +139,12 @@ def compute_matvec(p_in, Ap_out, rho, Ke, edof, is_fixed, E_min: float, penalty:
 def compute_sensitivity(u, rho, dc, dv, Ke, edof, E_min: float, penalty: float, n_elem: int,
                         dim: int) -> float:
     """dc, dv = 1 and the compliance (``kernels.py:234-327``)."""
-    _dim2(dim)
+    if _check_dim(dim) == 3:
+        g = _grid3(Ke, E_min, penalty)
+        c, dcv = h8.sensitivity(g, u.double().contiguous(), rho.double().contiguous())
+        dc.copy_(dcv)
+        dv.fill_(1.0)
+        return float(c.item())
     g = _grid(Ke, E_min, penalty)
     c, dcv = ops.sensitivity(g, u.double().contiguous(), rho.double().contiguous())
     dc.copy_(dcv)

@@ -64,10 +64,12 @@ def _state_for(E_min, penalty, max_iter, tol, preconditioner, precision):
 def solve(E_min: float, penalty: float, dim: int, max_iter: int = 2000, tol: float = 1e-8,
           preconditioner: str = "jacobi", warm_start: bool = True, *, precision: str = "fp64") -> int:
     """Solve K(F.rho) F.u = F.f; returns the number of CG iterations."""
-    if dim != 2:
-        raise NotImplementedError("3-D H8 solves are not on the GPU path")
     if preconditioner not in PRECONDITIONERS:
         raise ValueError(f"unknown preconditioner {preconditioner!r}")
+    if dim != F.DIM:
+        raise ValueError(f"dim={dim} does not match the allocated fields (dim {F.DIM})")
+    if dim == 3:
+        return _solve_3d(E_min, penalty, max_iter, tol, preconditioner, warm_start)
     key, (st, plan) = _state_for(E_min, penalty, max_iter, tol, preconditioner, precision)
     st.f.copy_(F.f)
     # |f|^2 sets the stopping tolerance and the zero-load test; the caller
@@ -87,3 +89,22 @@ def solve(E_min: float, penalty: float, dim: int, max_iter: int = 2000, tol: flo
     if status[0] != 4:            # zero load returns before u_prev is written (solver.py:60-63)
         F.u_prev.copy_(st.u)
     return int(it[0])
+
+
+def _solve_3d(E_min, penalty, max_iter, tol, preconditioner, warm_start) -> int:
+    """The 3-D solve: the whole PCG on the device (``h8.solve``), with the
+    reference's field semantics (u from u_prev on a warm start, u_prev <- u
+    unless the load is zero)."""
+    import torch
+
+    from .. import h8
+    from .kernels import _fixmask, _grid3, _scale
+    g = _grid3(F.Ke, E_min, penalty)
+    s = _scale(g, F.rho, torch.float64)
+    if warm_start:
+        F.u.copy_(F.u_prev)
+    n, status = h8.solve(g, preconditioner, F.f, _fixmask(F.is_fixed), s, F.u, warm=warm_start, tol=tol,
+                         max_iter=max_iter)
+    if status != 4:            # zero load returns before u_prev is written (solver.py:60-63)
+        F.u_prev.copy_(F.u)
+    return n

@@ -88,3 +88,24 @@ def grid_from_edof(edof: np.ndarray, n_elem: int) -> tuple[int, int]:
     if not np.array_equal(build_edof_2d(probe), edof.astype(np.int32)):
         raise ValueError("edof is not a structured Q4 grid numbering")
     return nelx, nely
+
+
+def grid_from_edof_3d(edof: np.ndarray, n_elem: int) -> tuple[int, int, int]:
+    """Recover (nelx, nely, nelz) from a 3-D structured ``edof`` and check it is
+    one: element 0's local node 3 is (0, 1, 0), node ``nelz+1``, and its node
+    1 is (1, 0, 0), node ``(nely+1)(nelz+1)`` (``mesh.py:91-100``)."""
+    edof = np.asarray(edof)
+    if edof.ndim != 2 or edof.shape != (n_elem, 24):
+        raise ValueError(f"expected a (n_elem, 24) H8 edof, got {edof.shape}")
+    nelz = int(edof[0, 9]) // 3 - 1
+    if nelz <= 0:
+        raise ValueError("edof is not a structured H8 grid numbering")
+    nely = int(edof[0, 3]) // (3 * (nelz + 1)) - 1
+    if nely <= 0 or n_elem % (nely * nelz):
+        raise ValueError("edof is not a structured H8 grid numbering")
+    nelx = n_elem // (nely * nelz)
+    probe = Problem.__new__(Problem)
+    probe.nelx, probe.nely, probe.nelz = nelx, nely, nelz
+    if not np.array_equal(build_edof_3d(probe), edof.astype(np.int32)):
+        raise ValueError("edof is not a structured H8 grid numbering")
+    return nelx, nely, nelz
