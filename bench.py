@@ -15,11 +15,15 @@ summary), slot A alone, the whole 200-iteration run, matvec GDOF/s, the
 plain FP64 GPU path on the same iterations, the 2x2 block-Jacobi path, the
 compliance agreement of the two precisions on one density, and the C
 restatement of the reference algorithm (FP32, Jacobi, as the reference) on
-the host cores as the CPU baseline.
+the host cores as the CPU baseline. The line's "configs" object carries the
+other workloads measured in the same run: the 300x100 MBB, the batch of 64
+cantilevers and the 512-request serving stream (--quick leaves them out).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+       python bench.py --mbb        (300x100 MBB, 100 OC iterations, three paths)
        python bench.py --batch      (64 x 128x64 cantilevers, batched vs one at a time)
        python bench.py --serving    (512-request stream: designs/hour, p50/p99 latency)
+       python bench.py --h8         (3-D 96x48x48 cantilever and the 3-D matvec)
 """
 
 from __future__ import annotations
@@ -130,13 +134,20 @@ def run_path(precision, warmup, steps, log, preconditioner=MAIN_PC):
     ncg = sum(s.n_cg for s in stats)
     log(f"  [{tag}] timed {steps} it: {dt:.3f} s ({steps / dt:.2f} OC it/s), {ncg} CG iterations, "
         f"last c={stats[-1].compliance:.6e}")
-    # the same iterations again with the phases timed by HIP events (the
-    # events add a host sync per iteration, so this pass is not the headline)
-    plan.set_timing(True)
-    stats_t = [plan.iteration()[0] for _ in range(steps)]
-    tm = plan.timing()
-    plan.set_timing(False)
+    # the same OC iterations again, from a fresh state, with the phases timed
+    # by HIP events on the launch stream (event reads between iterations, so
+    # this pass is not the headline); the CG counts must repeat exactly
+    st_t = DesignState(prob)
+    plan_t = Plan([st_t])
+    for _ in range(warmup):
+        plan_t.iteration()
+    plan_t.set_timing(True)
+    stats_t = [plan_t.iteration()[0] for _ in range(steps)]
+    tm = plan_t.timing()
+    plan_t.close()
     tm["n_cg"] = sum(s.n_cg for s in stats_t)
+    if tm["n_cg"] != ncg:
+        raise RuntimeError(f"timed replay took {tm['n_cg']} CG iterations against {ncg}")
     log(f"  [{tag}] device time per OC iteration: setup {tm['setup'] / steps:.3f} ms, PCG "
         f"{tm['pcg'] / steps:.3f} ms ({1e3 * tm['pcg'] / max(tm['n_cg'], 1):.1f} us per CG iteration), "
         f"post {tm['post'] / steps:.3f} ms")
@@ -394,7 +405,10 @@ def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="asy
             "requests": n, "arrival_rate_per_s": rate, "max_active": s["max_active"], "mode": mode,
             "preconditioner": preconditioner,
             "engine_steps": s["engine_steps"], "sequential_designs_per_hour": None if seq is None else round(seq, 1),
-            "sequential_sample": seq_sample, "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
+            "sequential_sample": seq_sample,
+            "sequential_note": (f"the first {seq_sample} requests of the same stream served one at a time "
+                                "(max_active 1), designs/hour of that sample"),
+            "dtype": "f32 vectors + f64 dots/refresh", "data": "synthetic"}
 
 
 def h8_bench(log, nelx=96, nely=48, nelz=48, iters=6, reps=50):
@@ -455,6 +469,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline only: skip the config 1, 3 and 4 measurements of the default line")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--probe", action="store_true",
                     help="profiling driver: three multigrid OC iterations, one block-Jacobi OC iteration, "
@@ -539,6 +555,13 @@ def main() -> int:
     mvb_ms, mvb_gdofs, mvb_gbs = matvec_gdofs(8192, 4096, log, reps=20)
     _, mvd_gdofs, mvd_gbs = matvec_gdofs(8192, 4096, log, reps=10, dtype="f64")
     cpu = None if args.no_cpu else cpu_baseline(log)
+    # the other benchmark configurations, measured in the same run (their
+    # own sub-objects; the headline value above is config 2's)
+    configs = None
+    if not args.quick:
+        configs = {"mbb_300x100": mbb_bench(log),
+                   "batch_64x128x64": batch_bench(log, seq_sample=8),
+                   "serving_512": serving_bench(log, n=512, seq_sample=32)}
 
     value = args.steps / mixed["seconds"]
     fp64_value = args.steps / fp64["seconds"]
@@ -590,7 +613,8 @@ def main() -> int:
                      "frac": round(gbs_cg / HBM_PEAK_GBS, 4),
                      "traffic": profile_traffic(PROFILE_MG, "cg_iteration"),
                      "bytes_per_cg_iteration": b_cg, "us_per_cg_iteration": round(us_cg, 2),
-                     "timing": "HIP events around the PCG chunks of the timed OC iterations (launch stream)"},
+                     "timing": "HIP events on the launch stream around the PCG chunks of the timed OC "
+                               "iterations, replayed from a fresh state (same CG count)"},
         "phase_ms_per_oc_iteration": {k: round(tm[k] / args.steps, 4) for k in ("setup", "pcg", "post")},
         "slot_a_roofline": {"kernel": "k_slot_a<float> (PCG slot A)", "achieved": round(gbs_a, 1),
                             "frac": round(gbs_a / HBM_PEAK_GBS, 4),
@@ -599,6 +623,7 @@ def main() -> int:
         "full_run_200": {"seconds": round(full["seconds"], 3), "oc_its_per_s": round(200 / full["seconds"], 2),
                          "cg_iterations": full["n_cg"], "final_compliance": full["final_compliance"]},
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(line), flush=True)
     return 0
