@@ -119,9 +119,10 @@ struct Layout {
 // TOPT_BUF_MG region (level 0 keeps x0 and r1 there; its b is the PCG r and
 // its 1/diag the pre buffer).
 struct MgLayout {
-  int nlev = 0, ncd = 0;
+  int nlev = 0, ncd = 0, lt = 1;
   int nx[TOPT_MG_MAXLEV], ny[TOPT_MG_MAXLEV];
   int64_t K[TOPT_MG_MAXLEV], x[TOPT_MG_MAXLEV], b[TOPT_MG_MAXLEV], r[TOPT_MG_MAXLEV], d[TOPT_MG_MAXLEV];
+  int64_t st[TOPT_MG_MAXLEV];
   int64_t inv = 0, total = 0;
 };
 
@@ -140,17 +141,30 @@ int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
   }
   int64_t o = 0;
   auto take = [&](int64_t bytes) { const int64_t at = o; o += align256(bytes); return at; };
+  const int L = M.nlev - 1;
+  // first level of the single-workgroup tail: small enough for one node
+  // per thread, and all tail levels' vectors within its LDS buffers
+  auto nodes = [&](int l) { return int64_t(M.nx[l] + 1) * (M.ny[l] + 1); };
+  auto tail_sum = [&](int from) {
+    int64_t t = 0;
+    for (int l = from; l <= L; ++l) t += nodes(l);
+    return t;
+  };
+  M.lt = 1;
+  while (M.lt < L && (nodes(M.lt) > TOPT_MG_TAIL || tail_sum(M.lt) > TOPT_MG_TAIL_SUM)) ++M.lt;
+  if (tail_sum(M.lt) > TOPT_MG_TAIL_SUM) return fail(TOPT_EINVAL, "multigrid tail exceeds its LDS buffers");
   // level 0 borrows b and 1/diag from the PCG (r, Jacobi diagonal); x is
   // stored at the coarsest level only (the sweeps form the pre-smoothed
-  // iterate from b), r at levels 1 .. L-1 (the corrected iterate)
-  const int L = M.nlev - 1;
+  // iterate from b), r at levels 1 .. L-1 (the corrected iterate); the
+  // tail's sweep levels lt .. L-1 also keep their node records
   for (int l = 0; l < M.nlev; ++l) {
-    const int64_t nn = int64_t(M.nx[l] + 1) * (M.ny[l] + 1), ne = int64_t(M.nx[l]) * M.ny[l];
+    const int64_t nn = nodes(l), ne = int64_t(M.nx[l]) * M.ny[l];
     M.K[l] = l > 0 ? take(ne * 36 * ts) : -1;
     M.x[l] = l == L ? take(nn * 2 * ts) : -1;
     M.r[l] = l > 0 && l < L ? take(nn * 2 * ts) : -1;
     M.b[l] = l > 0 ? take(nn * 2 * ts) : -1;
     M.d[l] = l > 0 ? take(nn * 2 * ts) : -1;
+    M.st[l] = l >= M.lt && l < L ? take(nn * TOPT_MG_ST * ts) : -1;
   }
   M.ncd = 2 * (M.nx[L] + 1) * (M.ny[L] + 1);
   M.inv = take(int64_t(M.ncd) * M.ncd * 8);
@@ -322,19 +336,7 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
     char* m = (char*)at(TOPT_BUF_MG);
     P.mg_nlev = M.nlev;
     P.mg_ncd = M.ncd;
-    const int L = M.nlev - 1;
-    // first level of the single-workgroup tail: small enough for one node
-    // per thread, and all tail levels' vectors within its LDS buffers
-    auto nodes = [&](int l) { return int64_t(M.nx[l] + 1) * (M.ny[l] + 1); };
-    auto tail_sum = [&](int from) {
-      int64_t t = 0;
-      for (int l = from; l <= L; ++l) t += nodes(l);
-      return t;
-    };
-    int lt = 1;
-    while (lt < L && (nodes(lt) > TOPT_MG_TAIL || tail_sum(lt) > TOPT_MG_TAIL_SUM)) ++lt;
-    if (tail_sum(lt) > TOPT_MG_TAIL_SUM) return fail(TOPT_EINVAL, "multigrid tail exceeds its LDS buffers");
-    P.mg_lt = lt;
+    P.mg_lt = M.lt;
     P.mg_inv = (double*)(m + M.inv);
     for (int l = 0; l < M.nlev; ++l) {
       MgLevel& V = P.mg[l];
@@ -345,6 +347,7 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
       V.r = M.r[l] >= 0 ? (void*)(m + M.r[l]) : nullptr;
       V.b = l > 0 ? (void*)(m + M.b[l]) : P.r;
       V.dinv = l > 0 ? (void*)(m + M.d[l]) : P.pre;
+      V.st = M.st[l] >= 0 ? (void*)(m + M.st[l]) : nullptr;
     }
   }
   return TOPT_OK;

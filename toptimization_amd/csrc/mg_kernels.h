@@ -18,11 +18,15 @@
 // omega 0.6, a 300x100 bridge after 20 OC iterations took 69 iterations
 // against 1,738 with Jacobi; the 1200x400 bridge takes 15-60 per solve).
 #pragma once
+#include <type_traits>
+
 #include "fem_kernels.h"
 
 namespace topt {
 
+#ifndef TOPT_MG_DENSE
 #define TOPT_MG_DENSE 128      // max DOFs of the coarsest level (inverted in LDS, FP64: 128 KB)
+#endif
 // Jacobi damping. Every element matrix (s_e Ke, with or without fixed-DOF
 // masking, and every Galerkin agglomerate of them) has lambda_max(D_e^-1
 // K_e) = 2.889 for the unit-square Q4 with nu = 0.3, so lambda_max(D^-1 K)
@@ -138,6 +142,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, 
 }
 
 // 1/diag of the assembled level-l operator (l >= 1); 0 on decoupled DOFs.
+// On the tail's sweep levels also the node records (TOPT_MG_ST): the node's
+// two rows of the assembled operator, each coefficient the FP64 sum of its
+// element entries in ascending element order. The entry coupling nodes i and
+// j sums the same (symmetric) element entries in the same order from either
+// side, so the assembled operator is bitwise symmetric.
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, const int2* map, ProbDev single, int l) {
   int lb;
@@ -147,7 +156,28 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
   if (n >= (long)(nx + 1) * (ny + 1)) return;
   const int ix = (int)(n / (ny + 1)), iy = (int)(n - (long)ix * (ny + 1));
   const GLB T* K = gp((const T*)P.mg[l].K);
-  double dx = 0.0, dy = 0.0;
+  if (P.mg[l].st == nullptr) {
+    double dx = 0.0, dy = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int ex = ix - 1 + a, ey = iy - 1 + b;
+        if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+        const int me = mg_me(a, b);
+        const GLB T* k = K + mg_kel(ex, ey, ny);
+        dx += (double)k[(long)sym8(2 * me, 2 * me) * ny];
+        dy += (double)k[(long)sym8(2 * me + 1, 2 * me + 1) * ny];
+      }
+    vec2<T> d;
+    d.x = dx > 0.0 ? (T)(1.0 / dx) : T(0);
+    d.y = dy > 0.0 ? (T)(1.0 / dy) : T(0);
+    gp((vec2<T>*)P.mg[l].dinv)[n] = d;
+    return;
+  }
+  double A[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) A[i] = 0.0;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -156,13 +186,39 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
       if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
       const int me = mg_me(a, b);
       const GLB T* k = K + mg_kel(ex, ey, ny);
-      dx += (double)k[(long)sym8(2 * me, 2 * me) * ny];
-      dy += (double)k[(long)sym8(2 * me + 1, 2 * me + 1) * ny];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        // element corner kk is neighbour (a - 1 + lx, b - 1 + ly) of the node
+        const int q = 3 * (a + mg_lx(kk)) + (b + mg_ly(kk));
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) A[18 * d + 2 * q + c] += (double)k[(long)sym8(2 * me + d, 2 * kk + c) * ny];
+      }
     }
+  // the diagonal entries are the sums k_mg_diag forms above, in the same order
+  const double dx = A[2 * 4], dy = A[18 + 2 * 4 + 1];
   vec2<T> d;
   d.x = dx > 0.0 ? (T)(1.0 / dx) : T(0);
   d.y = dy > 0.0 ? (T)(1.0 / dy) : T(0);
   gp((vec2<T>*)P.mg[l].dinv)[n] = d;
+  typedef T Q __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int W = 16 / (int)sizeof(T);
+  T rec[TOPT_MG_ST];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) rec[i] = (T)A[i];
+  rec[36] = d.x;
+  rec[37] = d.y;
+  rec[38] = T(0);
+  rec[39] = T(0);
+  GLB Q* o = gp((Q*)P.mg[l].st) + n * (TOPT_MG_ST / W);
+#pragma unroll
+  for (int i = 0; i < TOPT_MG_ST / W; ++i) {
+    Q v;
+#pragma unroll
+    for (int w = 0; w < W; ++w) v[w] = rec[i * W + w];
+    o[i] = v;
+  }
 }
 
 // Inverse of the coarsest operator (one workgroup, in LDS), formed per solve.
@@ -204,7 +260,8 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
   __shared__ double Ld[TOPT_MG_DENSE], D0[TOPT_MG_DENSE];
   __shared__ int px[TOPT_MG_DENSE / 2], py[TOPT_MG_DENSE / 2];
   __shared__ short2 tri[BW * (BW + 1) / 2];   // (ri, rj), rj <= ri: the trailing band of a column
-  static_assert(36 * 49 <= TOPT_MG_DENSE * (TOPT_MG_BAND + 1), "coarsest element matrices fit the band array");
+  // the coarsest level has fewer than TOPT_MG_DENSE / 2 elements
+  static_assert(36 * (TOPT_MG_DENSE / 2) <= TOPT_MG_DENSE * (TOPT_MG_BAND + 1), "coarsest element matrices fit the band array");
   const double tau = sizeof(T) == 4 ? 1e-3 : 1e-12;
   const int t = (int)threadIdx.x;
   // element matrices (<= 49 elements: <= 64 nodes) and node coordinates of
@@ -372,8 +429,8 @@ __device__ __forceinline__ PD& mg_entry(const ProbDev* tab, const int2* map, int
 __device__ __forceinline__ const void* mg_xup(PD& P, int l) {
   return l == P.mg_nlev - 1 ? P.mg[l].x : P.mg[l].r;
 }
-// The per-node bodies below (restriction, residual, post-smoothing) serve the
-// tail kernel, whose node vectors are in LDS.
+// The per-node restriction and prolongation below also serve the tail
+// kernel, whose node vectors are in LDS.
 
 // Restriction b_{l+1} = P^T r_l at coarse node N (tail levels, l >= 1).
 template <typename T, typename VR>
@@ -415,40 +472,6 @@ __device__ __forceinline__ d2v mg_prolong(VC xc, int cny, int i, int j) {
   return 0.5 * (v + w);
 }
 
-// Sum over node (ix, iy)'s elements of its two rows of K_E times x_E, where
-// xv(dx, dy) gives x at the neighbour (ix + dx, iy + dy) (level >= 1); K
-// holds the level's element matrices (global memory or an LDS copy).
-template <typename T, typename KP, typename XF>
-__device__ __forceinline__ d2v mg_apply_rows(PD& P, int l, KP K, int ix, int iy, XF xv) {
-  const int nx = P.mg[l].nx, ny = P.mg[l].ny;
-  double ax = 0.0, ay = 0.0;
-  // branch-free: elements outside the grid are read at a clamped address and
-  // weighted 0, so all the loads can be in flight together
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int ex = ix - 1 + a, ey = iy - 1 + b;
-      const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
-      const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
-      const int me = mg_me(a, b);
-      const KP k = K + mg_kel(cx, cy, ny);
-      double tx = 0.0, ty = 0.0;
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        // element corner kk is neighbour (a - 1 + lx, b - 1 + ly) of the node
-        const d2v xv_ = xv(a - 1 + mg_lx(kk), b - 1 + mg_ly(kk));
-        tx = fma((double)k[(long)sym8(2 * me, 2 * kk) * ny], xv_.x, tx);
-        tx = fma((double)k[(long)sym8(2 * me, 2 * kk + 1) * ny], xv_.y, tx);
-        ty = fma((double)k[(long)sym8(2 * me + 1, 2 * kk) * ny], xv_.x, ty);
-        ty = fma((double)k[(long)sym8(2 * me + 1, 2 * kk + 1) * ny], xv_.y, ty);
-      }
-      ax += ok ? tx : 0.0;
-      ay += ok ? ty : 0.0;
-    }
-  return d2v{ax, ay};
-}
-
 // Pre-smoothed iterate x = omega D^-1 b (from zero) at a node with 1/diag dd.
 template <typename T>
 __device__ __forceinline__ vec2<T> mg_jac(vec2<T> dd, vec2<T> bb) {
@@ -456,22 +479,6 @@ __device__ __forceinline__ vec2<T> mg_jac(vec2<T> dd, vec2<T> bb) {
   o.x = (T)(TOPT_MG_OMEGA * (double)dd.x * (double)bb.x);
   o.y = (T)(TOPT_MG_OMEGA * (double)dd.y * (double)bb.y);
   return o;
-}
-// Residual r = b - K x at node n of level l (tail: x, b, r in LDS).
-template <typename T, typename KP>
-__device__ __forceinline__ void mg_resid_p(PD& P, int l, KP K, long n, const vec2<T>* b, const vec2<T>* x,
-                                           vec2<T>* r) {
-  const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
-  const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-  const d2v ax = mg_apply_rows<T>(P, l, K, ix, iy, [&](int dx, int dy) {
-    const vec2<T> v = x[nbr_node(ix, iy, dx, dy, nx, ny)];   // clamped: meets void only
-    return d2v{(double)v.x, (double)v.y};
-  });
-  const vec2<T> bn = b[n];
-  vec2<T> ro;
-  ro.x = (T)((double)bn.x - ax.x);
-  ro.y = (T)((double)bn.y - ax.y);
-  r[n] = ro;
 }
 
 // ------------------------------------------------- down sweep + restriction
@@ -659,24 +666,6 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const ProbDev*
                                  gp((const vec2<T>*)P.pre), ek, gp((vec2<T>*)P.mg[1].b));
 }
 
-// Post-smoothing at node n of level l (tail): out = x' + omega D^-1 (b - K x'),
-// x' = x + P x_{l+1} already formed (in LDS), dd = 1/diag at n.
-template <typename T, typename KP>
-__device__ __forceinline__ void mg_smooth_p(PD& P, int l, KP K, long n, const vec2<T>* xp, const vec2<T>* b,
-                                            vec2<T> dd, vec2<T>* out) {
-  const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
-  const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-  const d2v ax = mg_apply_rows<T>(P, l, K, ix, iy, [&](int dx, int dy) {
-    const vec2<T> v = xp[nbr_node(ix, iy, dx, dy, nx, ny)];
-    return d2v{(double)v.x, (double)v.y};
-  });
-  const vec2<T> bn = b[n], xn = xp[n];
-  vec2<T> o;
-  o.x = (T)((double)xn.x + TOPT_MG_OMEGA * (double)dd.x * ((double)bn.x - ax.x));
-  o.y = (T)((double)xn.y + TOPT_MG_OMEGA * (double)dd.y * ((double)bn.y - ax.y));
-  out[n] = o;
-}
-
 // Up sweep at levels 1 .. lt-1 on tiles of UpTile::TX x UpTile::TY nodes
 // (one per thread): x' = omega D^-1 b + P x_{l+1} (the pre-smoothed iterate
 // formed again from b) on the tile's patch in LDS, the element rows of each
@@ -766,197 +755,237 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
 // Levels lt..L (the small ones) in one workgroup: down sweeps and
 // restrictions to the coarsest level, its dense solve, up sweeps back to lt.
 // The levels' x, b and r live in LDS, so a phase hands its results to the
-// next through LDS and a workgroup barrier. The element matrices of levels
-// lt..L-1 are copied to LDS at the start (coalesced), as many levels as fit
-// from the coarsest up; a level that does not fit is read from global
-// memory (one CU reading 64 scattered entries per node is what bounds a
-// sweep otherwise). 1/diag and the coarse inverse are read from global
-// memory. The kernel starts from b_lt (written by the fused sweep launched
-// before it), and level lt's corrected iterate goes back to global memory
-// (mg_xup) for the up sweep after it.
-// One workgroup per problem.
+// next through LDS and a workgroup barrier. Every node of the sweep levels
+// lt..L-1 has a slot (levels in order, nodes in order) in the LDS vectors,
+// and slot g belongs to thread g mod TOPT_MG_TAIL_BLOCK: the thread applies
+// the node's two rows of the assembled level operator from its record
+// (written by k_mg_diag), 18 neighbour values per row. With FP32 vectors a
+// thread's first record and its share of the coarse inverse come into
+// registers on entry, all loads issued together, so the later phases read
+// only LDS; with FP64 (no registers to spare) each phase reads them from
+// global memory. The kernel starts from b_lt (written by the fused sweep
+// launched before it), and level lt's corrected iterate goes back to global
+// memory (mg_xup) for the up sweep after it. One workgroup per problem.
 #ifndef TOPT_MG_TAIL
-#define TOPT_MG_TAIL 1024        // levels with at most this many nodes (one per thread:
-#endif                           // each phase is one round trip)
+#define TOPT_MG_TAIL 1024        // levels with at most this many nodes
+#endif
 #define TOPT_MG_TAIL_BLOCK 1024
 #ifndef TOPT_MG_TAIL_SUM
 #define TOPT_MG_TAIL_SUM 2560    // nodes of all tail levels together (LDS: 3 vectors each)
 #endif
-// Element matrices held in LDS: whole elements in what the 160 KB of LDS
-// leaves beside the three vectors and the level offsets (FP32: 707
-// elements, FP64: 140).
-template <typename T>
-struct TailK {
-  static constexpr int N = (163840 - 3 * TOPT_MG_TAIL_SUM * 2 * (int)sizeof(T) - 512) / (36 * (int)sizeof(T)) * 36;
+
+// Slot g of the tail: its level, node, node position, the level's offset in
+// the LDS vectors and its node records.
+struct TailSlot {
+  int l, n, ix, iy, nx, ny, off;
+  const void* st;
 };
+__device__ __forceinline__ TailSlot tail_slot(PD& P, int lt, int L, int g) {
+  TailSlot s{lt, g, 0, 0, P.mg[lt].nx, P.mg[lt].ny, 0, P.mg[lt].st};
+  int acc = 0;
+  for (int k = lt; k < L; ++k) {   // uniform loop over the levels, per-lane selects
+    if (g >= acc) {
+      s.l = k;
+      s.n = g - acc;
+      s.nx = P.mg[k].nx;
+      s.ny = P.mg[k].ny;
+      s.off = acc;
+      s.st = P.mg[k].st;
+    }
+    acc += (P.mg[k].nx + 1) * (P.mg[k].ny + 1);
+  }
+  s.ix = s.n / (s.ny + 1);
+  s.iy = s.n - s.ix * (s.ny + 1);
+  return s;
+}
+template <typename T>
+__device__ __forceinline__ void tail_rec_load(const TailSlot& s, T (&rec)[TOPT_MG_ST]) {
+  typedef T Q __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int W = 16 / (int)sizeof(T);
+  const GLB Q* src = gp((const Q*)s.st) + (long)s.n * (TOPT_MG_ST / W);
+#pragma unroll
+  for (int i = 0; i < TOPT_MG_ST / W; ++i) {
+    const Q v = src[i];
+#pragma unroll
+    for (int w = 0; w < W; ++w) rec[i * W + w] = v[w];
+  }
+}
+// The node's two rows of the level operator applied to x (the level's LDS
+// vector); neighbours outside the grid have zero coefficients (read clamped).
+// ra(i) gives record entry i; SEQ walks the neighbours one at a time (a
+// record read from global memory while the first one stays in registers).
+template <typename T, bool SEQ, typename RA>
+__device__ __forceinline__ d2v tail_apply(const TailSlot& s, RA ra, const vec2<T>* x) {
+  double ax = 0.0, ay = 0.0;
+  auto step = [&](int q) {
+    const vec2<T> v = x[nbr_node(s.ix, s.iy, q / 3 - 1, q % 3 - 1, s.nx, s.ny)];
+    ax = fma((double)ra(2 * q), (double)v.x, ax);
+    ax = fma((double)ra(2 * q + 1), (double)v.y, ax);
+    ay = fma((double)ra(18 + 2 * q), (double)v.x, ay);
+    ay = fma((double)ra(18 + 2 * q + 1), (double)v.y, ay);
+  };
+  if constexpr (SEQ) {
+#pragma unroll 1
+    for (int q = 0; q < 9; ++q) step(q);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) step(q);
+  }
+  return d2v{ax, ay};
+}
+
 template <typename T>
 __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
+  constexpr int NB = TOPT_MG_TAIL_BLOCK;
+  constexpr bool REG = sizeof(T) == 4;
+  constexpr int G = NB / TOPT_MG_DENSE;   // thread groups of the dense solve
+  constexpr int JD = TOPT_MG_DENSE / G;   // its columns per thread
+  static_assert(NB % TOPT_MG_DENSE == 0, "tail block: whole groups of coarse rows");
   __shared__ vec2<T> sx[TOPT_MG_TAIL_SUM], sb[TOPT_MG_TAIL_SUM], sr[TOPT_MG_TAIL_SUM];
-  __shared__ __align__(16) T skb[TailK<T>::N];
-  __shared__ int off[TOPT_MG_MAXLEV], koff[TOPT_MG_MAXLEV];   // level offsets in the vectors and skb
-  constexpr int G = TOPT_MG_TAIL_BLOCK / TOPT_MG_DENSE;
-  static_assert(TOPT_MG_TAIL_BLOCK % TOPT_MG_DENSE == 0, "tail block: whole groups of coarse rows");
   static_assert(sizeof(sr) >= sizeof(double) * G * TOPT_MG_DENSE, "dense partial sums fit in sr");
   int lb;
   PD& P = mg_entry(tab, map, blockIdx.x, lb);
   if (mg_stopped(P, par)) return;
-  const int lt = P.mg_lt;
-  const int L = P.mg_nlev - 1;
-  auto nodes = [&](int l) { return (long)(P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
-  // The element matrices that fit in skb are those of levels ls..L-1
-  // (coarsest first; a level that does not fit is followed by larger ones),
-  // level L-1 at offset 0. Every thread knows them, so the first phase (b of
-  // level lt, which starts every vector at offset 0, its 1/diag, and the
-  // copies into skb) needs no offset from thread 0.
-  int ls = L;
-  {
-    int k = 0;
-    while (ls > lt) {
-      const int ne = P.mg[ls - 1].nx * P.mg[ls - 1].ny * 36;
-      if (k + ne > TailK<T>::N) break;
-      k += ne;
-      --ls;
+  const int lt = P.mg_lt, L = P.mg_nlev - 1;
+  const int t = (int)threadIdx.x;
+  auto nodes = [&](int l) { return (P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
+  int tot = 0;   // sweep slots; level L's vectors follow them
+  for (int l = lt; l < L; ++l) tot += nodes(l);
+  const int nL = nodes(L), N = P.mg_ncd;
+  // entry: the first record (FP32)
+  const TailSlot s0 = tail_slot(P, lt, L, t < tot ? t : 0);
+  T rec0[TOPT_MG_ST];
+  if (REG && t < tot) tail_rec_load<T>(s0, rec0);
+  const int row = t % TOPT_MG_DENSE, grp = t / TOPT_MG_DENSE;
+  // f(g, slot) for each slot of level l held by this thread
+  auto each_slot = [&](int l, auto&& f) {
+    for (int g = t; g < tot; g += NB) {
+      const TailSlot s = g == t ? s0 : tail_slot(P, lt, L, g);
+      if (s.l == l) f(g, s);
     }
-  }
-  if (threadIdx.x == 0) {
-    int o = 0;
-    for (int l = lt; l <= L; ++l) {
-      off[l] = o;
-      o += (int)nodes(l);
+  };
+  // h(ra) for slot g, ra(i) its record entry i: the registers for the first
+  // slot (FP32), else global memory
+  auto with_rec = [&](int g, const TailSlot& s, auto&& h) {
+    if (REG && g == t) {
+      h([&](int i) { return rec0[i]; }, std::integral_constant<bool, false>{});
+    } else {
+      const GLB T* rp = gp((const T*)s.st) + (long)s.n * TOPT_MG_ST;
+      h([=](int i) { return rp[i]; }, std::integral_constant<bool, REG>{});
     }
-    int k = 0;
-    for (int l = L - 1; l >= lt; --l) {
-      koff[l] = l >= ls ? k : -1;
-      if (l >= ls) k += P.mg[l].nx * P.mg[l].ny * 36;
+  };
+  auto dinv_of = [&](int g, const TailSlot& s) {
+    vec2<T> d;
+    if (REG && g == t) {
+      d.x = rec0[36];
+      d.y = rec0[37];
+    } else {
+      d = gp((const vec2<T>*)P.mg[s.l].dinv)[s.n];
     }
-  }
-  // b of level lt, formed by the fused sweep of level lt-1, and x_lt
-  for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) {
-    const vec2<T> bb = gp((const vec2<T>*)P.mg[lt].b)[n];
-    sb[n] = bb;
-    if (lt < L) sx[n] = mg_jac<T>(gp((const vec2<T>*)P.mg[lt].dinv)[n], bb);
-  }
-  // the element matrices that fit, in 16-byte pieces (a level's matrices
-  // start 256-byte aligned and span 36 ne entries, a whole number of pieces),
-  // each thread's loads issued before its stores
-  {
-    int k = 0;
-    for (int l = L - 1; l >= ls; --l) {
-      typedef unsigned Q __attribute__((ext_vector_type(4)));
-      constexpr int S = (TailK<T>::N * (int)sizeof(T) / 16 + TOPT_MG_TAIL_BLOCK - 1) / TOPT_MG_TAIL_BLOCK;
-      const GLB Q* src = gp((const Q*)P.mg[l].K);
-      Q* dst = (Q*)(skb + k);
-      const int ne36 = P.mg[l].nx * P.mg[l].ny * 36;
-      const int n16 = ne36 * (int)sizeof(T) / 16;
-      Q v[S];
-#pragma unroll
-      for (int q = 0; q < S; ++q) {
-        const int u = (int)threadIdx.x + q * TOPT_MG_TAIL_BLOCK;
-        v[q] = src[u < n16 ? u : n16 - 1];
-      }
-#pragma unroll
-      for (int q = 0; q < S; ++q) {
-        const int u = (int)threadIdx.x + q * TOPT_MG_TAIL_BLOCK;
-        if (u < n16) dst[u] = v[q];
-      }
-      k += ne36;
-    }
+    return d;
+  };
+  // b of level lt (formed by the fused sweep of level lt-1) and x_lt
+  const GLB vec2<T>* blt = gp((const vec2<T>*)P.mg[lt].b);
+  if (lt < L) {
+    each_slot(lt, [&](int g, const TailSlot& s) {
+      const vec2<T> bb = blt[s.n];
+      sb[g] = bb;
+      sx[g] = mg_jac<T>(dinv_of(g, s), bb);
+    });
+  } else {
+    for (int n = t; n < nL; n += NB) sb[n] = blt[n];
   }
   __syncthreads();
-  // down: r_l = b_l - K x_l (x_l = omega D^-1 b_l formed with b_l), then
-  // b_{l+1} = P^T r_l and x_{l+1} in one phase
-  // (a tail level has at most one node per thread: with FP32 vectors each
-  // thread fetches the 1/diag its next phase needs before the barrier in
-  // between; FP64 has no registers to spare for it)
-  constexpr bool PF = sizeof(T) == 4;
+  // down: r_l = b_l - K x_l (x_l = omega D^-1 b_l), then b_{l+1} = P^T r_l
+  // and x_{l+1}
+  int ol = 0;
   for (int l = lt; l < L; ++l) {
-    const vec2<T>* b = (const vec2<T>*)(sb + off[l]);
-    const vec2<T>* x = (const vec2<T>*)(sx + off[l]);
-    vec2<T>* r = (vec2<T>*)(sr + off[l]);
-    const GLB vec2<T>* dnx = gp((const vec2<T>*)P.mg[l + 1].dinv);
-    vec2<T> dnext = vec2<T>{T(0), T(0)};
-    if (PF && l + 1 < L && threadIdx.x < nodes(l + 1)) dnext = dnx[threadIdx.x];
-    if (koff[l] >= 0) {
-      const T* K = skb + koff[l];
-      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_resid_p<T>(P, l, K, n, b, x, r);
+    each_slot(l, [&](int g, const TailSlot& s) {
+      with_rec(g, s, [&](auto ra, auto seq) {
+        const d2v ax = tail_apply<T, decltype(seq)::value>(s, ra, sx + ol);
+        const vec2<T> bn = sb[g];
+        vec2<T> ro;
+        ro.x = (T)((double)bn.x - ax.x);
+        ro.y = (T)((double)bn.y - ax.y);
+        sr[g] = ro;
+      });
+    });
+    __syncthreads();
+    const vec2<T>* rl = sr + ol;
+    if (l + 1 < L) {
+      each_slot(l + 1, [&](int g, const TailSlot& s) {
+        const vec2<T> bc = mg_restrict_p<T>(P, l, s.n, rl);
+        sb[g] = bc;
+        sx[g] = mg_jac<T>(dinv_of(g, s), bc);
+      });
     } else {
-      const GLB T* K = gp((const T*)P.mg[l].K);
-      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) mg_resid_p<T>(P, l, K, n, b, x, r);
+      for (int n = t; n < nL; n += NB) sb[tot + n] = mg_restrict_p<T>(P, l, n, rl);
     }
     __syncthreads();
-    for (long n = threadIdx.x; n < nodes(l + 1); n += TOPT_MG_TAIL_BLOCK) {
-      const vec2<T> bc = mg_restrict_p<T>(P, l, n, (const vec2<T>*)(sr + off[l]));
-      sb[off[l + 1] + n] = bc;
-      if (l + 1 < L) sx[off[l + 1] + n] = mg_jac<T>(PF && n < TOPT_MG_TAIL_BLOCK ? dnext : dnx[n], bc);
-    }
-    __syncthreads();
+    ol += nodes(l);
   }
   {
-    // x = A^-1 b; A is symmetric, so row `row` is read as column `row`
-    // (consecutive threads, consecutive addresses). Thread group g takes
-    // columns j = g mod G, with every load issued unconditionally (rows past
-    // N read row N-1 and weigh 0), then the groups' partial sums meet in LDS
-    // (in sr, free between the restrictions and the up sweeps).
+    // x_L = A^-1 b_L: thread group grp sums columns j = grp mod G, then the
+    // groups' partial sums meet in LDS (in sr, free between the restrictions
+    // and the up sweeps)
     double* dpart = (double*)sr;
-    const int N = P.mg_ncd;
-    const T* b = (const T*)(sb + off[L]);
-    const int row = threadIdx.x % TOPT_MG_DENSE, g = threadIdx.x / TOPT_MG_DENSE;
+    const T* bL = (const T*)(sb + tot);
     if (row < N) {
+      // A is symmetric, so row `row` is read as column `row` (consecutive
+      // threads, consecutive addresses); columns past N read column N-1 and
+      // weigh 0
       const GLB double* A = gp(P.mg_inv) + row;
       double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < TOPT_MG_DENSE / G; ++k) {
-        const int j = g + k * G;
+      for (int k = 0; k < JD; ++k) {
+        const int j = grp + k * G;
         const int jc = j < N ? j : N - 1;
         const double a = A[(long)jc * N];
-        acc = fma(j < N ? a : 0.0, (double)b[jc], acc);
+        acc = fma(j < N ? a : 0.0, (double)bL[jc], acc);
       }
-      dpart[g * TOPT_MG_DENSE + row] = acc;
+      dpart[grp * TOPT_MG_DENSE + row] = acc;
     }
     __syncthreads();
-    if (g == 0 && row < N) {
+    if (grp == 0 && row < N) {
       double acc = dpart[row];
 #pragma unroll
       for (int h = 1; h < G; ++h) acc += dpart[h * TOPT_MG_DENSE + row];
-      ((T*)(sx + off[L]))[row] = (T)acc;
+      ((T*)(sx + tot))[row] = (T)acc;
     }
   }
   __syncthreads();
   // up: x' = x + P x_{l+1} in place, then the post-smoothing into r
   for (int l = L - 1; l >= lt; --l) {
-    const vec2<T>* xc = (const vec2<T>*)(l + 1 == L ? sx + off[L] : sr + off[l + 1]);
-    vec2<T>* x = (vec2<T>*)(sx + off[l]);
-    const int nny = P.mg[l].ny + 1, cny = P.mg[l + 1].ny;
-    const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
-    vec2<T> dcur = vec2<T>{T(0), T(0)};
-    if (PF && threadIdx.x < nodes(l)) dcur = dv[threadIdx.x];
-    for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK) {
-      const int ix = (int)(n / nny), iy = (int)(n - (long)ix * nny);
-      const vec2<T> v = x[n];
-      const d2v c = mg_prolong<T>(xc, cny, ix, iy);
+    const int oc = ol;   // level l+1's offset
+    ol -= nodes(l);
+    const vec2<T>* xc = l + 1 == L ? sx + oc : sr + oc;
+    const int cny = P.mg[l + 1].ny;
+    each_slot(l, [&](int g, const TailSlot& s) {
+      const vec2<T> v = sx[g];
+      const d2v c = mg_prolong<T>(xc, cny, s.ix, s.iy);
       vec2<T> o;
       o.x = (T)((double)v.x + c.x);
       o.y = (T)((double)v.y + c.y);
-      x[n] = o;
-    }
+      sx[g] = o;
+    });
     __syncthreads();
-    const vec2<T>* b = (const vec2<T>*)(sb + off[l]);
-    vec2<T>* o = (vec2<T>*)(sr + off[l]);
-    if (koff[l] >= 0) {
-      const T* K = skb + koff[l];
-      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
-        mg_smooth_p<T>(P, l, K, n, x, b, PF && n < TOPT_MG_TAIL_BLOCK ? dcur : dv[n], o);
-    } else {
-      const GLB T* K = gp((const T*)P.mg[l].K);
-      for (long n = threadIdx.x; n < nodes(l); n += TOPT_MG_TAIL_BLOCK)
-        mg_smooth_p<T>(P, l, K, n, x, b, PF && n < TOPT_MG_TAIL_BLOCK ? dcur : dv[n], o);
-    }
+    each_slot(l, [&](int g, const TailSlot& s) {
+      with_rec(g, s, [&](auto ra, auto seq) {
+        const d2v ax = tail_apply<T, decltype(seq)::value>(s, ra, sx + ol);
+        const vec2<T> bn = sb[g], xn = sx[g];
+        vec2<T> o;
+        o.x = (T)((double)xn.x + TOPT_MG_OMEGA * (double)ra(36) * ((double)bn.x - ax.x));
+        o.y = (T)((double)xn.y + TOPT_MG_OMEGA * (double)ra(37) * ((double)bn.y - ax.y));
+        sr[g] = o;
+      });
+    });
     __syncthreads();
   }
-  {
-    const vec2<T>* src = (const vec2<T>*)(lt == L ? sx + off[L] : sr + off[lt]);
-    GLB vec2<T>* dst = gp((vec2<T>*)mg_xup(P, lt));
-    for (long n = threadIdx.x; n < nodes(lt); n += TOPT_MG_TAIL_BLOCK) dst[n] = src[n];
+  GLB vec2<T>* dst = gp((vec2<T>*)mg_xup(P, lt));
+  if (lt < L) {
+    each_slot(lt, [&](int g, const TailSlot& s) { dst[s.n] = sr[g]; });
+  } else {
+    for (int n = t; n < nL; n += NB) dst[n] = sx[n];
   }
 }
 
