@@ -414,8 +414,15 @@ __device__ __forceinline__ bool mg_stopped(PD& P, int par) {
   // the plan with another three-slot preconditioner. A slot B that confirmed
   // a stop on the true residual (B_REF with a pending status) is followed by
   // the slot C that ends the solve: its V-cycle would go unused.
-  const CST CgState& c = cp((const CgState*)P.cg)[par];
-  return P.precond != PC_MG || c.status != ST_RUN || (c.op == OP_B_REF && c.pending != 0);
+  // it, status, op and pending come in one 16-byte scalar load and combine
+  // without branches (short-circuit tests would wait for each in turn).
+  typedef int i4 __attribute__((ext_vector_type(4)));
+  static_assert(offsetof(CgState, it) % 16 == 0 && offsetof(CgState, status) == offsetof(CgState, it) + 4 &&
+                    offsetof(CgState, op) == offsetof(CgState, it) + 8 &&
+                    offsetof(CgState, pending) == offsetof(CgState, it) + 12 && sizeof(CgState) % 16 == 0,
+                "it, status, op, pending: one aligned 16-byte word");
+  const i4 w = *(const CST i4*)&cp((const CgState*)P.cg)[par].it;
+  return (int)(P.precond != PC_MG) | (int)(w.y != ST_RUN) | ((int)(w.z == OP_B_REF) & (int)(w.w != 0));
 }
 // The V-cycle kernels cover a batch through (problem, local block) maps: one
 // map per level and kind, holding only the problems that take that step.
@@ -583,15 +590,28 @@ __device__ __forceinline__ void mg_down_restrict_tile(PD& P, int lb, int fnx, in
   unsigned fb = 0u;
   if constexpr (MASK) fb = fixbits(P.fix, mn);
   // x on the fine patch (clamped into the grid: a clamped value meets only
-  // elements outside it, whose factor is 0)
-  for (int u = t; u < D::XX * D::XY; u += TOPT_BLOCK) {
+  // elements outside it, whose factor is 0); all of a thread's patch loads
+  // are issued before its first LDS store (one round trip, not one per
+  // patch row of threads)
+  constexpr int NPU = (D::XX * D::XY + TOPT_BLOCK - 1) / TOPT_BLOCK;
+  vec2<T> pb[NPU], pd[NPU];
+#pragma unroll
+  for (int k = 0; k < NPU; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < D::XX * D::XY ? u0 : D::XX * D::XY - 1;
     const int px = u / D::XY, py = u - (u / D::XY) * D::XY;
     int jx = 2 * I0 - 2 + px, jy = 2 * J0 - 2 + py;
     jx = jx < 0 ? 0 : (jx > fnx ? fnx : jx);
     jy = jy < 0 ? 0 : (jy > fny ? fny : jy);
     const long m = (long)jx * fnny + jy;
-    const vec2<T> bb = b[m], dd = dv[m];
-    xs[u] = d2v{TOPT_MG_OMEGA * (double)dd.x * (double)bb.x, TOPT_MG_OMEGA * (double)dd.y * (double)bb.y};
+    pb[k] = b[m];
+    pd[k] = dv[m];
+  }
+#pragma unroll
+  for (int k = 0; k < NPU; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < D::XX * D::XY)
+      xs[u] = d2v{TOPT_MG_OMEGA * (double)pd[k].x * (double)pb[k].x, TOPT_MG_OMEGA * (double)pd[k].y * (double)pb[k].y};
   }
   __syncthreads();
   if (hasr) {
@@ -647,11 +667,16 @@ template <typename T>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const ProbDev* tab, const int2* map, int l, int par) {
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
-  if (mg_stopped(P, par)) return;
-  const MgElems<T> ek{gp((const T*)P.mg[l].K), P.mg[l].nx, P.mg[l].ny};
-  mg_down_restrict_tile<T, false>(P, lb, P.mg[l].nx, P.mg[l].ny, P.mg[l + 1].nx, P.mg[l + 1].ny,
-                                  gp((const vec2<T>*)P.mg[l].b), gp((const vec2<T>*)P.mg[l].dinv), ek,
-                                  gp((vec2<T>*)P.mg[l + 1].b));
+  // the level's fields enter the test, so they are read with the state (one
+  // scalar round trip; a level without buffers has nothing to sweep)
+  const int fnx = P.mg[l].nx, fny = P.mg[l].ny, cnx = P.mg[l + 1].nx, cny = P.mg[l + 1].ny;
+  const void *K = P.mg[l].K, *b = P.mg[l].b, *dv = P.mg[l].dinv, *bc = P.mg[l + 1].b;
+  if ((int)mg_stopped(P, par) | (int)(K == nullptr) | (int)(b == nullptr) | (int)(dv == nullptr) |
+      (int)(bc == nullptr) | (int)((lb | fnx | fny | cnx | cny) < 0))
+    return;
+  const MgElems<T> ek{gp((const T*)K), fnx, fny};
+  mg_down_restrict_tile<T, false>(P, lb, fnx, fny, cnx, cny, gp((const vec2<T>*)b), gp((const vec2<T>*)dv), ek,
+                                  gp((vec2<T>*)bc));
 }
 
 // Level 0: b_1 from the PCG residual r (x0 = omega D^-1 r; K(s) has fixed
@@ -660,10 +685,15 @@ template <typename T, bool KST>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const ProbDev* tab, const int2* map, int par) {
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
-  if (mg_stopped(P, par)) return;
-  const MgElems0<T, KST> ek{P, gp((const T*)P.s)};
-  mg_down_restrict_tile<T, true>(P, lb, P.nelx, P.nely, P.mg[1].nx, P.mg[1].ny, gp((const vec2<T>*)P.r),
-                                 gp((const vec2<T>*)P.pre), ek, gp((vec2<T>*)P.mg[1].b));
+  // the fields enter the test, so they are read with the state
+  const int fnx = P.nelx, fny = P.nely, cnx = P.mg[1].nx, cny = P.mg[1].ny;
+  const void *r = P.r, *dv = P.pre, *sc = P.s, *bc = P.mg[1].b;
+  if ((int)mg_stopped(P, par) | (int)(r == nullptr) | (int)(dv == nullptr) | (int)(sc == nullptr) |
+      (int)(bc == nullptr) | (int)((lb | fnx | fny | cnx | cny) < 0))
+    return;
+  const MgElems0<T, KST> ek{P, gp((const T*)sc)};
+  mg_down_restrict_tile<T, true>(P, lb, fnx, fny, cnx, cny, gp((const vec2<T>*)r), gp((const vec2<T>*)dv), ek,
+                                 gp((vec2<T>*)bc));
 }
 
 // Up sweep at levels 1 .. lt-1 on tiles of UpTile::TX x UpTile::TY nodes
@@ -691,9 +721,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   __shared__ d2v xs[U::PX * U::PY];
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
-  if (mg_stopped(P, par)) return;
-  const int nx = P.mg[l].nx, ny = P.mg[l].ny, nny = ny + 1;
-  const int cny = P.mg[l + 1].ny;
+  // the level's fields enter the test, so they are read with the state
+  const int nx = P.mg[l].nx, ny = P.mg[l].ny, cny = P.mg[l + 1].ny;
+  const void *K = P.mg[l].K, *bl = P.mg[l].b, *dl = P.mg[l].dinv, *rl = P.mg[l].r, *xup = mg_xup(P, l + 1);
+  if ((int)mg_stopped(P, par) | (int)(K == nullptr) | (int)(bl == nullptr) | (int)(dl == nullptr) |
+      (int)(rl == nullptr) | (int)(xup == nullptr) | (int)((lb | nx | ny | cny) < 0))
+    return;
+  const int nny = ny + 1;
   const int nty = (ny + U::TY) / U::TY;
   const int tx = lb / nty, ty = lb - tx * nty;
   const int x0 = tx * U::TX, y0 = ty * U::TY;
@@ -702,27 +736,41 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   const int ix = x0 + ox, iy = y0 + oy;
   const bool act = ix <= nx && iy <= ny;
   const long n = (long)(ix <= nx ? ix : nx) * nny + (iy <= ny ? iy : ny);
-  const GLB vec2<T>* b = gp((const vec2<T>*)P.mg[l].b);
-  const GLB vec2<T>* dv = gp((const vec2<T>*)P.mg[l].dinv);
-  const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, l + 1));
+  const GLB vec2<T>* b = gp((const vec2<T>*)bl);
+  const GLB vec2<T>* dv = gp((const vec2<T>*)dl);
+  const GLB vec2<T>* xc = gp((const vec2<T>*)xup);
   const vec2<T> bn = b[n], dn = dv[n];
-  const MgElems<T> ek{gp((const T*)P.mg[l].K), nx, ny};
+  const MgElems<T> ek{gp((const T*)K), nx, ny};
   typename MgElems<T>::R er[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c) er[a][c] = ek.fetch(ix - 1 + a, iy - 1 + c, mg_me(a, c));
   // x' on the patch (clamped into the grid: a clamped value meets only
-  // elements outside it, whose factor is 0)
-  for (int u = t; u < U::PX * U::PY; u += TOPT_BLOCK) {
+  // elements outside it, whose factor is 0); all of a thread's patch loads
+  // are issued before its first LDS store
+  constexpr int NPU = (U::PX * U::PY + TOPT_BLOCK - 1) / TOPT_BLOCK;
+  vec2<T> pb[NPU], pd[NPU];
+  d2v pc[NPU];
+#pragma unroll
+  for (int k = 0; k < NPU; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < U::PX * U::PY ? u0 : U::PX * U::PY - 1;
     const int px = u / U::PY, py = u - (u / U::PY) * U::PY;
     int jx = x0 - 1 + px, jy = y0 - 1 + py;
     jx = jx < 0 ? 0 : (jx > nx ? nx : jx);
     jy = jy < 0 ? 0 : (jy > ny ? ny : jy);
     const long m = (long)jx * nny + jy;
-    const vec2<T> bb = b[m], dd = dv[m];
-    xs[u] = d2v{TOPT_MG_OMEGA * (double)dd.x * (double)bb.x, TOPT_MG_OMEGA * (double)dd.y * (double)bb.y} +
-            mg_prolong<T>(xc, cny, jx, jy);
+    pb[k] = b[m];
+    pd[k] = dv[m];
+    pc[k] = mg_prolong<T>(xc, cny, jx, jy);
+  }
+#pragma unroll
+  for (int k = 0; k < NPU; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < U::PX * U::PY)
+      xs[u] = d2v{TOPT_MG_OMEGA * (double)pd[k].x * (double)pb[k].x, TOPT_MG_OMEGA * (double)pd[k].y * (double)pb[k].y} +
+              pc[k];
   }
   __syncthreads();
   if (!act) return;
@@ -749,7 +797,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   vec2<T> o;
   o.x = (T)(xo.x + TOPT_MG_OMEGA * (double)dn.x * ((double)bn.x - ax));
   o.y = (T)(xo.y + TOPT_MG_OMEGA * (double)dn.y * ((double)bn.y - ay));
-  gp((vec2<T>*)P.mg[l].r)[n] = o;
+  gp((vec2<T>*)rl)[n] = o;
 }
 
 // Levels lt..L (the small ones) in one workgroup: down sweeps and
