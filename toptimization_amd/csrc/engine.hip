@@ -521,8 +521,9 @@ struct topt_plan {
   int64_t q = 0;          // PCG slot sequence number of the next launch
   int64_t na = 0;         // slot-A launches so far in this solve (p buffer parity)
   // (status, iteration) per problem after a chunk: pinned host slots mapped
-  // into the device address space (dstat: their device addresses), which
-  // k_gather_status writes directly (no copy operation behind each chunk)
+  // into the device address space (dstat: their device addresses), written
+  // directly (no copy operation behind each chunk) by the last slot kernel
+  // of a captured chunk, else by k_gather_status
   int* dstat[2] = {nullptr, nullptr};
   int* hstat[2] = {nullptr, nullptr};
   topt_iter_stats* dstats = nullptr;
@@ -533,8 +534,8 @@ struct topt_plan {
   // hipGraphLaunch on the caller's stream
   bool use_graph = false;
   hipStream_t cap = nullptr;
-  // one graph per status slot: the chunk's slots followed by its status
-  // gather into dstat[slot]
+  // one graph per status slot: the chunk's slots, the last one writing the
+  // status snapshot into dstat[slot]
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   hipGraph_t mgraph[2] = {nullptr, nullptr};       // multigrid plans: one 2-iteration chunk
@@ -652,7 +653,8 @@ void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
 }
 
 template <typename T>
-int launch_slots(topt_plan* pl, int count, hipStream_t s) {
+// stat: when set, the chunk's last slot kernel writes the status snapshot there
+int launch_slots(topt_plan* pl, int count, hipStream_t s, int* stat = nullptr) {
   const ProbDev dummy{};
   const int nb = pl->npcur;
   const int2* map = pl->pcur;
@@ -662,6 +664,7 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
     return TOPT_OK;
   }
   for (int k = 0; k < count; ++k) {
+    int* st_last = k == count - 1 ? stat : nullptr;
     // slot A number na reads p buffer (na & 1) and writes the other one
     const int pa = (int)(pl->na & 1);
     pl->na++;
@@ -675,22 +678,22 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s) {
     if (pl->all_mg) {
       // slot C fused with the level-0 up sweep
       hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
-                         (int)(pl->q++ & 1), pa ^ 1);
+                         (int)(pl->q++ & 1), pa ^ 1, (int*)nullptr);
       mg_vcycle<T>(pl, (int)(pl->q & 1), true, s);
       if (pl->kstruct)
         hipLaunchKernelGGL((k_slot_c_mg<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
-                           (int)(pl->q++ & 1), pa ^ 1);
+                           (int)(pl->q++ & 1), pa ^ 1, st_last);
       else
         hipLaunchKernelGGL((k_slot_c_mg<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
-                           (int)(pl->q++ & 1), pa ^ 1);
+                           (int)(pl->q++ & 1), pa ^ 1, st_last);
       continue;
     }
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
-                       (int)(pl->q++ & 1), pa ^ 1);
+                       (int)(pl->q++ & 1), pa ^ 1, pl->three_slot ? (int*)nullptr : st_last);
     if (pl->mg) mg_vcycle<T>(pl, (int)(pl->q & 1), false, s);
     if (pl->three_slot)
       hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
-                         (int)(pl->q++ & 1), pa ^ 1);
+                         (int)(pl->q++ & 1), pa ^ 1, st_last);
   }
   LAUNCH_TRY();
   return TOPT_OK;
@@ -726,20 +729,16 @@ int capture_graph(topt_plan* pl, const std::function<int(hipStream_t)>& body, hi
   return TOPT_OK;
 }
 
-// A full chunk's slot launches and its status gather into dstat[slot]. The
-// gather reads the state parity after the chunk, which a full chunk (even
-// start, even length in slots) leaves even.
+// A full chunk's slot launches, the last one also writing the status
+// snapshot into dstat[slot] (the state every problem's slot writes after the
+// chunk: a full chunk, even start and even length in slots, leaves it at
+// parity 0; chunk graphs cover every problem of the plan).
 int chunk_graph(topt_plan* pl, int count, int slot, hipGraph_t* graph, hipGraphExec_t* exec) {
   if (*exec) return TOPT_OK;
   const int64_t q0 = pl->q, na0 = pl->na;
   const int rc = capture_graph(pl, [&](hipStream_t cs) {
-    const int r = pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, cs)
-                                        : launch_slots<double>(pl, count, cs);
-    if (r) return r;
-    hipLaunchKernelGGL(k_gather_status, dim3(cdiv(pl->n, 256)), dim3(256), 0, cs, pl->dtab, pl->n,
-                       (int)(pl->q & 1), pl->dstat[slot]);
-    LAUNCH_TRY();
-    return (int)TOPT_OK;
+    return pl->dtype == TOPT_F32 ? launch_slots<float>(pl, count, cs, pl->dstat[slot])
+                                 : launch_slots<double>(pl, count, cs, pl->dstat[slot]);
   }, graph, exec);
   pl->q = q0;     // capturing launched nothing
   pl->na = na0;

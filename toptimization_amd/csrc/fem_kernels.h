@@ -585,7 +585,7 @@ __device__ __forceinline__ void pcg_advance(PD& P, CgState& s, const double (&t)
 enum SlotKind : int { SK_A = 0, SK_B = 1, SK_C = 2 };
 template <typename T, int K>
 __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevParts& pp, double* sh,
-                                              bool* upd) {
+                                              bool* upd, int* stat = nullptr, int pi = 0) {
   const CST CgState* cgs = cp((const CgState*)P.cg);
   CgState s;
   {
@@ -637,7 +637,16 @@ __device__ __forceinline__ CgState slot_begin(PD& P, int lb, int q, const PrevPa
     *upd = pend && !run;
     if (*upd) s.upend = 0;
   }
-  if (lb == 0 && threadIdx.x == 0) P.cg[(q + 1) & 1] = s;
+  if (lb == 0 && threadIdx.x == 0) {
+    P.cg[(q + 1) & 1] = s;
+    // the last slot of a captured chunk also writes the host's status
+    // snapshot (mapped pinned memory, read after the chunk's event)
+    if (stat != nullptr) {
+      stat[2 * pi] = s.status;
+      stat[2 * pi + 1] = s.it;
+      __threadfence_system();
+    }
+  }
   return s;
 }
 
@@ -812,7 +821,7 @@ __device__ __forceinline__ vec2<T> precond_apply(int pc, vec2<T> d, T o, vec2<T>
 // converges in the FP64 iteration count with an FP64 one).
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const int2* pmap,
-                                                       ProbDev single, int q, int par) {
+                                                       ProbDev single, int q, int par, int* stat) {
   using Tl = Tile<T>;
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -841,7 +850,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
     if (pc == PC_BLOCK2) ov[j] = (gp((const T*)P.pre2))[n];
   }
   bool upd;
-  const CgState st = slot_begin<T, SK_B>(P, lb, q, pp, sh, &upd);
+  const CgState st = slot_begin<T, SK_B>(P, lb, q, pp, sh, &upd, stat, B ? pmap[b].x : 0);
   if (upd) {
     // deferred u += alpha p of the last B_CG (after a refresh, or at the stop);
     // p is the buffer the preceding slot A wrote (par)
@@ -917,7 +926,7 @@ __device__ __forceinline__ void slot_c_stop_update(PD& P, TileOrg o, double alph
 
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const int2* pmap,
-                                                       ProbDev single, int q, int par) {
+                                                       ProbDev single, int q, int par, int* stat) {
   using Tl = Tile<T>;
   __shared__ double sh[16];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -927,7 +936,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
   PrevParts pp;
   parts_fetch(P, q, pp);
   bool upd;
-  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
+  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd, stat, B ? pmap[b].x : 0);
   if (st.status != ST_RUN) {
     if (upd) slot_c_stop_update<T>(P, o, st.alpha, par);
     return;

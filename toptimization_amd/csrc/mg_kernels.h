@@ -1114,14 +1114,16 @@ namespace topt {
 // Level-0 up sweep + slot C: z = x0 + P x_1 + omega D^-1 (r - K x'), z = 0
 // on fixed DOFs, partial r.z (x0 = omega D^-1 r, formed again from r).
 template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, const int2* pmap, int q, int par) {
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, const int2* pmap, int q, int par,
+                                                          int* stat) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   __shared__ double sh[16];
   int lb;
-  PD& P = mg_entry(tab, pmap, xcd_remap(blockIdx.x, gridDim.x), lb);
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  PD& P = mg_entry(tab, pmap, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, 1));
   const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
@@ -1145,7 +1147,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   }
   const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   bool upd;
-  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd);
+  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd, stat, pmap[b].x);
   if (st.status != ST_RUN) {
     if (upd) slot_c_stop_update<T>(P, o, st.alpha, par);
     return;
