@@ -624,27 +624,35 @@ void mg_setup_all(topt_plan* pl, hipStream_t s) {
                      pl->mg_tail, dummy);
 }
 
+// The PCG tile map of a launch: null (the identity) for a one-problem plan
+// whose map covers its only problem.
+const int2* pcg_map(const topt_plan* pl) {
+  return pl->n == 1 && pl->pcur == pl->dpfull ? nullptr : pl->pcur;
+}
+
 // fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)
 template <typename T>
 void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
   const ProbDev* tab = pl->dtab;
   const int nt = pl->npcur;        // live problems' tiles (the PCG map)
-  const int2* tmap = pl->pcur;
+  const int2* tmap = pcg_map(pl);
+  // a one-problem plan's level maps are identity maps: passed as null
+  auto lmap = [&](const int2* m) { return pl->n == 1 ? (const int2*)nullptr : m; };
   // down sweeps fused with the restrictions: b_{l+1} from b_l (b_0 = r)
   if (pl->kstruct)
-    hipLaunchKernelGGL((k_mg_down_restrict0<T, true>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[0],
+    hipLaunchKernelGGL((k_mg_down_restrict0<T, true>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, lmap(pl->mg_rs[0]),
                        par);
   else
     hipLaunchKernelGGL((k_mg_down_restrict0<T, false>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab,
-                       pl->mg_rs[0], par);
+                       lmap(pl->mg_rs[0]), par);
   for (int l = 1; l < pl->mg_lt_max; ++l)
     if (pl->n_rs[l] > 0)
-      hipLaunchKernelGGL((k_mg_down_restrict<T>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_rs[l], l,
+      hipLaunchKernelGGL((k_mg_down_restrict<T>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, tab, lmap(pl->mg_rs[l]), l,
                          par);
-  hipLaunchKernelGGL((k_mg_tail<T>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, tab, pl->mg_tail, par);
+  hipLaunchKernelGGL((k_mg_tail<T>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, tab, lmap(pl->mg_tail), par);
   for (int l = pl->mg_lt_max - 1; l >= 1; --l)
     if (pl->n_dn[l] > 0)
-      hipLaunchKernelGGL((k_mg_up<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, pl->mg_dn[l], l, par);
+      hipLaunchKernelGGL((k_mg_up<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, lmap(pl->mg_dn[l]), l, par);
   if (fused) return;
   if (pl->kstruct)
     hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
@@ -657,7 +665,7 @@ template <typename T>
 int launch_slots(topt_plan* pl, int count, hipStream_t s, int* stat = nullptr) {
   const ProbDev dummy{};
   const int nb = pl->npcur;
-  const int2* map = pl->pcur;
+  const int2* map = pcg_map(pl);
   if (nb == 0) {   // every solve has ended: the sequence numbers still advance
     pl->q += (int64_t)(pl->three_slot ? 3 : 2) * count;
     pl->na += count;
@@ -701,7 +709,7 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s, int* stat = nullptr) {
 
 #define TOPT_GRAPH_CHUNK 32
 #ifndef TOPT_MG_CHUNK
-#define TOPT_MG_CHUNK 2   // CG iterations per chunk of a multigrid solve
+#define TOPT_MG_CHUNK 4   // CG iterations per chunk of a multigrid solve
 #endif
 
 // The slot launches of one full chunk (count iterations), captured into a

@@ -850,7 +850,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b(const ProbDev* tab, const
     if (pc == PC_BLOCK2) ov[j] = (gp((const T*)P.pre2))[n];
   }
   bool upd;
-  const CgState st = slot_begin<T, SK_B>(P, lb, q, pp, sh, &upd, stat, B ? pmap[b].x : 0);
+  const CgState st = slot_begin<T, SK_B>(P, lb, q, pp, sh, &upd, stat, B && pmap ? pmap[b].x : 0);
   if (upd) {
     // deferred u += alpha p of the last B_CG (after a refresh, or at the stop);
     // p is the buffer the preceding slot A wrote (par)
@@ -936,7 +936,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c(const ProbDev* tab, const
   PrevParts pp;
   parts_fetch(P, q, pp);
   bool upd;
-  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd, stat, B ? pmap[b].x : 0);
+  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd, stat, B && pmap ? pmap[b].x : 0);
   if (st.status != ST_RUN) {
     if (upd) slot_c_stop_update<T>(P, o, st.alpha, par);
     return;

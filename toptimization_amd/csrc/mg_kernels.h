@@ -425,8 +425,13 @@ __device__ __forceinline__ bool mg_stopped(PD& P, int par) {
   return (int)(P.precond != PC_MG) | (int)(w.y != ST_RUN) | ((int)(w.z == OP_B_REF) & (int)(w.w != 0));
 }
 // The V-cycle kernels cover a batch through (problem, local block) maps: one
-// map per level and kind, holding only the problems that take that step.
+// map per level and kind, holding only the problems that take that step (a
+// null map: the identity map of a one-problem plan).
 __device__ __forceinline__ PD& mg_entry(const ProbDev* tab, const int2* map, int b, int& lb) {
+  if (map == nullptr) {
+    lb = b;
+    return *(PD*)tab;
+  }
   const int2 e = map[b];
   lb = e.y;
   return *(PD*)(tab + e.x);
@@ -1147,7 +1152,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   }
   const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   bool upd;
-  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd, stat, pmap[b].x);
+  const CgState st = slot_begin<T, SK_C>(P, lb, q, pp, sh, &upd, stat, pmap ? pmap[b].x : 0);
   if (st.status != ST_RUN) {
     if (upd) slot_c_stop_update<T>(P, o, st.alpha, par);
     return;

@@ -183,11 +183,17 @@ __device__ __forceinline__ PD& get_prob(const ProbDev* tab, const int* bmap,
 
 // Batched grids: the block map holds (problem, local block) pairs, so a
 // launch can cover any subset of a plan's problems (each with all its
-// blocks). Single-problem launches pass the descriptor by value.
+// blocks); a null map stands for the identity map of a one-problem plan
+// (problem 0, block b), which saves the map's load on the way to the
+// descriptor. Single-problem launches pass the descriptor by value.
 template <bool BATCHED>
 __device__ __forceinline__ PD& grid_prob(const ProbDev* tab, const int2* bmap, const ProbDev& single, int b,
                                          int& lb) {
   if constexpr (BATCHED) {
+    if (bmap == nullptr) {
+      lb = b;
+      return *(PD*)tab;
+    }
     const int2 e = bmap[b];
     lb = e.y;
     return *(PD*)(tab + e.x);
