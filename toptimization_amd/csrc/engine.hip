@@ -471,11 +471,14 @@ OcScratch oc_scratch(int nb_oc) {
   return o;
 }
 
-// The OC update's launches (optimizer/oc.py:52-93): state reset, ten
-// depth-5 bisection passes (the reference's <= 50 steps), commit, results.
+// The OC update's launches (optimizer/oc.py:52-93): state reset, the chain
+// pass (at least one bisection step), ten depth-5 subtree passes (with it,
+// the reference's <= 50 steps; passes after the stop return at once),
+// commit, results.
 int launch_oc_update(ProbDev& P, double* out, hipStream_t st) {
   constexpr int D = 5;
   hipLaunchKernelGGL((k_oc_reset<false>), dim3(1), dim3(64), 0, st, nullptr, nullptr, 1, P);
+  hipLaunchKernelGGL((k_oc_eval<D, false, true>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
   for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
     hipLaunchKernelGGL((k_oc_eval<D, false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
   hipLaunchKernelGGL((k_oc_apply<false>), dim3(P.nb[G_OC]), dim3(TOPT_BLOCK), 0, st, nullptr, nullptr, P);
@@ -808,6 +811,8 @@ int plan_oc(topt_plan* pl, hipStream_t s) {
   constexpr int D = 5;
   hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n,
                      dummy);
+  hipLaunchKernelGGL((k_oc_eval<D, true, true>), dim3(grid_of(pl, G_OC).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                     grid_of(pl, G_OC).map, dummy);
   for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
     hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(grid_of(pl, G_OC).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
                        grid_of(pl, G_OC).map, dummy);
@@ -982,6 +987,7 @@ int subset_post(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_f
   hipLaunchKernelGGL((k_filter<true>), dim3(gf.nb), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s, pl->dtab, gf.map,
                      dummy, (int)(sens_filter ? F_SENS : F_DCAVG));
   hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, dummy);
+  hipLaunchKernelGGL((k_oc_eval<D, true, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
   for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
     hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
   hipLaunchKernelGGL((k_oc_apply<true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);

@@ -110,11 +110,16 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_filter_tables(int nelx, int nely
 // ---------------------------------------------------------------- OC
 // Bisection on lambda (optimizer/oc.py:52-93): bracket [0, 1e9], at most 50
 // steps, stop when (hi-lo)/(hi+lo+1e-30) < 1e-4, commit rho(last lambda).
-// Each k_oc_eval pass evaluates the volumes of a whole depth-D subtree of
-// bisection midpoints (2^D - 1 lambdas) in one sweep over the elements, then
-// the last block replays D bisection steps from those volumes. The midpoint
-// sequence is exactly the reference's (mid = 0.5*(lo+hi) in FP64); volumes
-// are FP64 sums.
+// Each k_oc_eval pass evaluates the volumes of 2^D - 1 bisection midpoints
+// in one sweep over the elements, then the last block replays bisection steps
+// from those volumes. The first pass of an OC step (CHAIN) takes the chain of
+// midpoints 1e9 / 2^(k+1), k < 2^D - 1: from lo = 0 every midpoint is an exact
+// halving, and the bisection keeps moving hi down until the first volume
+// above volfrac, so one sweep covers that whole prefix (about 25 steps for
+// typical lambdas) and the replay stops at the first step that moves lo. The
+// later passes take a whole depth-D subtree of the current bracket. The
+// midpoint sequence is exactly the reference's (mid = 0.5*(lo+hi) in FP64);
+// volumes are FP64 sums.
 
 template <bool B>
 __global__ void k_oc_reset(const ProbDev* tab, const int* idx, int nprob, ProbDev single) {
@@ -149,19 +154,31 @@ __device__ __forceinline__ double subtree_lambda(int j, double lo, double hi) {
   return mid;
 }
 
-template <int D, bool B>
+// Lambda of chain entry j from [0, hi]: j + 1 halvings (exact).
+__device__ __forceinline__ double chain_lambda(int j, double hi) {
+  double mid = 0.5 * hi;
+  for (int k = 0; k < j; ++k) mid = 0.5 * mid;
+  return mid;
+}
+
+template <int D, bool B, bool CHAIN = false>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, const int2* bmap,
                                                         ProbDev single) {
   constexpr int NL = (1 << D) - 1;
   __shared__ double qs[NL];
   __shared__ double red[4 * NL];
+  __shared__ double vtot[NL];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int lb;
   PD& P = grid_prob<B>(tab, bmap, single, b, lb);
   OcState* o = P.oc;
   if (o->stopped) return;
   const double lo = o->lo, hi = o->hi;
-  if (threadIdx.x < NL) qs[threadIdx.x] = 1.0 / sqrt(subtree_lambda(threadIdx.x, lo, hi) + 1e-20);
+  // the chain is only valid from the initial bracket (lo = 0)
+  const bool chain = CHAIN && lo == 0.0;
+  if (threadIdx.x < NL)
+    qs[threadIdx.x] =
+        1.0 / sqrt((chain ? chain_lambda(threadIdx.x, hi) : subtree_lambda(threadIdx.x, lo, hi)) + 1e-20);
   __syncthreads();
   const long ne = (long)P.nelx * P.nely;
   const double mv = P.move;
@@ -200,19 +217,27 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
   }
   double tot[NL];
   if (!tree_reduce<NL>(vsum, P.opart, P.ctr[G_OC], lb, P.nb[G_OC], tot)) return;
+  // the replay indexes the volumes with the step's node: from LDS, not a
+  // dynamically indexed register array (scratch)
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NL; ++j) vtot[j] = tot[j];
+  }
   if (threadIdx.x == 0) {
     double l = lo, h = hi;
     int node = 0, steps = o->steps;
     const double nd = (double)ne;
-    for (int d = 0; d < D; ++d) {
+    for (int d = 0; d < (chain ? NL : D); ++d) {
       const double mid = 0.5 * (l + h);
-      const double vol = tot[node] / nd;
+      const double vol = vtot[node] / nd;
+      bool right = false;
       if (vol > P.volfrac) {
         l = mid;
         node = 2 * node + 2;
+        right = true;
       } else {
         h = mid;
-        node = 2 * node + 1;
+        node = chain ? d + 1 : 2 * node + 1;
       }
       ++steps;
       o->last_mid = mid;
@@ -220,6 +245,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_eval(const ProbDev* tab, cons
         o->stopped = 1;
         break;
       }
+      if (chain && right) break;   // off the chain: the subtree passes go on
     }
     o->lo = l;
     o->hi = h;
