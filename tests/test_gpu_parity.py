@@ -124,7 +124,11 @@ def test_filter_modes_match_oracle(rmin):
     assert np.abs(got - O.density_filter(rho, nb, wt, cnt)).max() <= 4e-16
 
 
-@pytest.mark.parametrize("scale", [1.0, 1e-4, 1e-7])
+# lambda scales with dc: 1e10 and 3e8 move lo at the first or an early
+# bisection step (the chain pass ends at once), 1e-12 takes lambda below the
+# last chain midpoint 1e9 / 2^31 (the chain runs out, the subtree passes go
+# on to the 50-step cap)
+@pytest.mark.parametrize("scale", [1.0, 1e-4, 1e-7, 1e10, 3e8, 1e-12])
 def test_oc_update_matches_oracle(scale):
     from toptimization_amd import ops
     nelx, nely = 50, 30

@@ -156,7 +156,7 @@ int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
   // level 0 borrows b and 1/diag from the PCG (r, Jacobi diagonal); x is
   // stored at the coarsest level only (the sweeps form the pre-smoothed
   // iterate from b), r at levels 1 .. L-1 (the corrected iterate); the
-  // tail's sweep levels lt .. L-1 also keep their node records
+  // sweep levels 1 .. L-1 also keep their node records
   for (int l = 0; l < M.nlev; ++l) {
     const int64_t nn = nodes(l), ne = int64_t(M.nx[l]) * M.ny[l];
     M.K[l] = l > 0 ? take(ne * 36 * ts) : -1;
@@ -164,7 +164,7 @@ int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
     M.r[l] = l > 0 && l < L ? take(nn * 2 * ts) : -1;
     M.b[l] = l > 0 ? take(nn * 2 * ts) : -1;
     M.d[l] = l > 0 ? take(nn * 2 * ts) : -1;
-    M.st[l] = l >= M.lt && l < L ? take(nn * TOPT_MG_ST * ts) : -1;
+    M.st[l] = l > 0 && l < L ? take(nn * TOPT_MG_ST * ts) : -1;
   }
   M.ncd = 2 * (M.nx[L] + 1) * (M.ny[L] + 1);
   M.inv = take(int64_t(M.ncd) * M.ncd * 8);

@@ -142,9 +142,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, 
 }
 
 // 1/diag of the assembled level-l operator (l >= 1); 0 on decoupled DOFs.
-// On the tail's sweep levels also the node records (TOPT_MG_ST): the node's
-// two rows of the assembled operator, each coefficient the FP64 sum of its
-// element entries in ascending element order. The entry coupling nodes i and
+// On levels 1 .. L-1 also the node records (TOPT_MG_ST) the sweeps apply: the
+// node's two rows of the assembled operator, each coefficient the FP64 sum of
+// its element entries in ascending element order. The entry coupling nodes i and
 // j sums the same (symmetric) element entries in the same order from either
 // side, so the assembled operator is bitwise symmetric.
 template <typename T, bool B>
@@ -520,34 +520,56 @@ __host__ __device__ inline int mg_dr_tiles(int cnx, int cny) {
   return ((cnx + DrTile::CX) / DrTile::CX) * ((cny + DrTile::CY) / DrTile::CY);
 }
 
-// Element rows of level l >= 1 (stored Galerkin matrices): the two rows of
-// local node me, fetched from a clamped address; factor 0 outside the grid.
+// A node record of level l >= 1 (k_mg_diag): ten 16-byte loads.
 template <typename T>
-struct MgElems {
-  const GLB T* K;
-  int nx, ny;
-  struct R {
-    T k[16];
-    double f;
-  };
-  __device__ __forceinline__ R fetch(int ex, int ey, int me) const {
-    const bool ok = ex >= 0 && ex < nx && ey >= 0 && ey < ny;
-    const int cx = ex < 0 ? 0 : (ex >= nx ? nx - 1 : ex), cy = ey < 0 ? 0 : (ey >= ny ? ny - 1 : ey);
-    const GLB T* k = K + mg_kel(cx, cy, ny);
-    R r;
+__device__ __forceinline__ void mg_rec_load(const void* st, long n, T (&rec)[TOPT_MG_ST]) {
+  typedef T Q __attribute__((ext_vector_type(16 / sizeof(T))));
+  constexpr int W = 16 / (int)sizeof(T);
+  const GLB Q* src = gp((const Q*)st) + n * (TOPT_MG_ST / W);
 #pragma unroll
-    for (int d = 0; d < 2; ++d)
+  for (int i = 0; i < TOPT_MG_ST / W; ++i) {
+    const Q v = src[i];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) r.k[d * 8 + c] = k[(long)sym8(2 * me + d, c) * ny];
-    r.f = ok ? 1.0 : 0.0;
-    return r;
+    for (int w = 0; w < W; ++w) rec[i * W + w] = v[w];
   }
-  __device__ __forceinline__ double at(const R& r, int, int d, int c) const { return (double)r.k[d * 8 + c]; }
+}
+// The record's two rows applied to x at the node's 3 x 3 neighbourhood, x(q)
+// the value at neighbour q = 3 (dx + 1) + (dy + 1); neighbours outside the
+// grid have zero coefficients.
+template <typename T, typename XQ>
+__device__ __forceinline__ d2v mg_rec_apply(const T (&rec)[TOPT_MG_ST], XQ x) {
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const d2v v = x(q);
+    ax = fma((double)rec[2 * q], v.x, ax);
+    ax = fma((double)rec[2 * q + 1], v.y, ax);
+    ay = fma((double)rec[18 + 2 * q], v.x, ay);
+    ay = fma((double)rec[18 + 2 * q + 1], v.y, ay);
+  }
+  return d2v{ax, ay};
+}
+// Level l >= 1 rows from the node records.
+template <typename T>
+struct MgRec {
+  static constexpr bool kRec = true;
+  const void* st;
 };
+// The per-element row type of an element policy (none for records).
+template <typename EK, bool = EK::kRec>
+struct ElemsRows {
+  using R = typename EK::R;
+};
+template <typename EK>
+struct ElemsRows<EK, true> {
+  struct R {};
+};
+
 // Element rows of level 0: s_e Ke (KST: Ke from its six magnitudes); the
 // factor is the element scale, 0 outside the grid.
 template <typename T, bool KST>
 struct MgElems0 {
+  static constexpr bool kRec = false;
   PD& P;
   const GLB T* s;
   struct R {
@@ -587,11 +609,16 @@ __device__ __forceinline__ void mg_down_restrict_tile(PD& P, int lb, int fnx, in
   const int ic = i < 0 ? 0 : (i > fnx ? fnx : i), jc = j < 0 ? 0 : (j > fny ? fny : j);
   const long mn = (long)ic * fnny + jc;
   const vec2<T> bn = b[mn];
-  typename EK::R er[2][2];
+  [[maybe_unused]] T rec[TOPT_MG_ST];
+  [[maybe_unused]] typename ElemsRows<EK>::R er[2][2];
+  if constexpr (EK::kRec) {
+    mg_rec_load<T>(ek.st, mn, rec);
+  } else {
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) er[a][c] = ek.fetch(i - 1 + a, j - 1 + c, mg_me(a, c));
+      for (int c = 0; c < 2; ++c) er[a][c] = ek.fetch(i - 1 + a, j - 1 + c, mg_me(a, c));
+  }
   unsigned fb = 0u;
   if constexpr (MASK) fb = fixbits(P.fix, mn);
   // x on the fine patch (clamped into the grid: a clamped value meets only
@@ -621,24 +648,31 @@ __device__ __forceinline__ void mg_down_restrict_tile(PD& P, int lb, int fnx, in
   __syncthreads();
   if (hasr) {
     double ax = 0.0, ay = 0.0;
+    if constexpr (EK::kRec) {
+      // neighbour q of the node sits at patch position (qx + 1 + dx, qy + 1 + dy)
+      const d2v a2 = mg_rec_apply<T>(rec, [&](int q) { return xs[(qx + q / 3) * D::XY + qy + q % 3]; });
+      ax = a2.x;
+      ay = a2.y;
+    } else {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int me = mg_me(a, c);
-        double tx_ = 0.0, ty_ = 0.0;
+        for (int c = 0; c < 2; ++c) {
+          const int me = mg_me(a, c);
+          double tx_ = 0.0, ty_ = 0.0;
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          // corner kk of element (i - 1 + a, j - 1 + c): patch position below
-          const d2v v = xs[(qx + a + mg_lx(kk)) * D::XY + qy + c + mg_ly(kk)];
-          tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk), v.x, tx_);
-          tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk + 1), v.y, tx_);
-          ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk), v.x, ty_);
-          ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk + 1), v.y, ty_);
+          for (int kk = 0; kk < 4; ++kk) {
+            // corner kk of element (i - 1 + a, j - 1 + c): patch position below
+            const d2v v = xs[(qx + a + mg_lx(kk)) * D::XY + qy + c + mg_ly(kk)];
+            tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk), v.x, tx_);
+            tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk + 1), v.y, tx_);
+            ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk), v.x, ty_);
+            ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk + 1), v.y, ty_);
+          }
+          ax = fma(er[a][c].f, tx_, ax);
+          ay = fma(er[a][c].f, ty_, ay);
         }
-        ax = fma(er[a][c].f, tx_, ax);
-        ay = fma(er[a][c].f, ty_, ay);
-      }
+    }
     d2v r = d2v{(double)bn.x - ax, (double)bn.y - ay};
     if (!inr || (fb & 1u)) r.x = 0.0;
     if (!inr || (fb & 2u)) r.y = 0.0;
@@ -675,11 +709,11 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const ProbDev* 
   // the level's fields enter the test, so they are read with the state (one
   // scalar round trip; a level without buffers has nothing to sweep)
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny, cnx = P.mg[l + 1].nx, cny = P.mg[l + 1].ny;
-  const void *K = P.mg[l].K, *b = P.mg[l].b, *dv = P.mg[l].dinv, *bc = P.mg[l + 1].b;
-  if ((int)mg_stopped(P, par) | (int)(K == nullptr) | (int)(b == nullptr) | (int)(dv == nullptr) |
+  const void *st = P.mg[l].st, *b = P.mg[l].b, *dv = P.mg[l].dinv, *bc = P.mg[l + 1].b;
+  if ((int)mg_stopped(P, par) | (int)(st == nullptr) | (int)(b == nullptr) | (int)(dv == nullptr) |
       (int)(bc == nullptr) | (int)((lb | fnx | fny | cnx | cny) < 0))
     return;
-  const MgElems<T> ek{gp((const T*)K), fnx, fny};
+  const MgRec<T> ek{st};
   mg_down_restrict_tile<T, false>(P, lb, fnx, fny, cnx, cny, gp((const vec2<T>*)b), gp((const vec2<T>*)dv), ek,
                                   gp((vec2<T>*)bc));
 }
@@ -728,8 +762,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   // the level's fields enter the test, so they are read with the state
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, cny = P.mg[l + 1].ny;
-  const void *K = P.mg[l].K, *bl = P.mg[l].b, *dl = P.mg[l].dinv, *rl = P.mg[l].r, *xup = mg_xup(P, l + 1);
-  if ((int)mg_stopped(P, par) | (int)(K == nullptr) | (int)(bl == nullptr) | (int)(dl == nullptr) |
+  const void *st = P.mg[l].st, *bl = P.mg[l].b, *dl = P.mg[l].dinv, *rl = P.mg[l].r, *xup = mg_xup(P, l + 1);
+  if ((int)mg_stopped(P, par) | (int)(st == nullptr) | (int)(bl == nullptr) | (int)(dl == nullptr) |
       (int)(rl == nullptr) | (int)(xup == nullptr) | (int)((lb | nx | ny | cny) < 0))
     return;
   const int nny = ny + 1;
@@ -744,16 +778,12 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   const GLB vec2<T>* b = gp((const vec2<T>*)bl);
   const GLB vec2<T>* dv = gp((const vec2<T>*)dl);
   const GLB vec2<T>* xc = gp((const vec2<T>*)xup);
-  const vec2<T> bn = b[n], dn = dv[n];
-  const MgElems<T> ek{gp((const T*)K), nx, ny};
-  typename MgElems<T>::R er[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) er[a][c] = ek.fetch(ix - 1 + a, iy - 1 + c, mg_me(a, c));
+  const vec2<T> bn = b[n];
+  T rec[TOPT_MG_ST];   // the node's rows and 1/diag
+  mg_rec_load<T>(st, n, rec);
   // x' on the patch (clamped into the grid: a clamped value meets only
-  // elements outside it, whose factor is 0); all of a thread's patch loads
-  // are issued before its first LDS store
+  // neighbours outside it, whose coefficients are 0); all of a thread's patch
+  // loads are issued before its first LDS store
   constexpr int NPU = (U::PX * U::PY + TOPT_BLOCK - 1) / TOPT_BLOCK;
   vec2<T> pb[NPU], pd[NPU];
   d2v pc[NPU];
@@ -779,29 +809,12 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   }
   __syncthreads();
   if (!act) return;
-  double ax = 0.0, ay = 0.0;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int me = mg_me(a, c);
-      double tx_ = 0.0, ty_ = 0.0;
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        // corner kk of element (ix - 1 + a, iy - 1 + c): patch position below
-        const d2v v = xs[(ox + a + mg_lx(kk)) * U::PY + oy + c + mg_ly(kk)];
-        tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk), v.x, tx_);
-        tx_ = fma(ek.at(er[a][c], me, 0, 2 * kk + 1), v.y, tx_);
-        ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk), v.x, ty_);
-        ty_ = fma(ek.at(er[a][c], me, 1, 2 * kk + 1), v.y, ty_);
-      }
-      ax = fma(er[a][c].f, tx_, ax);
-      ay = fma(er[a][c].f, ty_, ay);
-    }
+  // neighbour q of the node sits at patch position (ox + 1 + dx, oy + 1 + dy)
+  const d2v ax = mg_rec_apply<T>(rec, [&](int q) { return xs[(ox + q / 3) * U::PY + oy + q % 3]; });
   const d2v xo = xs[(ox + 1) * U::PY + oy + 1];
   vec2<T> o;
-  o.x = (T)(xo.x + TOPT_MG_OMEGA * (double)dn.x * ((double)bn.x - ax));
-  o.y = (T)(xo.y + TOPT_MG_OMEGA * (double)dn.y * ((double)bn.y - ay));
+  o.x = (T)(xo.x + TOPT_MG_OMEGA * (double)rec[36] * ((double)bn.x - ax.x));
+  o.y = (T)(xo.y + TOPT_MG_OMEGA * (double)rec[37] * ((double)bn.y - ax.y));
   gp((vec2<T>*)rl)[n] = o;
 }
 

@@ -79,7 +79,7 @@ template <typename T> struct NptMv { static constexpr int v = sizeof(T) == 4 ? T
 // agglomerate of a 2x2 patch of level-l elements (those past the level-l
 // grid count as void). Node vectors hold 2 T per node, as the PCG vectors.
 #define TOPT_MG_MAXLEV 14
-// Node record of a tail sweep level: entry 18 d + 2 q + c is the coefficient
+// Node record of a sweep level (l >= 1): entry 18 d + 2 q + c is the coefficient
 // of component c at neighbour q = 3 (dx + 1) + (dy + 1) in the node's row
 // for component d; entries 36, 37 are 1/diag; 38, 39 are zero (16-byte
 // multiple).
@@ -92,9 +92,9 @@ struct MgLevel {
   void* b;        // right-hand side (level 0: the PCG residual r)
   void* r;        // levels 1 .. L-1: the corrected, post-smoothed iterate (null elsewhere)
   void* dinv;     // 1 / diag (level 0: the Jacobi data in pre); 0 on decoupled DOFs
-  void* st;       // sweep levels of the single-workgroup tail: one record of
-                  // TOPT_MG_ST T per node (its two rows of the assembled level
-                  // operator, then 1/diag); null elsewhere
+  void* st;       // levels 1 .. L-1: one record of TOPT_MG_ST T per node (its
+                  // two rows of the assembled level operator, then 1/diag);
+                  // null elsewhere
 };
 
 struct ProbDev {
