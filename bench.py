@@ -71,15 +71,17 @@ def mg_levels(nelx, nely, dense=128):
 
 def mg_cg_bytes(nelx, nely, ts=4, tail=1024, tail_sum=2560):
     """Algorithmic bytes of one multigrid PCG iteration of the mixed path
-    (vectors in ts-byte words, two per node; the FP64 solution u; level
-    element matrices of 36 words; the FP64 coarsest inverse):
+    (vectors in ts-byte words, two per node; the FP64 solution u; node
+    records of 40 words (the two rows of the assembled level operator and
+    1/diag) at levels 1 .. L-1; the FP64 coarsest inverse):
       slot A        read z, p, u; write p, Ap, u; read s
       slot B        read r, Ap; write r
       down 0        read r, 1/diag, s; write b_1
-      down l, up l  (1 <= l < tail level) read b_l, 1/diag_l, K_l, [x_{l+1}];
-                    write b_{l+1} / x_l
-      tail          read b, 1/diag, K of its levels and the coarsest inverse;
-                    write x of its first level
+      down l, up l  (1 <= l < tail level) read b_l, 1/diag_l, records_l,
+                    [x_{l+1}]; write b_{l+1} / x_l
+      tail          read b of its first level, the records of its sweep
+                    levels and the coarsest inverse; write x of its first
+                    level
       up 0 + slot C read r, 1/diag, x_1, s; write z"""
     lv = mg_levels(nelx, nely)
     L = len(lv) - 1
@@ -89,13 +91,14 @@ def mg_cg_bytes(nelx, nely, ts=4, tail=1024, tail_sum=2560):
     while lt < L and (nn[lt] > tail or sum(nn[lt:]) > tail_sum):
         lt += 1
     v = 2 * ts
+    rec = 40 * ts
     b = 4 * v * nn[0] + 32 * nn[0] + ts * ne[0]          # slot A
     b += 3 * v * nn[0]                                      # slot B
     b += 2 * v * nn[0] + ts * ne[0] + v * nn[1]             # level-0 down sweep + restriction
     for l in range(1, lt):
-        b += 2 * v * nn[l] + 36 * ts * ne[l] + v * nn[l + 1]   # down sweep + restriction
-        b += 3 * v * nn[l] + 36 * ts * ne[l] + v * nn[l + 1]   # up sweep
-    b += v * nn[lt] + sum(v * nn[l] + 36 * ts * ne[l] for l in range(lt, L)) + 8 * (2 * nn[L]) ** 2 \
+        b += 2 * v * nn[l] + rec * nn[l] + v * nn[l + 1]       # down sweep + restriction
+        b += 3 * v * nn[l] + rec * nn[l] + v * nn[l + 1]       # up sweep
+    b += v * nn[lt] + sum(rec * nn[l] for l in range(lt, L)) + 8 * (2 * nn[L]) ** 2 \
         + v * nn[lt]                                        # tail
     b += 3 * v * nn[0] + ts * ne[0] + v * nn[1]             # level-0 up sweep + slot C
     return b
