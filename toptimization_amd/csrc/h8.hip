@@ -57,6 +57,7 @@ __global__ void k3_zero_fixed(double* u, const uint32_t* fix, long nd) {
 struct Dims3 {
   int64_t nn, ne, nd;
   int nbn, nbe;
+  int nbt;   // tiles of the matvec (k3_matvec)
 };
 
 Dims3 dims3(int nelx, int nely, int nelz) {
@@ -66,6 +67,7 @@ Dims3 dims3(int nelx, int nely, int nelz) {
   d.nd = 3 * d.nn;
   d.nbn = (int)cdiv(d.nn, TOPT_BLOCK);
   d.nbe = (int)cdiv(d.ne, TOPT_BLOCK);
+  d.nbt = mv3_tiles(nelx, nely, nelz);
   return d;
 }
 
@@ -79,8 +81,10 @@ Ws3 ws3(const Dims3& d) {
   Ws3 w;
   int64_t o = 0;
   auto take = [&](int64_t bytes) { const int64_t at = o; o += align256(bytes); return at; };
-  w.part = take(tree_parts(d.nbn) * 2 * 8);
-  w.ctr_words = (tree_ctrs(d.nbn) + 1) * TOPT_CTR_STRIDE;
+  // the node-grid and tile-grid reductions share the partials and counters
+  const int64_t nbr = std::max<int64_t>(d.nbn, d.nbt);
+  w.part = take(tree_parts(nbr) * 2 * 8);
+  w.ctr_words = (tree_ctrs(nbr) + 1) * TOPT_CTR_STRIDE;
   w.ctr = take(w.ctr_words * 4);
   w.cg = take(sizeof(Cg3));
   w.taps = take(int64_t(2 * kMaxTapRadius + 1) * (2 * kMaxTapRadius + 1) * (2 * kMaxTapRadius + 1) * sizeof(Tap3));
@@ -222,7 +226,7 @@ int topt3_matvec(const topt_grid3* g, const double* p, double* Ap, const double*
   Ws3 w;
   int rc = make_grid3(g, s, fix, pAp ? ws : nullptr, st, G, d, w);
   if (rc) return rc;
-  hipLaunchKernelGGL(k3_matvec, dim3(d.nbn), dim3(TOPT_BLOCK), 0, st, G, p, Ap, pAp ? (int)MV3_DOT : (int)MV3_PLAIN,
+  hipLaunchKernelGGL(k3_matvec, dim3(d.nbt), dim3(TOPT_BLOCK), 0, st, G, p, Ap, pAp ? (int)MV3_DOT : (int)MV3_PLAIN,
                      pAp);
   LAUNCH_TRY();
   return TOPT_OK;
@@ -332,7 +336,7 @@ int topt3_solve(const topt_grid3* g, int32_t precond, const double* Ke_inv, doub
   if (precond == TOPT_PC_JACOBI) hipLaunchKernelGGL(k3_diag, dim3(d.nbn), dim3(TOPT_BLOCK), 0, st, G, dg);
   if (!warm) hipLaunchKernelGGL(k3_fill, dim3(nbv), dim3(TOPT_BLOCK), 0, st, u, (long)d.nd, 0.0);
   hipLaunchKernelGGL(k3_zero_fixed, dim3(nbv), dim3(TOPT_BLOCK), 0, st, u, fix, (long)d.nd);
-  hipLaunchKernelGGL(k3_matvec, dim3(d.nbn), dim3(TOPT_BLOCK), 0, st, G, (const double*)u, Ap, (int)MV3_PLAIN,
+  hipLaunchKernelGGL(k3_matvec, dim3(d.nbt), dim3(TOPT_BLOCK), 0, st, G, (const double*)u, Ap, (int)MV3_PLAIN,
                      (double*)nullptr);
   hipLaunchKernelGGL(k3_cg_init, dim3(d.nbn), dim3(TOPT_BLOCK), 0, st, G, f, (const double*)Ap, u, r, z, p,
                      (const double*)dg, tol, (int)max_iter);
@@ -344,7 +348,7 @@ int topt3_solve(const topt_grid3* g, int32_t precond, const double* Ke_inv, doub
   // iterations in chunks; the state of chunk c is polled while chunk c+1 runs
   auto enqueue = [&](int count, int slot) -> int {
     for (int k = 0; k < count; ++k) {
-      hipLaunchKernelGGL(k3_matvec, dim3(d.nbn), dim3(TOPT_BLOCK), 0, st, G, (const double*)p, Ap, (int)MV3_CG,
+      hipLaunchKernelGGL(k3_matvec, dim3(d.nbt), dim3(TOPT_BLOCK), 0, st, G, (const double*)p, Ap, (int)MV3_CG,
                          (double*)nullptr);
       hipLaunchKernelGGL(k3_cg_update, dim3(d.nbn), dim3(TOPT_BLOCK), 0, st, G, u, r, z, (const double*)p,
                          (const double*)Ap, (const double*)dg, (long)d.nd);

@@ -170,24 +170,124 @@ enum Mv3 : int { MV3_PLAIN = 0, MV3_DOT = 1, MV3_CG = 2 };
 // Ap = K(s) p, fixed rows identity (kernels.py:179-212). MV3_DOT: p.Ap into
 // out[0] (fused_kernels.py:59-94). MV3_CG: skip unless the solve runs; the
 // last block sets alpha = rz / pAp, or stops on |pAp| < 1e-30 (solver.py:114-117).
+// Tiled: a workgroup covers MV3_TX x MV3_TY columns of MV3_TZ nodes (one wave
+// per column, z fastest) and first stages the tile's node patch of p (with
+// its one-node ring, zero outside the grid) and the scales of the elements
+// around it in LDS with coalesced loads. A thread then takes its 8 elements
+// in pairs along z: the 12 nodes a pair touches (36 values) come from LDS
+// into registers, the pair's rows of Ke from scalar loads. The sums run in
+// the order of h8_gather (elements o = 0..7, corners b, components d), so the
+// result is that gather's, bit for bit.
+#define MV3_TX 2
+#define MV3_TY 2
+#define MV3_TZ 64
+static_assert(MV3_TX * MV3_TY * MV3_TZ == TOPT_BLOCK, "one node per thread");
+__host__ __device__ inline int mv3_tiles(int nelx, int nely, int nelz) {
+  return ((nelx + MV3_TX) / MV3_TX) * ((nely + MV3_TY) / MV3_TY) * ((nelz + MV3_TZ) / MV3_TZ);
+}
 __global__ __launch_bounds__(TOPT_BLOCK) void k3_matvec(Grid3 G, const double* p, double* Ap, int mode,
                                                         double* out) {
+  constexpr int PX = MV3_TX + 2, PY = MV3_TY + 2, PZ = MV3_TZ + 2;   // node patch
+  constexpr int EX = MV3_TX + 1, EY = MV3_TY + 1, EZ = MV3_TZ + 1;   // elements around it
+  constexpr int NP = PX * PY * PZ, NE = EX * EY * EZ;
+  __shared__ double sp[NP * 3];
+  __shared__ double ss[NE];
   __shared__ double sh[4];
   if (mode == MV3_CG && G.cg->status != ST_RUN) return;
-  const long nn = (long)(G.nelx + 1) * (G.nely + 1) * (G.nelz + 1);
-  const long n = (long)blockIdx.x * TOPT_BLOCK + threadIdx.x;
+  const int nx1 = G.nelx + 1, ny1 = G.nely + 1, nz1 = G.nelz + 1;
+  const int ntz = (nz1 + MV3_TZ - 1) / MV3_TZ, nty = (ny1 + MV3_TY - 1) / MV3_TY;
+  const int b = (int)blockIdx.x;
+  const int tz = b % ntz, t2 = b / ntz;
+  const int ty = t2 % nty, tx = t2 / nty;
+  const int x0 = tx * MV3_TX, y0 = ty * MV3_TY, z0 = tz * MV3_TZ;
+  const int t = (int)threadIdx.x;
+  const GLB double* P = gp(p);
+  const GLB double* S = gp(G.s);
+  {
+    // every load of the thread issued before its LDS stores
+    constexpr int KP = (NP + TOPT_BLOCK - 1) / TOPT_BLOCK, KE = (NE + TOPT_BLOCK - 1) / TOPT_BLOCK;
+    double v[KP][3], e[KE];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int u = t + k * TOPT_BLOCK;
+      const int px = u / (PY * PZ), py = (u / PZ) % PY, pz = u % PZ;
+      const int jx = x0 - 1 + px, jy = y0 - 1 + py, jz = z0 - 1 + pz;
+      const bool in = u < NP && jx >= 0 && jx < nx1 && jy >= 0 && jy < ny1 && jz >= 0 && jz < nz1;
+      const long m = in ? ((long)jx * ny1 + jy) * nz1 + jz : 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[k][d] = in ? P[3 * m + d] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int u = t + k * TOPT_BLOCK;
+      const int ex = x0 - 1 + u / (EY * EZ), ey = y0 - 1 + (u / EZ) % EY, ez = z0 - 1 + u % EZ;
+      const bool in = u < NE && ex >= 0 && ex < G.nelx && ey >= 0 && ey < G.nely && ez >= 0 && ez < G.nelz;
+      e[k] = in ? S[((long)ex * G.nely + ey) * G.nelz + ez] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int u = t + k * TOPT_BLOCK;
+      if (u < NP) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sp[3 * u + d] = v[k][d];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int u = t + k * TOPT_BLOCK;
+      if (u < NE) ss[u] = e[k];
+    }
+  }
+  __syncthreads();
+  const int w = t / MV3_TZ, cz = t % MV3_TZ;
+  const int cx = w / MV3_TY, cy = w % MV3_TY;
+  const int ix = x0 + cx, iy = y0 + cy, iz = z0 + cz;
+  const bool act = ix < nx1 && iy < ny1 && iz < nz1;
   double dot[1] = {0.0};
-  if (n < nn) {
-    int ix, iy, iz;
-    node3(G, n, ix, iy, iz);
-    double q[27][3], se[8], r[3];
-    gather27<false>(G, p, ix, iy, iz, q);
-    scales8(G, G.s, ix, iy, iz, se, false);
-    h8_gather(G.Ke, se, q, r);
+  if (act) {
+    const CST double* M = cp(G.Ke);
+    double r[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ox = 0; ox < 2; ++ox)
+#pragma unroll
+      for (int oy = 0; oy < 2; ++oy) {
+        // nodes (ix - 1 + ox + bx, iy - 1 + oy + by, iz - 1 + z), bx, by in {0, 1}, z in {0, 1, 2}:
+        // patch position (cx + ox + bx, cy + oy + by, cz + z)
+        double q[2][2][3][3];
+#pragma unroll
+        for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+          for (int by = 0; by < 2; ++by)
+#pragma unroll
+            for (int z = 0; z < 3; ++z) {
+              const int u = ((cx + ox + bx) * PY + (cy + oy + by)) * PZ + (cz + z);
+#pragma unroll
+              for (int d = 0; d < 3; ++d) q[bx][by][z][d] = sp[3 * u + d];
+            }
+#pragma unroll
+        for (int oz = 0; oz < 2; ++oz) {
+          // element (ix - 1 + ox, iy - 1 + oy, iz - 1 + oz); our local node a
+          const int a = h8loc(1 - ox, 1 - oy, 1 - oz);
+          const double se = ss[((cx + ox) * EY + (cy + oy)) * EZ + (cz + oz)];
+          double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+          for (int bb = 0; bb < 8; ++bb) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                acc[c] = fma(M[(3 * a + c) * 24 + 3 * bb + d], q[h8x(bb)][h8y(bb)][oz + h8z(bb)][d], acc[c]);
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) r[c] = fma(se, acc[c], r[c]);
+        }
+      }
+    const long n = ((long)ix * ny1 + iy) * nz1 + iz;
+    const int uc = ((cx + 1) * PY + (cy + 1)) * PZ + (cz + 1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long i = 3 * n + c;
-      const double pc = q[13][c];
+      const double pc = sp[3 * uc + c];
       const double v = fixbit3(G.fix, i) ? pc : r[c];
       gp(Ap)[i] = v;
       dot[0] = fma(pc, v, dot[0]);
