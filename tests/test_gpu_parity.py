@@ -54,6 +54,85 @@ def test_matvec_matches_oracle(shape, dtype):
     assert torch.equal(Ap, Ap2)
 
 
+def _ke_quadrature(D):
+    """Q4 Ke on the unit square by 2x2 Gauss quadrature of B^T D B for a
+    given 3x3 constitutive matrix (nodes BL, BR, TR, TL; DOFs x, y per node)."""
+    g = (0.5 - 0.5 / math.sqrt(3.0), 0.5 + 0.5 / math.sqrt(3.0))
+    Ke = np.zeros((8, 8))
+    for x in g:
+        for y in g:
+            dx = np.array([-(1 - y), 1 - y, y, -y])
+            dy = np.array([-(1 - x), -x, x, 1 - x])
+            B = np.zeros((3, 8))
+            B[0, 0::2] = dx
+            B[1, 1::2] = dy
+            B[2, 0::2] = dy
+            B[2, 1::2] = dx
+            Ke += 0.25 * B.T @ D @ B
+    return Ke
+
+
+def _matvec_case(prob, Ke, dtype, seed):
+    from toptimization_amd import ops
+    edof = O.edof_q4(prob.nelx, prob.nely)
+    rng = np.random.default_rng(seed)
+    fixed_dofs = np.union1d(prob.fixed_dofs, rng.choice(prob.n_dofs, 40, replace=False))
+    fixed = O.fixed_mask(prob.n_dofs, fixed_dofs)
+    rho = rng_density(prob.n_elem, seed)
+    g = ops.make_grid(prob.nelx, prob.nely, Ke, prob.E_min, prob.penalty)
+    fix = ops.fix_mask(prob.n_dofs, fixed_dofs)
+    p = rng.normal(size=prob.n_dofs)
+    ref = O.matvec(p, O.scale(rho, prob.E_min, prob.penalty), Ke, edof, fixed)
+    s = ops.scale(g, _cuda(rho), dtype)
+    return g, fix, s, p, ref
+
+
+# The modal walk (topt_dev.h): walks of 7, 15, 31 and 63 node columns over a
+# grid with several walks and bands (partial last walk and band), scattered
+# fixed DOFs. Every node's sum has the same order whichever wave forms it,
+# so all walk lengths give the same bits.
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_matvec_modal_walks_agree(dtype, monkeypatch):
+    from toptimization_amd import ops
+    prob = mbb(200, 250)
+    g, fix, s, p, ref = _matvec_case(prob, O.ke_q4(prob.E, prob.nu), dtype, seed=5)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    outs = []
+    for walk in ("7", "15", "31", "63"):
+        monkeypatch.setenv("TOPT_MV_WALK", walk)
+        Ap, dot = ops.matvec(g, _cuda(p, dtype), s, fix, with_dot=True)
+        got = Ap.double().cpu().numpy()
+        assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), walk
+        assert float(dot) == pytest.approx(float(p @ ref), rel=tol * 10), walk
+        outs.append(Ap)
+    for a in outs[1:]:
+        assert torch.equal(a, outs[0])
+
+
+# An orthotropic D (c11 != c22, hx != hy in the modal weights) and a Ke with
+# no element structure at all (the LDS-tiled gather serves it).
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("kind", ["orthotropic", "general"])
+def test_matvec_other_stiffness(dtype, kind):
+    from toptimization_amd import ops
+    prob = mbb(70, 130)
+    if kind == "orthotropic":
+        D = np.array([[2.0, 0.4, 0.0], [0.4, 0.7, 0.0], [0.0, 0.0, 0.35]])
+        Ke = _ke_quadrature(D)
+        nu = 0.3
+        Di = 1.0 / (1.0 - nu * nu) * np.array([[1.0, nu, 0.0], [nu, 1.0, 0.0], [0.0, 0.0, (1.0 - nu) / 2]])
+        assert np.abs(_ke_quadrature(Di) - O.ke_q4(1.0, nu)).max() < 1e-15
+    else:
+        A = np.random.default_rng(9).normal(size=(8, 8))
+        Ke = A @ A.T / 8.0
+    g, fix, s, p, ref = _matvec_case(prob, Ke, dtype, seed=6)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    Ap, dot = ops.matvec(g, _cuda(p, dtype), s, fix, with_dot=True)
+    got = Ap.double().cpu().numpy()
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+    assert float(dot) == pytest.approx(float(p @ ref), rel=tol * 10)
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_diagonal_block2_elemblock(dtype):
     from toptimization_amd import ops

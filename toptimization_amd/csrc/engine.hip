@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -74,7 +75,8 @@ Geom geom(int nelx, int nely, int dtype = TOPT_F64) {
   // partials of the widest reduction: node blocks, PCG tiles, op-level matvec tiles
   g.ntree = (int)std::max<int64_t>(
       std::max<int64_t>(g.nb[G_NODE], std::max(pcg_tiles<float>(nelx, nely), pcg_tiles<double>(nelx, nely))),
-      std::max(pcg_tiles<float, NptMv<float>::v>(nelx, nely), pcg_tiles<double, NptMv<double>::v>(nelx, nely)));
+      std::max(std::max(pcg_tiles<float, NptMv<float>::v>(nelx, nely), pcg_tiles<double, NptMv<double>::v>(nelx, nely)),
+               mv_blocks(nelx, nely, TOPT_MV_LMIN)));
   return g;
 }
 
@@ -275,6 +277,19 @@ int fill_material(ProbDev& P, const double* Ke, double E_min, double penal) {
     }
   P.kstruct = (kmax > 0.0 && dev <= 1e-14 * kmax) ? 1 : 0;
   for (int t = 0; t < 6; ++t) P.kc[t] = kc[t];
+  // modal form (k_matvec_modal): the six weights from four entries of the
+  // first two rows, then the whole matrix rebuilt from them and compared
+  double km[6];
+  modal_weights(Ke, km);
+  double mdev = 0.0;
+  for (int j = 0; j < 8; ++j) {
+    double e[8] = {0, 0, 0, 0, 0, 0, 0, 0}, col[8];
+    e[j] = 1.0;
+    modal_apply_host(km, e, col);
+    for (int i = 0; i < 8; ++i) mdev = std::max(mdev, std::fabs(col[i] - Ke[i * 8 + j]));
+  }
+  P.kmodal = (kmax > 0.0 && mdev <= 1e-14 * kmax) ? 1 : 0;
+  for (int t = 0; t < 6; ++t) P.km[t] = km[t];
   return TOPT_OK;
 }
 
@@ -1613,6 +1628,37 @@ int topt_matvec(const topt_grid* g, int32_t dtype, const void* p, void* Ap, cons
   }
   P.s = (void*)s;
   P.fix = fix;
+  if (P.kmodal) {
+    // modal walks of TOPT_MV_WALK node columns, or of the length the
+    // environment variable TOPT_MV_WALK names (3, 7, 15, 31 or 63)
+    int L = TOPT_MV_WALK;
+    const char* fwalk = std::getenv("TOPT_MV_WALK");
+    const int forced = fwalk ? std::atoi(fwalk) : 0;
+    if (forced == 3 || forced == 7 || forced == 15 || forced == 31 || forced == 63) L = forced;
+    P.nb[G_PCG] = (int)mv_blocks(P.nelx, P.nely, L);   // <= geom().ntree (L >= TOPT_MV_LMIN)
+    const dim3 grid(P.nb[G_PCG]), blk(TOPT_BLOCK);
+#define MV_MODAL(TT, LL)                                                                                  \
+  do {                                                                                                    \
+    if (pAp)                                                                                              \
+      hipLaunchKernelGGL((k_matvec_modal<TT, LL, true>), grid, blk, 0, st, P, (const TT*)p, (TT*)Ap, pAp); \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_matvec_modal<TT, LL, false>), grid, blk, 0, st, P, (const TT*)p, (TT*)Ap, pAp); \
+  } while (0)
+#define MV_MODAL_L(TT)             \
+  do {                             \
+    if (L == 63) MV_MODAL(TT, 63); \
+    else if (L == 31) MV_MODAL(TT, 31); \
+    else if (L == 15) MV_MODAL(TT, 15); \
+    else if (L == 7) MV_MODAL(TT, 7); \
+    else MV_MODAL(TT, 3);          \
+  } while (0)
+    if (dtype == TOPT_F32) MV_MODAL_L(float);
+    else MV_MODAL_L(double);
+#undef MV_MODAL_L
+#undef MV_MODAL
+    LAUNCH_TRY();
+    return TOPT_OK;
+  }
   // one block per tile; the tree over those blocks (sized by geom().ntree)
   P.nb[G_PCG] = (int)(dtype == TOPT_F32 ? pcg_tiles<float, NptMv<float>::v>(P.nelx, P.nely)
                                          : pcg_tiles<double, NptMv<double>::v>(P.nelx, P.nely));

@@ -1053,6 +1053,211 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* 
   }
 }
 
+// Op-level K(s) p for a Ke of the modal form (topt_dev.h), same BC rule and
+// optional p.Ap as k_matvec, in the arithmetic of T throughout.
+// Each wave owns 64 consecutive node rows y0 - 1 .. y0 + 62 (one per lane)
+// and walks the node columns X0 - 1 .. X0 + L. At column x it holds the
+// corner values of element (x - 1, y) in registers (the row above through
+// a DPP shift from lane + 1, the left column from the previous step), forms
+// the element's four node contributions, keeps the two of its own row and
+// hands the two of the row above to lane + 1 (DPP shift the other way).
+// Node (x - 1, y) is then complete and lanes 1..62 store it. No LDS, no
+// barrier, no atomics; each element is formed once per walk (bands overlap
+// by two node rows, walks by two node columns). Loads run D columns ahead
+// through a register ring (with L + 1 = D every load of the walk is issued
+// before the first element). Every lane stays active through the walk (the
+// DPP sources); only the stores are predicated.
+#ifndef TOPT_MV_D
+#define TOPT_MV_D 8
+#endif
+template <typename T>
+struct MvWave {
+  const GLB vec2<T>* p;
+  const GLB T* s;
+  const GLB uint32_t* fix;
+  GLB vec2<T>* ap;
+  int nelx, nely, nny, X0, y, yc, eyc;
+  bool erow, slane;
+  T c11, c12, c22, c33, hx, hy;
+};
+
+// Walk length and prefetch depth (every load of a walk of L + 1 <= TOPT_MV_D
+// columns is issued before the first element).
+template <int L>
+struct MvDepth {
+  static constexpr int D = (L + 1) < TOPT_MV_D ? (L + 1) : TOPT_MV_D;
+  static_assert((L + 1) % D == 0, "whole prefetch rounds per walk");
+};
+// Node values of column x and the fixed-bit word of node (x, y) at clamped
+// addresses: a node off the grid (clamped onto it, so its value is finite)
+// only meets elements off the grid, whose scale is zero. Element scales are
+// loaded clamped and zeroed where they are used.
+template <typename T>
+__device__ __forceinline__ vec2<T> mv_ldp(const MvWave<T>& W, int x) {
+  const int xc = x < 0 ? 0 : (x > W.nelx ? W.nelx : x);
+  return W.p[(long)xc * W.nny + W.yc];
+}
+template <typename T>
+__device__ __forceinline__ T mv_lds(const MvWave<T>& W, int ex) {
+  const int xc = ex < 0 ? 0 : (ex >= W.nelx ? W.nelx - 1 : ex);
+  return W.s[(long)xc * W.nely + W.eyc];
+}
+template <typename T>
+__device__ __forceinline__ uint32_t mv_ldf(const MvWave<T>& W, int x) {
+  const int xc = x < 0 ? 0 : (x > W.nelx ? W.nelx : x);
+  return W.fix[((long)xc * W.nny + W.yc) >> 4];
+}
+
+// The walk of one wave from its first D columns (pl: column X0 - 1, pq[d]:
+// column X0 + d, sq[d]: element column X0 - 1 + d, loaded by the caller).
+// FIX: apply the fixed-row rule from the bitmask, loaded at the step that
+// stores the node (the few walks whose bitmask footprint is not all zero;
+// the others skip those loads).
+template <typename T, int L, bool DOT, bool FIX>
+__device__ __forceinline__ double mv_walk(const MvWave<T>& W, vec2<T> (&pq)[MvDepth<L>::D],
+                                          T (&sq)[MvDepth<L>::D], vec2<T> pl) {
+  // no implicit contraction: every element and node sum rounds the same
+  // way at every position of every walk (explicit fma only)
+#pragma clang fp contract(off)
+  constexpr int D = MvDepth<L>::D;
+  const int nelx = W.nelx, nny = W.nny, X0 = W.X0, y = W.y;
+  vec2<T> plu;
+  plu.x = wave_shift<TOPT_DPP_WAVE_SHL1>(pl.x);
+  plu.y = wave_shift<TOPT_DPP_WAVE_SHL1>(pl.y);
+  T dlx = plu.x - pl.x, dly = plu.y - pl.y;   // left column: TL - BL
+  vec2<T> accL;
+  accL.x = accL.y = T(0);
+  double dot = 0.0;
+  // step k (1 .. L + 1): column x = X0 - 1 + k arrives; element (x - 1, y)
+  // is formed; node (x - 1, y) completes (stored from k = 2 on)
+  auto step = [&](const int d, const int k, const bool refill) {
+    const int x = X0 - 1 + k, ex = x - 1;
+    const vec2<T> pc = pq[d];
+    const T sc = (W.erow & (ex >= 0) & (ex < nelx)) ? sq[d] : T(0);
+    uint32_t fw = 0u;
+    if constexpr (FIX) fw = mv_ldf(W, ex);
+    if (refill) {
+      pq[d] = mv_ldp(W, x + D);
+      sq[d] = mv_lds(W, ex + D);
+    }
+    vec2<T> pu;
+    pu.x = wave_shift<TOPT_DPP_WAVE_SHL1>(pc.x);
+    pu.y = wave_shift<TOPT_DPP_WAVE_SHL1>(pc.y);
+    const T dbx = pc.x - pl.x, dby = pc.y - pl.y;      // BR - BL
+    const T dtx = pu.x - plu.x, dty = pu.y - plu.y;    // TR - TL
+    const T drx = pu.x - pc.x, dry = pu.y - pc.y;      // TR - BR
+    const T A1 = dbx + dtx, A2 = dlx + drx, A3 = dtx - dbx;
+    const T B1 = dby + dty, B2 = dly + dry, B3 = dty - dby;
+    const T k11 = sc * W.c11, k12 = sc * W.c12, k22 = sc * W.c22, k33 = sc * W.c33;
+    const T q1 = fma(k11, A1, k12 * B2), q2 = fma(k12, A1, k22 * B2), q3 = k33 * (A2 + B1);
+    const T tx = (sc * W.hx) * A3, ty = (sc * W.hy) * B3;
+    const T Ux = q1 + q3, Vx = q1 - q3, Uy = q2 + q3, Vy = q2 - q3;
+    accL.x += tx - Ux;
+    accL.y += ty - Uy;
+    vec2<T> accC;
+    accC.x = Vx - tx;
+    accC.y = -Vy - ty;
+    accC.x += wave_shift<TOPT_DPP_WAVE_SHR1>(Ux + tx);    // TR of lane - 1's element
+    accC.y += wave_shift<TOPT_DPP_WAVE_SHR1>(Uy + ty);
+    accL.x += wave_shift<TOPT_DPP_WAVE_SHR1>(-Vx - tx);   // TL of lane - 1's element
+    accL.y += wave_shift<TOPT_DPP_WAVE_SHR1>(Vy - ty);
+    if (k >= 2 && ex <= nelx && W.slane) {   // node (ex, y)
+      const long col = (long)ex * nny;
+      vec2<T> o = accL;
+      if constexpr (FIX) {
+        const unsigned fb = (fw >> (2u * (((unsigned)col + (unsigned)y) & 15u))) & 3u;
+        if (fb & 1u) o.x = pl.x;
+        if (fb & 2u) o.y = pl.y;
+      }
+      (W.ap + col)[y] = o;
+      if constexpr (DOT) dot += (double)pl.x * (double)o.x + (double)pl.y * (double)o.y;
+    }
+    pl = pc;
+    plu = pu;
+    dlx = drx;
+    dly = dry;
+    accL = accC;
+  };
+  for (int k0 = 1; k0 < L + 1 - D + 1; k0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) step(d, k0 + d, true);
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) step(d, L + 2 - D + d, false);
+  return dot;
+}
+
+#ifndef TOPT_MV_WPE
+#define TOPT_MV_WPE 1
+#endif
+template <typename T, int L, bool DOT>
+__global__ __launch_bounds__(TOPT_BLOCK, TOPT_MV_WPE) void k_matvec_modal(ProbDev single, const T* __restrict__ pin,
+                                                             T* __restrict__ Ap, double* out) {
+  constexpr int D = MvDepth<L>::D;
+  PD& P = get_prob<false>(nullptr, nullptr, single, 0);
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  MvWave<T> W;
+  W.nelx = P.nelx;
+  W.nely = P.nely;
+  W.nny = P.nely + 1;
+  const int nby = mv_band_groups(W.nely);
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int bx = lb / nby;
+  const int yf = ((lb - bx * nby) * (TOPT_BLOCK / TOPT_WAVE) + w) * TOPT_MV_RB - 1;   // lane 0's row
+  W.y = yf + lane;
+  W.X0 = bx * L;
+  W.erow = (W.y >= 0) & (W.y < W.nely);
+  W.yc = W.y < 0 ? 0 : (W.y > W.nely ? W.nely : W.y);
+  W.eyc = W.y < 0 ? 0 : (W.y >= W.nely ? W.nely - 1 : W.y);
+  W.slane = (lane >= 1) & (lane <= TOPT_MV_RB) & (W.y >= 0) & (W.y <= W.nely);
+  W.c11 = (T)P.km[0];
+  W.c12 = (T)P.km[1];
+  W.c22 = (T)P.km[2];
+  W.c33 = (T)P.km[3];
+  W.hx = (T)P.km[4];
+  W.hy = (T)P.km[5];
+  W.p = gp((const vec2<T>*)pin);
+  W.s = gp((const T*)P.s);
+  W.fix = gp(P.fix);
+  W.ap = gp((vec2<T>*)Ap);
+  // Fixed-bit footprint of the walk: the (at most five) bitmask words of
+  // each of its L + 2 columns over the wave's 64 rows, one word per lane;
+  // an all-zero footprint (most walks) skips the per-node bit loads. Its
+  // load goes first, so the test waits for it alone.
+  constexpr bool FOOT = (L + 2) * 5 <= TOPT_WAVE;
+  uint32_t fv = 0u;
+  bool fok = false;
+  if constexpr (FOOT) {
+    const int c = lane < (L + 2) * 5 ? lane / 5 : 0, kq = lane - 5 * (lane / 5);
+    const int xc = W.X0 - 1 + c < 0 ? 0 : (W.X0 - 1 + c > W.nelx ? W.nelx : W.X0 - 1 + c);
+    const int ylo = yf < 0 ? 0 : (yf > W.nely ? W.nely : yf);
+    const int yhi = yf + 63 > W.nely ? W.nely : yf + 63;
+    const long wlo = ((long)xc * W.nny + ylo) >> 4, whi = ((long)xc * W.nny + yhi) >> 4;
+    fok = (lane < (L + 2) * 5) & (wlo + kq <= whi);
+    fv = W.fix[fok ? wlo + kq : wlo];   // unconditional load, in bounds
+  }
+  // the walk's first D columns, in flight before anything waits
+  vec2<T> pq[D];
+  T sq[D];
+  const vec2<T> pl = mv_ldp(W, W.X0 - 1);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {   // step k = 1 + d: column X0 + d; element and store column X0 - 1 + d
+    pq[d] = mv_ldp(W, W.X0 + d);
+    sq[d] = mv_lds(W, W.X0 - 1 + d);
+  }
+  const bool anyfix = FOOT ? __ballot(fok & (fv != 0u)) != 0ull : true;
+  const double dot = anyfix ? mv_walk<T, L, DOT, true>(W, pq, sq, pl) : mv_walk<T, L, DOT, false>(W, pq, sq, pl);
+  if constexpr (DOT) {
+    __shared__ double sh[16];
+    double v[1] = {dot};
+    block_sum<1>(v, sh);
+    double tot[1];
+    if (tree_reduce<1>(v, P.part, P.ctr[G_NODE], lb, P.nb[G_PCG], tot)) {
+      if (threadIdx.x == 0) out[0] = tot[0];
+    }
+  }
+}
+
 // Op-level element-block Jacobi apply (+ r.z, r.r) for drop-in callers.
 template <typename T>
 __global__ __launch_bounds__(TOPT_BLOCK) void k_elemblock(ProbDev single, const T* __restrict__ rin,

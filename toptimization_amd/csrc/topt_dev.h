@@ -73,6 +73,13 @@ template <typename T> struct Npt { static constexpr int v = sizeof(T) == 4 ? TOP
 #define TOPT_NPT_MV_F64 2
 #endif
 template <typename T> struct NptMv { static constexpr int v = sizeof(T) == 4 ? TOPT_NPT_MV_F32 : TOPT_NPT_MV_F64; };
+// modal op-level matvec: walks of TOPT_MV_WALK node columns (3, 7, 15, 31
+// or 63; short walks keep more loads in flight and no tail of long-running
+// workgroups: at 8192x4096, 7 columns ran 124 us, 15 131 us, 63 146 us)
+#ifndef TOPT_MV_WALK
+#define TOPT_MV_WALK 7
+#endif
+#define TOPT_MV_LMIN 3
 
 // One level of the geometric multigrid hierarchy (PC_MG). Level 0 is the
 // design grid; level l+1 has ceil(nx/2) x ceil(ny/2) elements, each the
@@ -110,6 +117,8 @@ struct ProbDev {
   double kc[6];          // Ke = KS pattern x kc when kstruct (see q4_gather_s)
   int kstruct, oc_max_iter;   // oc_max_iter, oc_tol: the OC stopping rule
   double oc_tol;
+  double km[6];          // Ke = G^T W G with W from km when kmodal (see modal_weights)
+  int kmodal, km_pad;
   // FEM vectors (node interleaved). T = float (mixed) or double (plain).
   void* x;   // CG iterate: aliases the FP64 solution u64 (both paths)
   void* r; void* z; void* p0; void* p1; void* Ap;
@@ -415,6 +424,95 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], double* part, unsigned
                                          double (&tot)[NV], double* sh) {
   block_sum<NV>(v, sh);
   return tree_reduce<NV>(v, part, ctr, lb, nb, tot);
+}
+
+// ------------------------------------------------------------ modal Q4 form
+// On the unit-square bilinear element 2x2 Gauss integrates the strain energy
+// exactly. For a stiffness symmetric under the square's reflections
+// (isotropic plane stress is), the energy splits into the three constant
+// strains and the two hourglass modes, which do not couple:
+//   Ke p = G^T W G p,  G p = (A1, B2, A2 + B1, A3, B3),
+// from differences of the corner values x_k, y_k (k = BL, BR, TR, TL):
+//   A1 = (x_BR - x_BL) + (x_TR - x_TL)   twice the mean x-gradient of u_x
+//   A2 = (x_TL - x_BL) + (x_TR - x_BR)   twice the mean y-gradient of u_x
+//   A3 = (x_TR - x_TL) - (x_BR - x_BL)   hourglass amplitude of u_x
+//   B1, B2, B3 the same for u_y,
+// and W given by six weights km = (c11, c12, c22, c33, hx, hy):
+//   q1 = c11 A1 + c12 B2, q2 = c12 A1 + c22 B2, q3 = c33 (A2 + B1),
+//   tx = hx A3, ty = hy B3;
+//   (Ke p)_BL = (tx - (q1 + q3),  ty - (q2 + q3))
+//   (Ke p)_BR = ((q1 - q3) - tx,  -(q2 - q3) - ty)
+//   (Ke p)_TR = ((q1 + q3) + tx,  (q2 + q3) + ty)
+//   (Ke p)_TL = (-(q1 - q3) - tx, (q2 - q3) - ty).
+// For plane stress with D = E/(1 - nu^2) [[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu)/2]]:
+// c11 = c22 = D11/4, c12 = D12/4, c33 = D33/4, hx = hy = (D11 + D33)/12.
+// About 40 operations per element instead of 64 multiply-adds, every one on
+// differences of neighbouring values, so a common translation of the
+// corners cancels in the first subtraction.
+//
+// The weights from four entries of Ke's first two rows (DOF order BLx, BLy,
+// BRx, BRy, TRx, TRy, TLx, TLy): Ke[0][0] = c11 + c33 + hx,
+// Ke[0][4] = hx - c11 - c33, Ke[0][1] = c12 + c33, Ke[0][3] = c12 - c33,
+// Ke[1][1] = c22 + c33 + hy, Ke[1][5] = hy - c22 - c33.
+inline void modal_weights(const double* Ke, double (&km)[6]) {
+  const double c33 = 0.5 * (Ke[1] - Ke[3]);
+  km[0] = 0.5 * (Ke[0] - Ke[4]) - c33;   // c11
+  km[1] = 0.5 * (Ke[1] + Ke[3]);         // c12
+  km[2] = 0.5 * (Ke[9] - Ke[13]) - c33;  // c22
+  km[3] = c33;
+  km[4] = 0.5 * (Ke[0] + Ke[4]);         // hx
+  km[5] = 0.5 * (Ke[9] + Ke[13]);        // hy
+}
+// Ke p_e of one element in the modal form (host check of a given Ke).
+inline void modal_apply_host(const double (&km)[6], const double* pe, double* out) {
+  const double* x = pe;
+  double xv[4], yv[4];
+  for (int k = 0; k < 4; ++k) {
+    xv[k] = x[2 * k];
+    yv[k] = x[2 * k + 1];
+  }
+  const double A1 = (xv[1] - xv[0]) + (xv[2] - xv[3]), A2 = (xv[3] - xv[0]) + (xv[2] - xv[1]);
+  const double A3 = (xv[2] - xv[3]) - (xv[1] - xv[0]);
+  const double B1 = (yv[1] - yv[0]) + (yv[2] - yv[3]), B2 = (yv[3] - yv[0]) + (yv[2] - yv[1]);
+  const double B3 = (yv[2] - yv[3]) - (yv[1] - yv[0]);
+  const double q1 = km[0] * A1 + km[1] * B2, q2 = km[1] * A1 + km[2] * B2, q3 = km[3] * (A2 + B1);
+  const double tx = km[4] * A3, ty = km[5] * B3;
+  out[0] = tx - (q1 + q3);
+  out[1] = ty - (q2 + q3);
+  out[2] = (q1 - q3) - tx;
+  out[3] = -(q2 - q3) - ty;
+  out[4] = (q1 + q3) + tx;
+  out[5] = (q2 + q3) + ty;
+  out[6] = -(q1 - q3) - tx;
+  out[7] = (q2 - q3) - ty;
+}
+
+// Cross-lane moves of the modal matvec (DPP wave shifts, VALU only): the
+// value of lane + 1 (lane 63 reads 0) and of lane - 1 (lane 0 reads 0).
+#define TOPT_DPP_WAVE_SHL1 0x130
+#define TOPT_DPP_WAVE_SHR1 0x138
+template <int CTRL>
+__device__ __forceinline__ float wave_shift(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Modal matvec walk geometry: a wave owns 64 consecutive node rows and
+// stores the 62 inner ones (bands overlap by two rows); a workgroup holds
+// four bands and one run of L node columns.
+#define TOPT_MV_RB 62
+__host__ __device__ inline int mv_band_groups(int nely) {
+  const int bands = (nely + 1 + TOPT_MV_RB - 1) / TOPT_MV_RB;
+  return (bands + TOPT_BLOCK / TOPT_WAVE - 1) / (TOPT_BLOCK / TOPT_WAVE);
+}
+__host__ __device__ inline long mv_blocks(int nelx, int nely, int L) {
+  return (long)((nelx + 1 + L - 1) / L) * mv_band_groups(nely);
 }
 
 // ------------------------------------------------------------ Q4 stencil
