@@ -73,7 +73,7 @@ def mg_cg_bytes(nelx, nely, ts=4, tail=1024, tail_sum=2560):
     """Algorithmic bytes of one multigrid PCG iteration of the mixed path
     (vectors in ts-byte words, two per node; the FP64 solution u; node
     records of 40 words (the two rows of the assembled level operator and
-    1/diag) at levels 1 .. L-1; the FP64 coarsest inverse):
+    1/diag) at levels 1 .. L-1; the coarsest inverse in ts-byte words):
       slot A        read z, p, u; write p, Ap, u; read s
       slot B        read r, Ap; write r
       down 0        read r, 1/diag, s; write b_1
@@ -98,7 +98,7 @@ def mg_cg_bytes(nelx, nely, ts=4, tail=1024, tail_sum=2560):
     for l in range(1, lt):
         b += 2 * v * nn[l] + rec * nn[l] + v * nn[l + 1]       # down sweep + restriction
         b += 3 * v * nn[l] + rec * nn[l] + v * nn[l + 1]       # up sweep
-    b += v * nn[lt] + sum(rec * nn[l] for l in range(lt, L)) + 8 * (2 * nn[L]) ** 2 \
+    b += v * nn[lt] + sum(rec * nn[l] for l in range(lt, L)) + ts * (2 * nn[L]) ** 2 \
         + v * nn[lt]                                        # tail
     b += 3 * v * nn[0] + ts * ne[0] + v * nn[1]             # level-0 up sweep + slot C
     return b

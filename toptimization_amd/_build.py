@@ -22,6 +22,9 @@ HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # kernel arguments arrive in SGPRs (no scalar load before a kernel's
+    # first dependent load; 2.6 % on the 1200x400 multigrid iteration)
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
     "-Wall", "-Wno-unused-function",
 ]
 

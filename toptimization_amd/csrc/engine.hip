@@ -352,7 +352,7 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
     P.mg_nlev = M.nlev;
     P.mg_ncd = M.ncd;
     P.mg_lt = M.lt;
-    P.mg_inv = (double*)(m + M.inv);
+    P.mg_inv = (void*)(m + M.inv);
     for (int l = 0; l < M.nlev; ++l) {
       MgLevel& V = P.mg[l];
       V.nx = M.nx[l];

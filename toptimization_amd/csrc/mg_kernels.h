@@ -400,7 +400,7 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
     const int nc = 2 * (px[c >> 1] * nny + py[c >> 1]) + (c & 1);
     for (int i = 0; i < N; ++i) {
       const int ni = 2 * (px[i >> 1] * nny + py[i >> 1]) + (i & 1);
-      gp(P.mg_inv)[(long)ni * N + nc] = A[i * N + c];
+      gp((T*)P.mg_inv)[(long)ni * N + nc] = (T)A[i * N + c];
     }
   }
 }
@@ -923,6 +923,22 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   T rec0[TOPT_MG_ST];
   if (REG && t < tot) tail_rec_load<T>(s0, rec0);
   const int row = t % TOPT_MG_DENSE, grp = t / TOPT_MG_DENSE;
+  // this thread's entries of the coarsest inverse (T: FP32 on the mixed
+  // path), fetched with the records, before the down sweeps need the block:
+  // A is symmetric, so row `row` is read as column `row` (consecutive
+  // threads, consecutive addresses); columns past N read column N-1 and
+  // weigh 0 in the solve. With the records in registers (FP32) they are
+  // parked in LDS (each thread's own slice) until the solve.
+  __shared__ T sinv[REG ? JD * NB : 1];
+  T ainv[JD];
+  if (row < N) {
+    const GLB T* A = gp((const T*)P.mg_inv) + row;
+#pragma unroll
+    for (int k = 0; k < JD; ++k) {
+      const int j = grp + k * G;
+      ainv[k] = A[(long)(j < N ? j : N - 1) * N];
+    }
+  }
   // f(g, slot) for each slot of level l held by this thread
   auto each_slot = [&](int l, auto&& f) {
     for (int g = t; g < tot; g += NB) {
@@ -961,6 +977,12 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   } else {
     for (int n = t; n < nL; n += NB) sb[n] = blt[n];
   }
+  if constexpr (REG) {
+    if (row < N) {
+#pragma unroll
+      for (int k = 0; k < JD; ++k) sinv[k * NB + t] = ainv[k];
+    }
+  }
   __syncthreads();
   // down: r_l = b_l - K x_l (x_l = omega D^-1 b_l), then b_{l+1} = P^T r_l
   // and x_{l+1}
@@ -997,17 +1019,13 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
     double* dpart = (double*)sr;
     const T* bL = (const T*)(sb + tot);
     if (row < N) {
-      // A is symmetric, so row `row` is read as column `row` (consecutive
-      // threads, consecutive addresses); columns past N read column N-1 and
-      // weigh 0
-      const GLB double* A = gp(P.mg_inv) + row;
       double acc = 0.0;
 #pragma unroll
       for (int k = 0; k < JD; ++k) {
         const int j = grp + k * G;
         const int jc = j < N ? j : N - 1;
-        const double a = A[(long)jc * N];
-        acc = fma(j < N ? a : 0.0, (double)bL[jc], acc);
+        const T a = REG ? sinv[k * NB + t] : ainv[k];
+        acc = fma(j < N ? (double)a : 0.0, (double)bL[jc], acc);
       }
       dpart[grp * TOPT_MG_DENSE + row] = acc;
     }

@@ -135,10 +135,11 @@ struct ProbDev {
   OcState* oc; double* opart;  // elem / OC-grid tree partials
   unsigned* ctr[4];            // tree counters per grid (G_NODE, G_ELEM, -, G_OC)
   // multigrid preconditioner (PC_MG): levels 0..mg_nlev-1, the last one
-  // solved with the dense inverse mg_inv (mg_ncd x mg_ncd, FP64)
+  // solved with the dense inverse mg_inv (mg_ncd x mg_ncd, in T: FP32 on
+  // the mixed path, FP64 on the plain one)
   int mg_nlev, mg_ncd;
   int mg_lt, mg_pad;          // first level of the single-workgroup tail
-  double* mg_inv;
+  void* mg_inv;
   MgLevel mg[TOPT_MG_MAXLEV];
 };
 
