@@ -648,34 +648,47 @@ const int2* pcg_map(const topt_plan* pl) {
   return pl->n == 1 && pl->pcur == pl->dpfull ? nullptr : pl->pcur;
 }
 
-// fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)
-template <typename T>
-void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
+// The V-cycle's launches (DIRECT: a one-problem plan, whose kernels take
+// its state slot as an argument).
+template <typename T, bool DIRECT>
+void mg_vcycle_k(topt_plan* pl, int par, bool fused, hipStream_t s) {
   const ProbDev* tab = pl->dtab;
   const int nt = pl->npcur;        // live problems' tiles (the PCG map)
   const int2* tmap = pcg_map(pl);
-  // a one-problem plan's level maps are identity maps: passed as null
+  // a one-problem plan's level maps are identity maps: passed as null, and
+  // its state slot goes straight to the kernels
   auto lmap = [&](const int2* m) { return pl->n == 1 ? (const int2*)nullptr : m; };
+  const CgState* cgd = DIRECT ? pl->host[0].cg + par : nullptr;
   // down sweeps fused with the restrictions: b_{l+1} from b_l (b_0 = r)
   if (pl->kstruct)
-    hipLaunchKernelGGL((k_mg_down_restrict0<T, true>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab, lmap(pl->mg_rs[0]),
-                       par);
+    hipLaunchKernelGGL((k_mg_down_restrict0<T, true, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
+                       lmap(pl->mg_rs[0]), par);
   else
-    hipLaunchKernelGGL((k_mg_down_restrict0<T, false>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, tab,
+    hipLaunchKernelGGL((k_mg_down_restrict0<T, false, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
                        lmap(pl->mg_rs[0]), par);
   for (int l = 1; l < pl->mg_lt_max; ++l)
     if (pl->n_rs[l] > 0)
-      hipLaunchKernelGGL((k_mg_down_restrict<T>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, tab, lmap(pl->mg_rs[l]), l,
-                         par);
-  hipLaunchKernelGGL((k_mg_tail<T>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, tab, lmap(pl->mg_tail), par);
+      hipLaunchKernelGGL((k_mg_down_restrict<T, DIRECT>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
+                         lmap(pl->mg_rs[l]), l, par);
+  hipLaunchKernelGGL((k_mg_tail<T, DIRECT>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, cgd, tab,
+                     lmap(pl->mg_tail), par);
   for (int l = pl->mg_lt_max - 1; l >= 1; --l)
     if (pl->n_dn[l] > 0)
-      hipLaunchKernelGGL((k_mg_up<T>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, tab, lmap(pl->mg_dn[l]), l, par);
+      hipLaunchKernelGGL((k_mg_up<T, DIRECT>), dim3(pl->n_dn[l]), dim3(TOPT_BLOCK), 0, s, cgd, tab, lmap(pl->mg_dn[l]),
+                         l, par);
   if (fused) return;
   if (pl->kstruct)
-    hipLaunchKernelGGL((k_mg_up0<T, true>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
+    hipLaunchKernelGGL((k_mg_up0<T, true, DIRECT>), dim3(nt), dim3(TOPT_BLOCK), 0, s, cgd, tab, tmap, par);
   else
-    hipLaunchKernelGGL((k_mg_up0<T, false>), dim3(nt), dim3(TOPT_BLOCK), 0, s, tab, tmap, par);
+    hipLaunchKernelGGL((k_mg_up0<T, false, DIRECT>), dim3(nt), dim3(TOPT_BLOCK), 0, s, cgd, tab, tmap, par);
+}
+// fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)
+template <typename T>
+void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
+  if (pl->n == 1)
+    mg_vcycle_k<T, true>(pl, par, fused, s);
+  else
+    mg_vcycle_k<T, false>(pl, par, fused, s);
 }
 
 template <typename T>

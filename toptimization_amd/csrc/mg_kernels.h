@@ -424,6 +424,15 @@ __device__ __forceinline__ bool mg_stopped(PD& P, int par) {
   const i4 w = *(const CST i4*)&cp((const CgState*)P.cg)[par].it;
   return (int)(P.precond != PC_MG) | (int)(w.y != ST_RUN) | ((int)(w.z == OP_B_REF) & (int)(w.w != 0));
 }
+// The same test with the state slot passed as a kernel argument (one-problem
+// plans, DIRECT): its load leaves with the descriptor's instead of after it.
+template <bool DIRECT>
+__device__ __forceinline__ bool mg_stopped(PD& P, int par, const CgState* cgd) {
+  if constexpr (!DIRECT) return mg_stopped(P, par);
+  typedef int i4 __attribute__((ext_vector_type(4)));
+  const i4 w = *(const CST i4*)&cp(cgd)->it;
+  return (int)(P.precond != PC_MG) | (int)(w.y != ST_RUN) | ((int)(w.z == OP_B_REF) & (int)(w.w != 0));
+}
 // The V-cycle kernels cover a batch through (problem, local block) maps: one
 // map per level and kind, holding only the problems that take that step (a
 // null map: the identity map of a one-problem plan).
@@ -702,15 +711,16 @@ __device__ __forceinline__ void mg_down_restrict_tile(PD& P, int lb, int fnx, in
 }
 
 // Levels 1 .. lt-1: b_{l+1} from b_l.
-template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const ProbDev* tab, const int2* map, int l, int par) {
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const CgState* cgd, const ProbDev* tab, const int2* map,
+                                                                 int l, int par) {
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   // the level's fields enter the test, so they are read with the state (one
   // scalar round trip; a level without buffers has nothing to sweep)
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny, cnx = P.mg[l + 1].nx, cny = P.mg[l + 1].ny;
   const void *st = P.mg[l].st, *b = P.mg[l].b, *dv = P.mg[l].dinv, *bc = P.mg[l + 1].b;
-  if ((int)mg_stopped(P, par) | (int)(st == nullptr) | (int)(b == nullptr) | (int)(dv == nullptr) |
+  if ((int)mg_stopped<DIRECT>(P, par, cgd) | (int)(st == nullptr) | (int)(b == nullptr) | (int)(dv == nullptr) |
       (int)(bc == nullptr) | (int)((lb | fnx | fny | cnx | cny) < 0))
     return;
   const MgRec<T> ek{st};
@@ -720,14 +730,15 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const ProbDev* 
 
 // Level 0: b_1 from the PCG residual r (x0 = omega D^-1 r; K(s) has fixed
 // rows identity, so r1 = 0 on fixed DOFs, and r, hence x0, is zero there).
-template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const ProbDev* tab, const int2* map, int par) {
+template <typename T, bool KST, bool DIRECT>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict0(const CgState* cgd, const ProbDev* tab,
+                                                                  const int2* map, int par) {
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
   // the fields enter the test, so they are read with the state
   const int fnx = P.nelx, fny = P.nely, cnx = P.mg[1].nx, cny = P.mg[1].ny;
   const void *r = P.r, *dv = P.pre, *sc = P.s, *bc = P.mg[1].b;
-  if ((int)mg_stopped(P, par) | (int)(r == nullptr) | (int)(dv == nullptr) | (int)(sc == nullptr) |
+  if ((int)mg_stopped<DIRECT>(P, par, cgd) | (int)(r == nullptr) | (int)(dv == nullptr) | (int)(sc == nullptr) |
       (int)(bc == nullptr) | (int)((lb | fnx | fny | cnx | cny) < 0))
     return;
   const MgElems0<T, KST> ek{P, gp((const T*)sc)};
@@ -754,8 +765,9 @@ static_assert(UpTile::TX * UpTile::TY == TOPT_BLOCK, "one node per thread");
 __host__ __device__ inline int mg_up_tiles(int nx, int ny) {
   return ((nx + UpTile::TX) / UpTile::TX) * ((ny + UpTile::TY) / UpTile::TY);
 }
-template <typename T>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const int2* map, int l, int par) {
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const CgState* cgd, const ProbDev* tab, const int2* map, int l,
+                                                      int par) {
   using U = UpTile;
   __shared__ d2v xs[U::PX * U::PY];
   int lb;
@@ -763,7 +775,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const ProbDev* tab, const 
   // the level's fields enter the test, so they are read with the state
   const int nx = P.mg[l].nx, ny = P.mg[l].ny, cny = P.mg[l + 1].ny;
   const void *st = P.mg[l].st, *bl = P.mg[l].b, *dl = P.mg[l].dinv, *rl = P.mg[l].r, *xup = mg_xup(P, l + 1);
-  if ((int)mg_stopped(P, par) | (int)(st == nullptr) | (int)(bl == nullptr) | (int)(dl == nullptr) |
+  if ((int)mg_stopped<DIRECT>(P, par, cgd) | (int)(st == nullptr) | (int)(bl == nullptr) | (int)(dl == nullptr) |
       (int)(rl == nullptr) | (int)(xup == nullptr) | (int)((lb | nx | ny | cny) < 0))
     return;
   const int nny = ny + 1;
@@ -900,8 +912,9 @@ __device__ __forceinline__ d2v tail_apply(const TailSlot& s, RA ra, const vec2<T
   return d2v{ax, ay};
 }
 
-template <typename T>
-__global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* tab, const int2* map, int par) {
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const CgState* cgd, const ProbDev* tab, const int2* map,
+                                                                int par) {
   constexpr int NB = TOPT_MG_TAIL_BLOCK;
   constexpr bool REG = sizeof(T) == 4;
   constexpr int G = NB / TOPT_MG_DENSE;   // thread groups of the dense solve
@@ -911,7 +924,7 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const ProbDev* t
   static_assert(sizeof(sr) >= sizeof(double) * G * TOPT_MG_DENSE, "dense partial sums fit in sr");
   int lb;
   PD& P = mg_entry(tab, map, blockIdx.x, lb);
-  if (mg_stopped(P, par)) return;
+  if (mg_stopped<DIRECT>(P, par, cgd)) return;
   const int lt = P.mg_lt, L = P.mg_nlev - 1;
   const int t = (int)threadIdx.x;
   auto nodes = [&](int l) { return (P.mg[l].nx + 1) * (P.mg[l].ny + 1); };
@@ -1093,13 +1106,14 @@ __device__ __forceinline__ d2v mg_ring_x(PD& P, long m, const GLB vec2<T>* r, co
 
 // Level 0 up sweep: x' = x0 + (masked) P x_1, z = x' + omega D^-1 (r - K x'),
 // z = 0 on fixed DOFs; x0 = omega D^-1 r, formed again from r.
-template <typename T, bool KST>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const ProbDev* tab, const int2* map, int par) {
+template <typename T, bool KST, bool DIRECT>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const CgState* cgd, const ProbDev* tab, const int2* map,
+                                                       int par) {
   using Tl = Tile<T>;
   constexpr int NPT = Tl::NPT;
   int lb;
   PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
-  if (mg_stopped(P, par)) return;
+  if (mg_stopped<DIRECT>(P, par, cgd)) return;
   __shared__ d2v hal[Tl::HN];
   __shared__ double esc[Tl::EN];
   const TileOrg o = tile_org<T>(P, lb);
