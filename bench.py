@@ -44,6 +44,7 @@ NELX, NELY = 1200, 400
 # the unit's launches, scaled by the copy calibration of the same runs)
 PROFILE_MG = os.path.join(ROOT, "profiles", "multigrid_bridge1200x400_mixed.md")
 PROFILE_SLOTS = os.path.join(ROOT, "profiles", "pcg_slots_bridge1200x400_mixed.md")
+PROFILE_MV = os.path.join(ROOT, "profiles", "matvec_modal_8192x4096.md")
 
 
 def profile_traffic(path, unit):
@@ -633,6 +634,12 @@ def main() -> int:
                      "timing": "HIP events on the launch stream around the PCG chunks of the timed OC "
                                "iterations, replayed from a fresh state (same CG count)"},
         "phase_ms_per_oc_iteration": {k: round(tm[k] / args.steps, 4) for k in ("setup", "pcg", "post")},
+        "matvec_roofline": {"kernel": "k_matvec_modal<float, 7> (op-level K(s) p, 8192x4096, FP32)",
+                            "achieved": round(mvb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(mvb_gbs / HBM_PEAK_GBS, 4),
+                            "traffic": profile_traffic(PROFILE_MV, "matvec_f32_8192x4096"),
+                            "bytes_per_launch": 4 * (4 * 8193 * 4097 + 8192 * 4096),
+                            "us_per_launch": round(mvb_ms * 1e3, 2)},
         "slot_a_roofline": {"kernel": "k_slot_a<float> (PCG slot A)", "achieved": round(gbs_a, 1),
                             "frac": round(gbs_a / HBM_PEAK_GBS, 4),
                             "traffic": profile_traffic(PROFILE_SLOTS, "slot_a_launch"),
