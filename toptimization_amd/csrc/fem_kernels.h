@@ -1070,6 +1070,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_matvec(ProbDev single, const T* 
 #ifndef TOPT_MV_D
 #define TOPT_MV_D 8
 #endif
+// FP32 walks read the element scales and write Ap with nontemporal hints
+// (neither is used again by the walk): 8192x4096 FP32 122.6 -> 118.1 us;
+// the FP64 walk is faster without them (231.7 against 235.8 us).
+template <typename T> struct MvNt { static constexpr bool v = sizeof(T) == 4; };
 template <typename T>
 struct MvWave {
   const GLB vec2<T>* p;
@@ -1100,7 +1104,8 @@ __device__ __forceinline__ vec2<T> mv_ldp(const MvWave<T>& W, int x) {
 template <typename T>
 __device__ __forceinline__ T mv_lds(const MvWave<T>& W, int ex) {
   const int xc = ex < 0 ? 0 : (ex >= W.nelx ? W.nelx - 1 : ex);
-  return W.s[(long)xc * W.nely + W.eyc];
+  if constexpr (MvNt<T>::v) return __builtin_nontemporal_load(W.s + (long)xc * W.nely + W.eyc);
+  else return W.s[(long)xc * W.nely + W.eyc];
 }
 template <typename T>
 __device__ __forceinline__ uint32_t mv_ldf(const MvWave<T>& W, int x) {
@@ -1169,7 +1174,8 @@ __device__ __forceinline__ double mv_walk(const MvWave<T>& W, vec2<T> (&pq)[MvDe
         if (fb & 1u) o.x = pl.x;
         if (fb & 2u) o.y = pl.y;
       }
-      (W.ap + col)[y] = o;
+      if constexpr (MvNt<T>::v) __builtin_nontemporal_store(o, (W.ap + col) + y);
+      else (W.ap + col)[y] = o;
       if constexpr (DOT) dot += (double)pl.x * (double)o.x + (double)pl.y * (double)o.y;
     }
     pl = pc;
