@@ -390,13 +390,14 @@ def mbb_bench(log, iters=100):
             "dtype": "f32 vectors + f64 dots/refresh/design fields", "data": "synthetic"}
 
 
-def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="async", preconditioner=MAIN_PC):
+def serving_bench(log, n=512, rate=20.0, max_active=64, seq_sample=32, mode="auto", preconditioner=MAIN_PC):
     """Serving stream: n seeded requests, Poisson arrivals at `rate`/s, nelx
     uniform over multiples of 16 in [64, 768], aspect 1/4, 1/3 or 1/2, BC
     cantilever / MBB / bridge, volfrac U[0.3, 0.5], rmin 0.008 nelx + 1, run to
     100 OC iterations or max change < 0.01; continuous batching with HBM
     admission (mode 'async': every design runs its own SIMP iterations;
-    'lockstep': one iteration of every active design per engine step).
+    'lockstep': one graph-replayed iteration of every active design per
+    engine step; 'auto': lockstep for multigrid designs, async otherwise).
     Sequential baseline: the first seq_sample requests of the same stream
     served one at a time (designs/hour of that sample)."""
     from toptimization_amd.serving import DesignServer, synthetic_stream
@@ -503,7 +504,7 @@ def main() -> int:
                     help="3-D H8 path: 96x48x48 cantilever OC it/s and 3-D matvec (own JSON line)")
     ap.add_argument("--requests", type=int, default=512)
     ap.add_argument("--seq-sample", type=int, default=32)
-    ap.add_argument("--serving-mode", choices=["async", "lockstep"], default="async")
+    ap.add_argument("--serving-mode", choices=["auto", "async", "lockstep"], default="auto")
     ap.add_argument("--pc", default=None, help="preconditioner of the --batch / --serving runs "
                     "(default: multigrid)")
     args = ap.parse_args()

@@ -77,7 +77,7 @@ def test_cli_serve_jsonl(tmp_path):
         assert abs(rho.mean() - 0.5) < 1e-3
 
 
-@pytest.mark.parametrize("mode", ["async", "lockstep"])
+@pytest.mark.parametrize("mode", ["async", "lockstep", "auto"])
 def test_stream_mixing_plan_classes(mode):
     """Requests naming their own preconditioner (three-slot multigrid,
     two-slot Jacobi and 2x2 block Jacobi) and filter type in one stream: the

@@ -77,3 +77,21 @@ def test_plan_class_groups_by_slot_count_and_filter():
     # a server-wide preconditioner overrides the request's
     p = make_problem(20, 10, preconditioner="jacobi", **kw)
     assert plan_class(p, "multigrid") == keys["multigrid", "density"]
+
+
+def test_server_mode_resolution():
+    """mode='auto' advances multigrid designs in lockstep and the others
+    asynchronously; the explicit modes apply to every design."""
+    from toptimization_amd.serving import DesignServer
+    kw = dict(fixed=[{"nodes": {"type": "edge", "edge": "left"}, "dofs": ["x", "y"]}],
+              loads=[{"nodes": {"type": "point", "location": [20, 5]}, "force": [0.0, -1.0]}])
+    mg = make_problem(20, 10, preconditioner="multigrid", **kw)
+    jac = make_problem(20, 10, preconditioner="jacobi", **kw)
+    auto = DesignServer()
+    assert auto.mode == "auto"
+    assert auto._lockstep(mg) and not auto._lockstep(jac)
+    assert DesignServer(preconditioner="multigrid")._lockstep(jac)
+    assert not DesignServer(mode="async")._lockstep(mg)
+    assert DesignServer(mode="lockstep")._lockstep(jac)
+    with pytest.raises(ValueError):
+        DesignServer(mode="eager")

@@ -32,9 +32,10 @@ def main(argv=None) -> int:
     srv_p.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     srv_p.add_argument("--clock", choices=["wall", "instant"], default="instant",
                        help="wall: honour arrival times; instant: all requests available at t=0")
-    srv_p.add_argument("--mode", choices=["async", "lockstep"], default="async",
+    srv_p.add_argument("--mode", choices=["auto", "async", "lockstep"], default="auto",
                        help="async: every design runs its own SIMP iterations; lockstep: one "
-                            "iteration of every active design per step")
+                            "graph-replayed iteration of every active design per step; auto: "
+                            "lockstep for multigrid designs, async for the others")
     args = ap.parse_args(argv)
     if args.command == "run":
         return _cmd_run(args)
