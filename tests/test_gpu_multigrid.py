@@ -24,8 +24,12 @@ def _solve(prob, precision, rho):
 
 
 # (56, 56): coarsest level 7x7 elements, 128 DOFs, half-bandwidth 19 (the
-# largest the coarse factorization takes)
-@pytest.mark.parametrize("shape", [(30, 12), (61, 23), (96, 32), (56, 56)])
+# largest the coarse factorization takes). The fused slot B + level-0 down
+# sweep restricts on PCG tiles of 8 x 64 nodes, a tile to the coarse nodes
+# over its own nodes: with odd nelx (nely) the coarse grid has one node
+# column (row) past those of the last tile, which that tile also takes;
+# (15, 31) and (63, 127) end the fine grid on the last node of a tile.
+@pytest.mark.parametrize("shape", [(30, 12), (61, 23), (96, 32), (56, 56), (15, 31), (63, 127)])
 @pytest.mark.parametrize("precision,tol,utol", [("fp64", 1e-10, 1e-7), ("mixed", 1e-8, 1e-5)])
 def test_mg_pcg_matches_oracle(shape, precision, tol, utol):
     nelx, nely = shape

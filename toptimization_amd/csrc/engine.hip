@@ -125,7 +125,7 @@ struct MgLayout {
   int nx[TOPT_MG_MAXLEV], ny[TOPT_MG_MAXLEV];
   int64_t K[TOPT_MG_MAXLEV], x[TOPT_MG_MAXLEV], b[TOPT_MG_MAXLEV], r[TOPT_MG_MAXLEV], d[TOPT_MG_MAXLEV];
   int64_t st[TOPT_MG_MAXLEV];
-  int64_t inv = 0, total = 0;
+  int64_t inv = 0, r0b = 0, total = 0;   // r0b: level 0's second residual buffer
 };
 
 int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
@@ -170,6 +170,7 @@ int mg_layout(int nelx, int nely, int64_t ts, MgLayout& M) {
   }
   M.ncd = 2 * (M.nx[L] + 1) * (M.ny[L] + 1);
   M.inv = take(int64_t(M.ncd) * M.ncd * 8);
+  M.r0b = take(nodes(0) * 2 * ts);
   M.total = o;
   return TOPT_OK;
 }
@@ -321,6 +322,7 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
   auto at = [&](int b) { return (void*)(w + L.off[b]); };
   P.x = at(TOPT_BUF_X);
   P.r = at(TOPT_BUF_R);
+  P.r1 = P.r;   // a plan of multigrid problems only points it at the second buffer
   P.z = at(TOPT_BUF_Z);
   P.p0 = at(TOPT_BUF_P0);
   P.p1 = at(TOPT_BUF_P1);
@@ -353,6 +355,7 @@ int fill_prob(ProbDev& P, const topt_problem& p, int dtype) {
     P.mg_ncd = M.ncd;
     P.mg_lt = M.lt;
     P.mg_inv = (void*)(m + M.inv);
+    P.r1 = (void*)(m + M.r0b);
     for (int l = 0; l < M.nlev; ++l) {
       MgLevel& V = P.mg[l];
       V.nx = M.nx[l];
@@ -651,7 +654,7 @@ const int2* pcg_map(const topt_plan* pl) {
 // The V-cycle's launches (DIRECT: a one-problem plan, whose kernels take
 // its state slot as an argument).
 template <typename T, bool DIRECT>
-void mg_vcycle_k(topt_plan* pl, int par, bool fused, hipStream_t s) {
+void mg_vcycle_k(topt_plan* pl, int par, bool fused, hipStream_t s, bool down0) {
   const ProbDev* tab = pl->dtab;
   const int nt = pl->npcur;        // live problems' tiles (the PCG map)
   const int2* tmap = pcg_map(pl);
@@ -659,13 +662,16 @@ void mg_vcycle_k(topt_plan* pl, int par, bool fused, hipStream_t s) {
   // its state slot goes straight to the kernels
   auto lmap = [&](const int2* m) { return pl->n == 1 ? (const int2*)nullptr : m; };
   const CgState* cgd = DIRECT ? pl->host[0].cg + par : nullptr;
-  // down sweeps fused with the restrictions: b_{l+1} from b_l (b_0 = r)
-  if (pl->kstruct)
-    hipLaunchKernelGGL((k_mg_down_restrict0<T, true, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
-                       lmap(pl->mg_rs[0]), par);
-  else
-    hipLaunchKernelGGL((k_mg_down_restrict0<T, false, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
-                       lmap(pl->mg_rs[0]), par);
+  // down sweeps fused with the restrictions: b_{l+1} from b_l (b_0 = r); in
+  // a fused plan level 0's runs inside slot B (k_slot_b_mg)
+  if (down0) {
+    if (pl->kstruct)
+      hipLaunchKernelGGL((k_mg_down_restrict0<T, true, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd,
+                         tab, lmap(pl->mg_rs[0]), par);
+    else
+      hipLaunchKernelGGL((k_mg_down_restrict0<T, false, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd,
+                         tab, lmap(pl->mg_rs[0]), par);
+  }
   for (int l = 1; l < pl->mg_lt_max; ++l)
     if (pl->n_rs[l] > 0)
       hipLaunchKernelGGL((k_mg_down_restrict<T, DIRECT>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
@@ -682,13 +688,14 @@ void mg_vcycle_k(topt_plan* pl, int par, bool fused, hipStream_t s) {
   else
     hipLaunchKernelGGL((k_mg_up0<T, false, DIRECT>), dim3(nt), dim3(TOPT_BLOCK), 0, s, cgd, tab, tmap, par);
 }
-// fused: the level-0 up sweep runs inside slot C (k_slot_c_mg)
+// fused: the level-0 down sweep runs inside slot B (k_slot_b_mg), the
+// level-0 up sweep inside slot C (k_slot_c_mg)
 template <typename T>
-void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s) {
+void mg_vcycle(topt_plan* pl, int par, bool fused, hipStream_t s, bool down0) {
   if (pl->n == 1)
-    mg_vcycle_k<T, true>(pl, par, fused, s);
+    mg_vcycle_k<T, true>(pl, par, fused, s, down0);
   else
-    mg_vcycle_k<T, false>(pl, par, fused, s);
+    mg_vcycle_k<T, false>(pl, par, fused, s, down0);
 }
 
 template <typename T>
@@ -714,11 +721,28 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s, int* stat = nullptr) {
     else
       hipLaunchKernelGGL((k_slot_a<T, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy, qa,
                          pa);
-    if (pl->all_mg) {
-      // slot C fused with the level-0 up sweep
+    if (pl->all_mg && !TOPT_MG_FUSE_B) {
+      // slot C fused with the level-0 up sweep only
       hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                          (int)(pl->q++ & 1), pa ^ 1, (int*)nullptr);
-      mg_vcycle<T>(pl, (int)(pl->q & 1), true, s);
+      mg_vcycle<T>(pl, (int)(pl->q & 1), true, s, true);
+      if (pl->kstruct)
+        hipLaunchKernelGGL((k_slot_c_mg<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
+                           (int)(pl->q++ & 1), pa ^ 1, st_last);
+      else
+        hipLaunchKernelGGL((k_slot_c_mg<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
+                           (int)(pl->q++ & 1), pa ^ 1, st_last);
+      continue;
+    }
+    if (pl->all_mg) {
+      // slot B fused with the level-0 down sweep, slot C with the level-0 up sweep
+      if (pl->kstruct)
+        hipLaunchKernelGGL((k_slot_b_mg<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
+                           (int)(pl->q++ & 1), pa ^ 1);
+      else
+        hipLaunchKernelGGL((k_slot_b_mg<T, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
+                           (int)(pl->q++ & 1), pa ^ 1);
+      mg_vcycle<T>(pl, (int)(pl->q & 1), true, s, false);
       if (pl->kstruct)
         hipLaunchKernelGGL((k_slot_c_mg<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map,
                            (int)(pl->q++ & 1), pa ^ 1, st_last);
@@ -729,7 +753,7 @@ int launch_slots(topt_plan* pl, int count, hipStream_t s, int* stat = nullptr) {
     }
     hipLaunchKernelGGL((k_slot_b<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                        (int)(pl->q++ & 1), pa ^ 1, pl->three_slot ? (int*)nullptr : st_last);
-    if (pl->mg) mg_vcycle<T>(pl, (int)(pl->q & 1), false, s);
+    if (pl->mg) mg_vcycle<T>(pl, (int)(pl->q & 1), false, s, true);
     if (pl->three_slot)
       hipLaunchKernelGGL((k_slot_c<T, true>), dim3(nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, dummy,
                          (int)(pl->q++ & 1), pa ^ 1, st_last);
@@ -1301,6 +1325,10 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
     pl->max_cg = std::max(pl->max_cg, P.max_it);
   }
   for (int g = 0; g < TOPT_NGRID; ++g) pl->nblk[g] = (int)maps[g].size();
+  // the residual alternates between two buffers only where slot B is fused
+  // with the level-0 down sweep
+  if (!pl->all_mg || !TOPT_MG_FUSE_B)
+    for (ProbDev& P : pl->host) P.r1 = P.r;
   auto cleanup = [&](int code) {
     topt_plan_destroy(pl);
     return code;

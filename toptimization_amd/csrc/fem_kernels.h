@@ -665,7 +665,7 @@ __device__ __forceinline__ void slot_end(PD& P, int lb, int q, double v0, double
 // on the fly. Rare (a few times per solve): direct loads.
 template <typename T>
 __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id, bool upd, double alpha,
-                                                        const GLB vec2<T>* pold) {
+                                                        const GLB vec2<T>* pold, GLB vec2<T>* rdst) {
   const GLB d2v* u = gp((const d2v*)P.u64);
   d2v q[3][3];
 #pragma unroll
@@ -694,7 +694,7 @@ __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id
   vec2<T> r;
   r.x = (T)rx;
   r.y = (T)ry;
-  gp((vec2<T>*)P.r)[id.n] = r;
+  rdst[id.n] = r;
   return rx * rx + ry * ry;
 }
 
@@ -703,7 +703,10 @@ __device__ __forceinline__ double slot_refresh_residual(PD& P, const NodeIdx& id
 // fem/solver.py:151-154 fused in); A_REF = residual refresh, and p carried
 // over unchanged into the other buffer. Every slot-A launch of a solve thus
 // swaps the p buffers, so the host passes the parity par (p_old = par ? p1 : p0)
-// and the p halo is fetched together with z, before the state is known.
+// and the p halo is fetched together with z, before the state is known. The
+// refreshed residual goes to the residual buffer of the same parity (r1 when
+// par is set), which the next slot B reads (one buffer unless the plan fuses
+// slot B with the multigrid down sweep).
 // FP32 storage, FP64 registers: the stencil of the stored p is summed with
 // the FP64 Ke, so the inner operator is K(s32) up to the final rounding of
 // Ap (an FP32 Ke and FP32 sums break the rigid-body cancellation and stall
@@ -755,9 +758,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_a(const ProbDev* tab, const
   double v = 0.0;
   if (st.op == OP_A_REF) {
     const bool later = st.upend != 0;   // B_REF applies it; form it on the fly
+    GLB vec2<T>* rdst = gp((vec2<T>*)(par ? P.r1 : P.r));
 #pragma unroll
     for (int j = 0; j < NPT; ++j) {
-      v += slot_refresh_residual<T>(P, S.ids[j], later, st.alpha, pold);
+      v += slot_refresh_residual<T>(P, S.ids[j], later, st.alpha, pold, rdst);
       if (S.ids[j].act) pnew[S.ids[j].n] = op[j];
     }
   } else {

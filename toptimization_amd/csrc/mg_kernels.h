@@ -1154,12 +1154,18 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const CgState* cgd, const
 
 namespace topt {
 
-// ---------------------------------------------------------------- fused slot
-// Plans whose problems all use multigrid run the level-0 up sweep and slot C
-// as one kernel (one launch less per CG iteration; arithmetic of the
-// separate kernels). Slot B is not fused with the down sweep: that sweep
-// needs r around its nodes, which neighbouring workgroups of the same launch
-// update, so it would read old or new values depending on workgroup timing.
+// ---------------------------------------------------------------- fused slots
+#ifndef TOPT_MG_FUSE_B
+#define TOPT_MG_FUSE_B 1   // 0: slot B and the level-0 down sweep as two launches
+#endif
+// Plans whose problems all use multigrid run slot B with the level-0 down
+// sweep and its restriction as one kernel (k_slot_b_mg), and the level-0 up
+// sweep with slot C as another (k_slot_c_mg): 11 launches per CG iteration
+// instead of 13. The down sweep needs the updated residual around its nodes,
+// which neighbouring workgroups of the same launch also update, so r
+// alternates between two buffers: iteration parity pa (the parity of its
+// slot A) reads r from buffer pa and writes the updated r to buffer pa ^ 1,
+// where slot C and the next slot A's residual refresh look for it.
 
 // Level-0 up sweep + slot C: z = x0 + P x_1 + omega D^-1 (r - K x'), z = 0
 // on fixed DOFs, partial r.z (x0 = omega D^-1 r, formed again from r).
@@ -1176,7 +1182,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   PD& P = mg_entry(tab, pmap, b, lb);
   const TileOrg o = tile_org<T>(P, lb);
   const GLB vec2<T>* xc = gp((const vec2<T>*)mg_xup(P, 1));
-  const GLB vec2<T>* r = gp((const vec2<T>*)P.r);
+  const GLB vec2<T>* r = gp((const vec2<T>*)(par ? P.r1 : P.r));   // what k_slot_b_mg wrote
   const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
   const int cny = P.mg[1].ny;
   // state-independent loads first
@@ -1218,6 +1224,221 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
     v += (double)orr[j].x * (double)zo.x + (double)orr[j].y * (double)zo.y;
   }
   slot_end(P, lb, q, v, 0.0, sh);
+}
+
+// r_new = r - alpha Ap of a B_CG step (one rounding per component, the same
+// wherever a node's value is formed); a refresh step keeps r.
+template <typename T>
+__device__ __forceinline__ vec2<T> mg_rnew(vec2<T> r, vec2<T> a, T alpha, bool cg) {
+  vec2<T> o;
+  o.x = cg ? fma(-alpha, a.x, r.x) : r.x;
+  o.y = cg ? fma(-alpha, a.y, r.y) : r.y;
+  return o;
+}
+
+// Slot B + level-0 down sweep + restriction (fem/fused_kernels.py:110-166
+// for the slot part). Workgroup = PCG tile lb (TX = 8 node columns x TY = 64
+// node rows from (x0, y0)), so its partial |r|^2 sums have the tile count
+// slot C expects (two workgroups per tile, each summing all of slot A's
+// partials, measured slower: 13.1 against 11.9 us at 1200x400). par = the
+// parity of the p buffer the preceding slot A wrote, as for slot B. Per tile:
+//   1. before the state arrives: slot A's partials, r_old (buffer par ^ 1),
+//      Ap and 1/diag on the (TX + 3) x (TY + 3) patch from (x0 - 2, y0 - 2)
+//      (clamped into the grid), the (TX + 2) x (TY + 2) element scales from
+//      (x0 - 2, y0 - 2) (0 outside the grid), the tile's own nodes' r_old
+//      and Ap, the residual nodes' fixed bits;
+//   2. slot_begin (slot B's state advance: alpha, breakdown), the deferred
+//      u += alpha p at the own nodes;
+//   3. r_new at the own nodes into buffer par, partial |r_new|^2 (the value
+//      slot B adds, in its position);
+//   4. x0 = omega D^-1 r_new on the patch (LDS), the fine residuals
+//      r_new - K(s) x0 at the (TX + 1) x (TY + 1) nodes from (x0 - 1, y0 - 1)
+//      (fixed DOFs and nodes outside the grid: 0), then b_1 = P^T res at the
+//      coarse nodes (I, J) with x0 <= 2I < x0 + TX, y0 <= 2J < y0 + TY; the
+//      last tile of a row or column also takes the coarse node one further
+//      (2I = nelx + 1 for odd nelx), whose fine neighbours past the tile lie
+//      outside the grid and add 0.
+template <typename T, bool KST>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b_mg(const ProbDev* tab, const int2* pmap, int q, int par) {
+  using Tl = Tile<T>;
+  constexpr int NPT = Tl::NPT;
+  constexpr int TX = Tl::TX, TY = Tl::TY;        // fine nodes of the tile
+  constexpr int CX = TX / 2, CY = TY / 2;        // coarse nodes it owns (one more at the grid's end)
+  constexpr int RX = TX + 1, RY = TY + 1;        // fine residual nodes
+  constexpr int XX = TX + 3, XY = TY + 3;        // x0 patch
+  constexpr int EX = TX + 2, EY = TY + 2;        // element scales
+  constexpr int NX = (XX * XY + TOPT_BLOCK - 1) / TOPT_BLOCK;
+  constexpr int NR = (RX * RY + TOPT_BLOCK - 1) / TOPT_BLOCK;
+  constexpr int NE = (EX * EY + TOPT_BLOCK - 1) / TOPT_BLOCK;
+  static_assert(TX % 2 == 0 && TY % 2 == 0, "tiles start on coarse nodes");
+  static_assert((CX + 1) * (CY + 1) <= TOPT_BLOCK, "one coarse node per thread");
+  __shared__ d2v xs[XX * XY];
+  __shared__ d2v rs[RX * RY];
+  __shared__ double es[EX * EY];
+  __shared__ double sh[16];
+  int lb;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  PD& P = mg_entry(tab, pmap, b, lb);
+  const TileOrg o = tile_org<T>(P, lb);
+  const int nelx = P.nelx, nely = P.nely, nny = nely + 1;
+  const GLB vec2<T>* rold = gp((const vec2<T>*)(par ? P.r : P.r1));
+  GLB vec2<T>* rnew = gp((vec2<T>*)(par ? P.r1 : P.r));
+  const GLB vec2<T>* Ap = gp((const vec2<T>*)P.Ap);
+  const GLB vec2<T>* dv = gp((const vec2<T>*)P.pre);
+  const GLB T* sc = gp((const T*)P.s);
+  const int t = (int)threadIdx.x;
+  // 1. state-independent loads
+  PrevParts pp;
+  parts_fetch(P, q, pp);
+  vec2<T> pr[NX], pa[NX], pd[NX];
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < XX * XY ? u0 : XX * XY - 1;
+    const int px = u / XY, py = u - (u / XY) * XY;
+    int jx = o.x0 - 2 + px, jy = o.y0 - 2 + py;
+    jx = jx < 0 ? 0 : (jx > nelx ? nelx : jx);
+    jy = jy < 0 ? 0 : (jy > nely ? nely : jy);
+    const long m = (long)jx * nny + jy;
+    pr[k] = rold[m];
+    pa[k] = Ap[m];
+    pd[k] = dv[m];
+  }
+  T pe[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < EX * EY ? u0 : EX * EY - 1;
+    const int ex = o.x0 - 2 + u / EY, ey = o.y0 - 2 + (u - (u / EY) * EY);
+    const bool ok = (ex >= 0) & (ex < nelx) & (ey >= 0) & (ey < nely);
+    const int kx = ex < 0 ? 0 : (ex >= nelx ? nelx - 1 : ex), ky = ey < 0 ? 0 : (ey >= nely ? nely - 1 : ey);
+    const T v = sc[(long)kx * nely + ky];
+    pe[k] = ok ? v : T(0);
+  }
+  NodeIdx ids[NPT];
+  vec2<T> orr[NPT], oa[NPT];
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    ids[j] = tile_node<T>(P, o, j);
+    orr[j] = rold[ids[j].n];
+    oa[j] = Ap[ids[j].n];
+  }
+  unsigned rfb[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < RX * RY ? u0 : RX * RY - 1;
+    int i = o.x0 - 1 + u / RY, j = o.y0 - 1 + (u - (u / RY) * RY);
+    i = i < 0 ? 0 : (i > nelx ? nelx : i);
+    j = j < 0 ? 0 : (j > nely ? nely : j);
+    rfb[k] = fixbits(P.fix, (long)i * nny + j);
+  }
+  // 2. the state (slot B's advance)
+  bool upd;
+  const CgState st = slot_begin<T, SK_B>(P, lb, q, pp, sh, &upd);
+  if (upd) {
+    // deferred u += alpha p of the last B_CG; p is the buffer slot A wrote (par)
+    const GLB vec2<T>* p = gp((const vec2<T>*)(par ? P.p1 : P.p0));
+    GLB d2v* u = gp((d2v*)P.u64);
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+      if (!ids[j].act) continue;
+      const long n = ids[j].n;
+      const vec2<T> pj = p[n];
+      d2v un = u[n];
+      un.x = fma(st.alpha, (double)pj.x, un.x);
+      un.y = fma(st.alpha, (double)pj.y, un.y);
+      u[n] = un;
+    }
+  }
+  if (st.status != ST_RUN) return;
+  const bool cg = st.op == OP_B_CG;
+  const T alpha = (T)st.alpha;
+  // 3. own nodes
+  double v1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < NPT; ++j) {
+    if (!ids[j].act) continue;
+    const vec2<T> rn = mg_rnew<T>(orr[j], oa[j], alpha, cg);
+    rnew[ids[j].n] = rn;
+    if (cg) v1 += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+  }
+  // 4. x0 on the patch; r_new at the residual nodes (patch positions 1 .. RX, 1 .. RY)
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < XX * XY) {
+      const vec2<T> rn = mg_rnew<T>(pr[k], pa[k], alpha, cg);
+      xs[u] = mg_x0<T>(rn, pd[k]);
+      const int px = u / XY, py = u - (u / XY) * XY;
+      if (px >= 1 && px <= RX && py >= 1 && py <= RY) rs[(px - 1) * RY + (py - 1)] = d2v{(double)rn.x, (double)rn.y};
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < EX * EY) es[u] = (double)pe[k];
+  }
+  __syncthreads();
+  double kk[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) kk[c] = P.kc[c];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < RX * RY) {
+      const int qx = u / RY, qy = u - (u / RY) * RY;
+      const int i = o.x0 - 1 + qx, j = o.y0 - 1 + qy;
+      const bool inr = i >= 0 && i <= nelx && j >= 0 && j <= nely;
+      // the node's 3 x 3 neighbourhood at patch positions (qx .. qx + 2, qy .. qy + 2),
+      // its four elements at element positions (qx .. qx + 1, qy .. qy + 1)
+      d2v nb[3][3];
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) nb[dx][dy] = xs[(qx + dx) * XY + qy + dy];
+      double s4[2][2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) s4[a][c] = es[(qx + a) * EY + qy + c];
+      double ax, ay;
+      if constexpr (KST) q4_gather_s(kk, s4, nb, ax, ay);
+      else q4_gather<double>(P.Ke64, s4, nb, ax, ay);
+      const d2v rn = rs[u];
+      d2v res = d2v{rn.x - ax, rn.y - ay};
+      if (!inr || (rfb[k] & 1u)) res.x = 0.0;
+      if (!inr || (rfb[k] & 2u)) res.y = 0.0;
+      rs[u] = res;
+    }
+  }
+  __syncthreads();
+  const int cnx = P.mg[1].nx, cny = P.mg[1].ny;
+  const int cxn = o.x0 + TX > nelx ? CX + 1 : CX, cyn = o.y0 + TY > nely ? CY + 1 : CY;
+  if (t < (CX + 1) * (CY + 1)) {
+    const int cx = t / (CY + 1), cy = t - (t / (CY + 1)) * (CY + 1);
+    const int I = o.x0 / 2 + cx, J = o.y0 / 2 + cy;
+    if (cx < cxn && cy < cyn && I <= cnx && J <= cny) {
+      double bx = 0.0, by = 0.0;
+#pragma unroll
+      for (int di = 0; di < 3; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          const int qx = 2 * cx + di, qy = 2 * cy + dj;
+          // residual nodes past the tile (the extra coarse node) lie outside the grid
+          if (qx >= RX || qy >= RY) continue;
+          const double w = (di == 1 ? 1.0 : 0.5) * (dj == 1 ? 1.0 : 0.5);
+          const d2v r = rs[qx * RY + qy];
+          bx = fma(w, r.x, bx);
+          by = fma(w, r.y, by);
+        }
+      vec2<T> ob;
+      ob.x = (T)bx;
+      ob.y = (T)by;
+      gp((vec2<T>*)P.mg[1].b)[(long)I * (cny + 1) + J] = ob;
+    }
+  }
+  slot_end(P, lb, q, 0.0, v1, sh);
 }
 
 }  // namespace topt

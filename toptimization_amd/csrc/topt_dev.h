@@ -122,6 +122,11 @@ struct ProbDev {
   // FEM vectors (node interleaved). T = float (mixed) or double (plain).
   void* x;   // CG iterate: aliases the FP64 solution u64 (both paths)
   void* r; void* z; void* p0; void* p1; void* Ap;
+  // the residual's second buffer: plans whose problems all use multigrid
+  // alternate r between r (parity 0) and r1 (parity 1) once per CG
+  // iteration (the fused slot B + level-0 down sweep reads one and writes
+  // the other); every other plan sets r1 = r
+  void* r1;
   void* pre; void* pre2;   // preconditioner data
   void* s;                 // element scale in T
   double* s64;             // element scale FP64 (mixed: residual refresh)
