@@ -533,6 +533,10 @@ struct topt_plan {
   int2* mg_dg[TOPT_MG_MAXLEV] = {};
   int n_ga[TOPT_MG_MAXLEV] = {}, n_dg[TOPT_MG_MAXLEV] = {};
   bool kstruct = true;    // every problem's Ke follows the unit-square Q4 pattern
+  // one-launch OC (k_oc_coop): its abort word and the number of its blocks
+  // the device holds at once; dbar stays null where it cannot run
+  unsigned* dbar = nullptr;
+  int occ_cap = 0;
   int maxR = 0;
   int max_cg = 0;
   std::vector<ProbDev> host;
@@ -858,9 +862,24 @@ int plan_filter(topt_plan* pl, int mode, hipStream_t s) {
   return TOPT_OK;
 }
 
+// One-launch OC over the `nvb` logical blocks of `map`. False when the plan
+// cannot use it; the caller then enqueues the multi-launch sequence, which
+// gives the same bits.
+bool oc_coop_launch(topt_plan* pl, const int2* map, int nvb, hipStream_t s) {
+  if (pl->dbar == nullptr || nvb <= 0) return false;
+  const int G = std::min(nvb, pl->occ_cap);
+  if (cdiv(nvb, G) > TOPT_OCC_MAXK) return false;
+  hipLaunchKernelGGL((k_oc_coop<5>), dim3(G), dim3(TOPT_BLOCK), 0, s, pl->dtab, map, nvb, pl->dbar);
+  return true;
+}
+
 int plan_oc(topt_plan* pl, hipStream_t s) {
   const ProbDev dummy{};
   constexpr int D = 5;
+  if (oc_coop_launch(pl, grid_of(pl, G_OC).map, grid_of(pl, G_OC).nb, s)) {
+    LAUNCH_TRY();
+    return TOPT_OK;
+  }
   hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n,
                      dummy);
   hipLaunchKernelGGL((k_oc_eval<D, true, true>), dim3(grid_of(pl, G_OC).nb), dim3(TOPT_BLOCK), 0, s, pl->dtab,
@@ -1038,11 +1057,13 @@ int subset_post(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_f
   hipLaunchKernelGGL((k_sens<true>), dim3(ge.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, ge.map, dummy, sens_filter ? 0 : 1);
   hipLaunchKernelGGL((k_filter<true>), dim3(gf.nb), dim3(TOPT_BLOCK), filter_shmem(pl->maxR), s, pl->dtab, gf.map,
                      dummy, (int)(sens_filter ? F_SENS : F_DCAVG));
-  hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, dummy);
-  hipLaunchKernelGGL((k_oc_eval<D, true, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
-  for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
-    hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
-  hipLaunchKernelGGL((k_oc_apply<true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
+  if (!oc_coop_launch(pl, go.map, go.nb, s)) {
+    hipLaunchKernelGGL((k_oc_reset<true>), dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, dummy);
+    hipLaunchKernelGGL((k_oc_eval<D, true, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
+    for (int pass = 0; pass < (50 + D - 1) / D; ++pass)
+      hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
+    hipLaunchKernelGGL((k_oc_apply<true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
+  }
   hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, pl->dstats);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->sub_hs[slot], pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
@@ -1090,6 +1111,7 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
   auto read_stats = [&](const Pending& p) {
     for (int i : p.ids) {
       stats[i] = pl->sub_hs[p.slot][i];
+      if (stats[i].oc_steps < 0) return fail(TOPT_EHIP, "OC bisection: grid barrier timed out");
       done_now[i] += 1;
       cg_now[i] += stats[i].n_cg;
       if (stats[i].done) {
@@ -1098,6 +1120,7 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
         became_done = true;
       }
     }
+    return (int)TOPT_OK;
   };
   pl->q = 0;
   pl->na = 0;
@@ -1136,7 +1159,7 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
     // statistics of OC steps that completed before chunk c
     for (auto it = pend.begin(); it != pend.end();) {
       if (it->ready <= c) {
-        read_stats(*it);
+        if ((rc = read_stats(*it))) return rc;
         it = pend.erase(it);
       } else {
         ++it;
@@ -1186,7 +1209,8 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
   // its solve ended in the last chunk: no later slot A has applied its
   // deferred u update yet, so the next call's first chunk covers it again.
   HIP_TRY(hipStreamSynchronize(s));
-  for (const Pending& p : pend) read_stats(p);
+  for (const Pending& p : pend)
+    if ((rc = read_stats(p))) return rc;
   pl->q = 0;
   pl->na = 0;
   pl->pcur = pl->dpfull;
@@ -1426,6 +1450,18 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
         return cleanup(rc);
     if ((rc = upload(tail, &pl->mg_tail, &pl->n_tail))) return cleanup(rc);
   }
+  // one-launch OC: at least one of its blocks resident per CU
+  {
+    int dev = 0, ncu = 0, occ = 0;
+    if (std::getenv("TOPT_OC_MULTI") == nullptr && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_oc_coop<5>, TOPT_BLOCK, 0) == hipSuccess &&
+        occ >= 1 && ncu >= 1) {
+      if (hipMalloc(&pl->dbar, TOPT_OCC_WORDS * 4) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc OC words"));
+      if (hipMemset(pl->dbar, 0, TOPT_OCC_WORDS * 4) != hipSuccess) return cleanup(fail(TOPT_EHIP, "clear OC words"));
+      pl->occ_cap = std::min(TOPT_OCC_MAXG, occ * ncu);
+    }
+  }
   *out = pl;
   return TOPT_OK;
 }
@@ -1433,6 +1469,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
 int topt_plan_destroy(topt_plan* pl) {
   if (pl == nullptr) return TOPT_OK;
   (void)hipFree(pl->dtab);
+  (void)hipFree(pl->dbar);
   for (int g = 0; g < TOPT_NGRID; ++g) (void)hipFree(pl->dmap[g]);
   (void)hipFree(pl->dstats);
   for (int k = 0; k < 2; ++k) {
@@ -1560,7 +1597,11 @@ int topt_plan_iteration(topt_plan* pl, int32_t sens_filter, topt_iter_stats* sta
     }
     pl->tcalls++;
   }
-  if (stats) std::memcpy(stats, pl->hstats, sizeof(topt_iter_stats) * pl->n);
+  if (stats) {
+    std::memcpy(stats, pl->hstats, sizeof(topt_iter_stats) * pl->n);
+    for (int i = 0; i < pl->n; ++i)
+      if (stats[i].oc_steps < 0) return fail(TOPT_EHIP, "OC bisection: grid barrier timed out");
+  }
   return TOPT_OK;
 }
 

@@ -44,6 +44,8 @@ enum CFlag : int { CF_CG = 0, CF_REFRESH = 1 };
 struct OcState {
   double lo, hi, last_mid, c, vol, change;
   int steps, stopped, iter, done;   // done: the OC stopping rule has been met
+  unsigned ver;                     // bracket version of the one-launch OC (k_oc_coop)
+  unsigned pad;
 };
 
 // Block grids a batch is launched over; each problem owns nb[g] consecutive
