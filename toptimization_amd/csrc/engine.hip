@@ -614,37 +614,32 @@ GridRef grid_of(const topt_plan* pl, int g) { return GridRef{pl->nblk[g], pl->dm
 template <typename T>
 void mg_setup(const ProbDev& P, hipStream_t s) {
   const int L = P.mg_nlev - 1;
-  for (int l = 0; l < L; ++l) {
-    const int nb = (int)cdiv(int64_t(P.mg[l + 1].nx) * P.mg[l + 1].ny, TOPT_BLOCK);
-    if (l == 0)
-      hipLaunchKernelGGL((k_mg_galerkin<T, true, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, l);
-    else
-      hipLaunchKernelGGL((k_mg_galerkin<T, false, false>), dim3(nb), dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, l);
-  }
-  for (int l = 1; l <= L; ++l)
-    hipLaunchKernelGGL((k_mg_diag<T, false>), dim3(cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK)),
-                       dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, l);
+  auto nodes = [&](int l) { return (int)cdiv(int64_t(P.mg[l].nx + 1) * (P.mg[l].ny + 1), TOPT_BLOCK); };
+  auto elems = [&](int l) { return (int)cdiv(int64_t(P.mg[l].nx) * P.mg[l].ny, TOPT_BLOCK); };
+  hipLaunchKernelGGL((k_mg_galerkin<T, true, false>), dim3(elems(1)), dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, 0);
+  for (int l = 1; l < L; ++l)
+    hipLaunchKernelGGL((k_mg_level<T, false>), dim3(elems(l + 1) + nodes(l)), dim3(TOPT_BLOCK), 0, s, nullptr,
+                       nullptr, nullptr, P, l, elems(l + 1));
+  hipLaunchKernelGGL((k_mg_diag<T, false>), dim3(nodes(L)), dim3(TOPT_BLOCK), 0, s, nullptr, nullptr, P, L);
   hipLaunchKernelGGL((k_mg_coarse_setup<T, false>), dim3(1), dim3(TOPT_MG_SETUP_BLOCK), 0, s, nullptr, nullptr, P);
 }
 
-// ... and of every multigrid problem of the plan, each step one launch
-// through the per-level maps (lockstep solves).
+// ... and of every multigrid problem of the plan through the per-level maps
+// (lockstep solves): the Galerkin product 0 -> 1, then one launch per level
+// l >= 1 for the product l -> l+1 (problems with a level l+1) and level l's
+// diagonal and records (problems with a level l), then the coarsest inverses.
 template <typename T>
 void mg_setup_all(topt_plan* pl, hipStream_t s) {
   const ProbDev dummy{};
-  for (int l = 0; l + 1 < TOPT_MG_MAXLEV; ++l) {
-    if (pl->n_ga[l] == 0) continue;
-    if (l == 0)
-      hipLaunchKernelGGL((k_mg_galerkin<T, true, true>), dim3(pl->n_ga[l]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                         pl->mg_ga[l], dummy, l);
-    else
-      hipLaunchKernelGGL((k_mg_galerkin<T, false, true>), dim3(pl->n_ga[l]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
-                         pl->mg_ga[l], dummy, l);
+  if (pl->n_ga[0] > 0)
+    hipLaunchKernelGGL((k_mg_galerkin<T, true, true>), dim3(pl->n_ga[0]), dim3(TOPT_BLOCK), 0, s, pl->dtab,
+                       pl->mg_ga[0], dummy, 0);
+  for (int l = 1; l < TOPT_MG_MAXLEV; ++l) {
+    const int nga = l + 1 < TOPT_MG_MAXLEV ? pl->n_ga[l] : 0, ndg = pl->n_dg[l];
+    if (nga + ndg == 0) continue;
+    hipLaunchKernelGGL((k_mg_level<T, true>), dim3(nga + ndg), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->mg_ga[l],
+                       pl->mg_dg[l], dummy, l, nga);
   }
-  for (int l = 1; l < TOPT_MG_MAXLEV; ++l)
-    if (pl->n_dg[l] > 0)
-      hipLaunchKernelGGL((k_mg_diag<T, true>), dim3(pl->n_dg[l]), dim3(TOPT_BLOCK), 0, s, pl->dtab, pl->mg_dg[l],
-                         dummy, l);
   hipLaunchKernelGGL((k_mg_coarse_setup<T, true>), dim3(pl->n_tail), dim3(TOPT_MG_SETUP_BLOCK), 0, s, pl->dtab,
                      pl->mg_tail, dummy);
 }

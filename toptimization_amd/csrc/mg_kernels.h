@@ -64,11 +64,8 @@ __host__ __device__ inline long mg_kel(int ex, int ey, int ny) { return (long)ex
 // ---------------------------------------------------------------- setup
 // Coarse element matrices of level l+1 from level l (FINE: l = 0, children
 // s_e Ke with fixed DOFs masked). One thread per coarse element, FP64 sums.
-template <typename T, bool FINE, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, const int2* map, ProbDev single,
-                                                           int l) {
-  int lb;
-  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
+template <typename T, bool FINE>
+__device__ __forceinline__ void mg_galerkin_body(PD& P, int lb, int l) {
   const int fnx = P.mg[l].nx, fny = P.mg[l].ny;
   const int cnx = P.mg[l + 1].nx, cny = P.mg[l + 1].ny;
   const long E = (long)lb * TOPT_BLOCK + threadIdx.x;
@@ -140,6 +137,13 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, 
 #pragma unroll
   for (int t = 0; t < 36; ++t) out[(long)t * cny] = (T)kc[t];
 }
+template <typename T, bool FINE, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, const int2* map, ProbDev single,
+                                                           int l) {
+  int lb;
+  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
+  mg_galerkin_body<T, FINE>(P, lb, l);
+}
 
 // 1/diag of the assembled level-l operator (l >= 1); 0 on decoupled DOFs.
 // On levels 1 .. L-1 also the node records (TOPT_MG_ST) the sweeps apply: the
@@ -147,10 +151,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_galerkin(const ProbDev* tab, 
 // its element entries in ascending element order. The entry coupling nodes i and
 // j sums the same (symmetric) element entries in the same order from either
 // side, so the assembled operator is bitwise symmetric.
-template <typename T, bool B>
-__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, const int2* map, ProbDev single, int l) {
-  int lb;
-  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
+template <typename T>
+__device__ __forceinline__ void mg_diag_body(PD& P, int lb, int l) {
   const int nx = P.mg[l].nx, ny = P.mg[l].ny;
   const long n = (long)lb * TOPT_BLOCK + threadIdx.x;
   if (n >= (long)(nx + 1) * (ny + 1)) return;
@@ -218,6 +220,29 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, cons
 #pragma unroll
     for (int w = 0; w < W; ++w) v[w] = rec[i * W + w];
     o[i] = v;
+  }
+}
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_diag(const ProbDev* tab, const int2* map, ProbDev single, int l) {
+  int lb;
+  PD& P = grid_prob<B>(tab, map, single, blockIdx.x, lb);
+  mg_diag_body<T>(P, lb, l);
+}
+
+// The Galerkin product l -> l+1 and the diagonal and node records of level l
+// (l >= 1) in one launch: blocks [0, nga) form the former, the rest the
+// latter. Both only read level l's element matrices, which the launch before
+// wrote; they write disjoint arrays.
+template <typename T, bool B>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_level(const ProbDev* tab, const int2* gmap, const int2* dmap,
+                                                        ProbDev single, int l, int nga) {
+  int lb;
+  if ((int)blockIdx.x < nga) {
+    PD& P = grid_prob<B>(tab, gmap, single, blockIdx.x, lb);
+    mg_galerkin_body<T, false>(P, lb, l);
+  } else {
+    PD& P = grid_prob<B>(tab, dmap, single, (int)blockIdx.x - nga, lb);
+    mg_diag_body<T>(P, lb, l);
   }
 }
 
