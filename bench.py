@@ -29,6 +29,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -128,11 +129,19 @@ def run_path(precision, warmup, steps, log, preconditioner=MAIN_PC):
     for k in range(warmup):
         s = plan.iteration()[0]
         log(f"  [{tag}] warmup it {k + 1}: c={s.compliance:.6e} n_cg={s.n_cg} refines={s.refines}")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = [plan.iteration()[0] for _ in range(steps)]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    # Python's cyclic garbage collector is kept out of the timed loop, as
+    # timeit does: a full collection over torch's objects stalls the host for
+    # milliseconds while the GPU waits for the next launch
+    gc.collect()
+    gc.disable()
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = [plan.iteration()[0] for _ in range(steps)]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        gc.enable()
     ncg = sum(s.n_cg for s in stats)
     log(f"  [{tag}] timed {steps} it: {dt:.3f} s ({steps / dt:.2f} OC it/s), {ncg} CG iterations, "
         f"last c={stats[-1].compliance:.6e}")
@@ -165,16 +174,21 @@ def full_run(log, iters=200):
     prob = bridge(NELX, NELY, "mixed", preconditioner=MAIN_PC)
     st = DesignState(prob)
     plan = Plan([st])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ncg = 0
-    for k in range(iters):
-        s = plan.iteration()[0]
-        ncg += s.n_cg
-        if k % 50 == 49:
-            log(f"  [full run] {k + 1} OC iterations, c={s.compliance:.6e}")
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    gc.collect()
+    gc.disable()   # as in run_path
+    try:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ncg = 0
+        for k in range(iters):
+            s = plan.iteration()[0]
+            ncg += s.n_cg
+            if k % 50 == 49:
+                log(f"  [full run] {k + 1} OC iterations, c={s.compliance:.6e}")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    finally:
+        gc.enable()
     plan.close()
     log(f"  [full run] {iters} OC iterations in {dt:.3f} s ({iters / dt:.1f} OC it/s), {ncg} CG iterations")
     return {"seconds": dt, "n_cg": ncg, "iterations": iters, "final_compliance": s.compliance}

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -857,6 +858,23 @@ int plan_filter(topt_plan* pl, int mode, hipStream_t s) {
   return TOPT_OK;
 }
 
+// Wait for a chunk's event by polling it: a chunk of multigrid iterations
+// lasts a few hundred microseconds and the host must enqueue the next one
+// before the GPU runs dry, which a blocking wait's wake-up does not always
+// allow. Falls back to the blocking wait after TOPT_SPIN_WAIT_US.
+#ifndef TOPT_SPIN_WAIT_US
+#define TOPT_SPIN_WAIT_US 20000
+#endif
+hipError_t wait_event(hipEvent_t ev) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) return e;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(TOPT_SPIN_WAIT_US))
+      return hipEventSynchronize(ev);
+  }
+}
+
 // One-launch OC over the `nvb` logical blocks of `map`. False when the plan
 // cannot use it; the caller then enqueues the multi-launch sequence, which
 // gives the same bits.
@@ -921,7 +939,7 @@ int plan_solve_chunks(topt_plan* pl, hipStream_t s) {
   }
   int wait = 0;
   for (;;) {
-    HIP_TRY(hipEventSynchronize(pl->ev[wait]));
+    HIP_TRY(wait_event(pl->ev[wait]));
     // the snapshot is one chunk old: a problem that had stopped then was
     // still covered by the chunk in flight, so its final state is in
     // cg[q & 1] for the current q; from the next chunk on it is left out
@@ -1149,7 +1167,7 @@ int plan_advance(topt_plan* pl, int sens_filter, int max_chunks, int32_t* phase,
   int64_t chunks = 0;
   while (any_live) {
     const int64_t c = cnext - 2;   // chunk whose snapshot is read now
-    HIP_TRY(hipEventSynchronize(pl->ev[wait]));
+    HIP_TRY(wait_event(pl->ev[wait]));
     ++chunks;
     // statistics of OC steps that completed before chunk c
     for (auto it = pend.begin(); it != pend.end();) {
