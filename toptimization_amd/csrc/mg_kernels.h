@@ -266,7 +266,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_level(const ProbDev* tab, con
 // about 1e-3 of d_k; tau = 1e-3 keeps the coarse solve symmetric positive
 // definite and bounded there, as PCG needs. FP64 levels keep those pivots
 // (tau = 1e-12); genuine pivots are far above tau in both cases.
-#define TOPT_MG_SETUP_BLOCK 128   // >= TOPT_MG_DENSE: one thread per column
+#define TOPT_MG_SETUP_BLOCK 256   // >= TOPT_MG_DENSE: one thread per column (the band assembly uses all)
 #define TOPT_MG_BAND 19           // max half-bandwidth (DOFs) for TOPT_MG_DENSE DOFs
 template <typename T, bool B>
 __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const ProbDev* tab, const int2* map,
@@ -333,7 +333,9 @@ __global__ __launch_bounds__(TOPT_MG_SETUP_BLOCK) void k_mg_coarse_setup(const P
   }
   __syncthreads();
   // banded Cholesky, column k: pivot, then the trailing band (k < j <= i <=
-  // k + w) loses a_ik a_jk / a_kk; column k itself is left unscaled
+  // k + w) loses a_ik a_jk / a_kk; column k itself is left unscaled. (One
+  // wave without block barriers, its loads batched, took 39-45 us here
+  // against 31-34 us: the barrier is not what bounds a column.)
   for (int k = 0; k < N; ++k) {
     const double akk = fmax(fabs(A[k * N + k]), tau * D0[k]);
     const double inv = 1.0 / akk;
