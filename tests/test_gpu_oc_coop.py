@@ -79,3 +79,32 @@ def test_oc_coop_batch_equals_multi_launch():
              cantilever(300, 100, volume_fraction=0.4, filter_radius=3.0, **kw)]
     print("[oc_coop] batch of 3 (265 OC blocks), 3 OC iterations", flush=True)
     _same(_run(probs, 3, multi=False), _run(probs, 3, multi=True))
+
+
+def test_oc_coop_abort_path_reports_error():
+    """A wait limit of one tick of the 100 MHz counter (set from the host)
+    makes the blocks that wait for another block's bracket time out: the
+    kernel raises the plan's abort word, every block leaves its loop, the
+    commit is skipped (rho unchanged), and the iteration returns an error
+    instead of hanging. The problem's reduction counters may be left mid-way,
+    so neither the plan nor the state is used again."""
+    import torch
+    from toptimization_amd._lib import ToptError
+    from toptimization_amd.engine import DesignState, Plan
+    prob = mbb(600, 200, volume_fraction=0.5, filter_radius=4.0, penalty=3.0, max_iterations=50,
+               tolerance=0.0, cg_tolerance=1e-6, max_cg_iterations=20000, preconditioner="multigrid",
+               precision="mixed")
+    st = DesignState(prob)
+    os.environ["TOPT_OC_SPIN_TICKS"] = "1"
+    try:
+        plan = Plan([st])
+    finally:
+        os.environ.pop("TOPT_OC_SPIN_TICKS", None)
+    rho0 = st.rho.clone()
+    print("[oc_coop] 600x200 MBB (59 OC blocks), wait limit 1 tick", flush=True)
+    try:
+        with pytest.raises(ToptError, match="grid barrier timed out"):
+            plan.iteration()
+        assert torch.equal(st.rho, rho0)
+    finally:
+        plan.close()

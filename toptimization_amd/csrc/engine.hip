@@ -387,7 +387,10 @@ __global__ void k_gather_status(const ProbDev* tab, int n, int parity, int* out)
   __threadfence_system();
 }
 
-__global__ void k_gather_stats(const ProbDev* tab, const int* idx, int n, topt_iter_stats* out) {
+// abort_w (may be null): the plan's one-launch OC abort word; when set, every
+// problem reports oc_steps = -1, which the host turns into an error.
+__global__ void k_gather_stats(const ProbDev* tab, const int* idx, int n, topt_iter_stats* out,
+                               const unsigned* abort_w) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = idx ? idx[t] : t;
@@ -401,7 +404,7 @@ __global__ void k_gather_stats(const ProbDev* tab, const int* idx, int n, topt_i
   s.cg_status = st->status;
   s.n_cg = (st->status == ST_BREAK || st->status == ST_MAXIT) ? tab[i].max_it
            : (st->status == ST_ZERO ? 0 : st->it);
-  s.oc_steps = o->steps;
+  s.oc_steps = (abort_w != nullptr && *abort_w != 0u) ? -1 : o->steps;
   s.refines = st->refines;
   s.iter = o->iter;
   s.done = o->done;
@@ -990,7 +993,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
                                  : launch_solve_setup<double>(pl, warm, use_filtered, s);
   if (rc) return rc;
   if ((rc = plan_solve_chunks(pl, s))) return rc;
-  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats);
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats, pl->dbar);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1002,7 +1005,7 @@ int plan_solve(topt_plan* pl, int warm, int use_filtered, int* n_iter, int* stat
 }
 
 int gather_stats(topt_plan* pl, topt_iter_stats* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats);
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, s, pl->dtab, nullptr, pl->n, pl->dstats, pl->dbar);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->hstats, pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1077,7 +1080,7 @@ int subset_post(topt_plan* pl, const std::vector<int>& ids, int slot, int sens_f
       hipLaunchKernelGGL((k_oc_eval<D, true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
     hipLaunchKernelGGL((k_oc_apply<true>), dim3(go.nb), dim3(TOPT_BLOCK), 0, s, pl->dtab, go.map, dummy);
   }
-  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, pl->dstats);
+  hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(n, 256)), dim3(256), 0, s, pl->dtab, pl->sub_di[slot], n, pl->dstats, pl->dbar);
   LAUNCH_TRY();
   HIP_TRY(hipMemcpyAsync(pl->sub_hs[slot], pl->dstats, sizeof(topt_iter_stats) * pl->n, hipMemcpyDeviceToHost, s));
   return TOPT_OK;
@@ -1472,6 +1475,13 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
         occ >= 1 && ncu >= 1) {
       if (hipMalloc(&pl->dbar, TOPT_OCC_WORDS * 4) != hipSuccess) return cleanup(fail(TOPT_ENOMEM, "hipMalloc OC words"));
       if (hipMemset(pl->dbar, 0, TOPT_OCC_WORDS * 4) != hipSuccess) return cleanup(fail(TOPT_EHIP, "clear OC words"));
+      // TOPT_OC_SPIN_TICKS: the limit of each wait in 100 MHz ticks instead
+      // of 0.2 s (tests lower it to exercise the abort path)
+      if (const char* e = std::getenv("TOPT_OC_SPIN_TICKS")) {
+        const unsigned long long lim = std::strtoull(e, nullptr, 10);
+        if (hipMemcpy(pl->dbar + 2, &lim, sizeof(lim), hipMemcpyHostToDevice) != hipSuccess)
+          return cleanup(fail(TOPT_EHIP, "set OC spin limit"));
+      }
       pl->occ_cap = std::min(TOPT_OCC_MAXG, occ * ncu);
     }
   }
@@ -1586,7 +1596,7 @@ int topt_plan_iteration(topt_plan* pl, int32_t sens_filter, topt_iter_stats* sta
     if ((r = plan_filter(pl, sens_filter ? F_SENS : F_DCAVG, cs))) return r;
     if ((r = plan_oc(pl, cs))) return r;
     hipLaunchKernelGGL(k_gather_stats, dim3(cdiv(pl->n, 256)), dim3(256), 0, cs, pl->dtab, nullptr, pl->n,
-                       pl->dstats);
+                       pl->dstats, pl->dbar);
     LAUNCH_TRY();
     return (int)TOPT_OK;
   }, &pl->igraph[m][1], &pl->iexec[m][1]);

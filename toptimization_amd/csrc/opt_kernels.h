@@ -397,16 +397,17 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_volume(ProbDev single, const dou
 // Blocks only wait on problems they hold, after publishing their sums for
 // every held problem, and tree_reduce never waits, so every wait ends once
 // all blocks are resident. Each wait is bounded by TOPT_OCC_SPIN_TICKS of
-// the 100 MHz real-time counter: a block that times out raises the plan's
-// abort word, which ends every other wait, and its problems report
-// steps = -1 with rho unchanged (the host turns that into an error). The
-// abort word stays set. Versions grow by one per pass and are compared
-// modulo 2^32 from the value each block reads on entry.
+// the 100 MHz real-time counter (or the plan's own limit): a block that
+// times out raises the plan's abort word, which ends every other wait; rho
+// stays unchanged, and while the word is set (it stays set) the plan's
+// statistics gather reports oc_steps = -1, which the host turns into an
+// error. Versions grow by one per pass and are compared modulo 2^32 from
+// the value each block reads on entry.
 #define TOPT_OCC_MAXG 256
 #define TOPT_OCC_MAXK 16
 #define TOPT_OCC_PASSES 12                 // >= 11: one chain step and ten 5-step passes reach 50
 #define TOPT_OCC_SPIN_TICKS 20000000ull    // 0.2 s at 100 MHz
-#define TOPT_OCC_WORDS 64                  // plan-level words: [0] abort
+#define TOPT_OCC_WORDS 64                  // plan-level words: [0] abort, [2..3] spin limit (0: default)
 
 template <typename X>
 __device__ __forceinline__ X occ_get(const X* p) {
@@ -418,12 +419,12 @@ __device__ __forceinline__ void occ_put(X* p, X v) {
 }
 
 // Thread 0: wait until the version at ver reaches want. False after a timeout
-// (which raises the abort word) or when the abort word is set.
-__device__ bool occ_wait(const unsigned* ver, unsigned want, unsigned* abort_w) {
+// of lim ticks (which raises the abort word) or when the abort word is set.
+__device__ bool occ_wait(const unsigned* ver, unsigned want, unsigned* abort_w, unsigned long long lim) {
   const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
   while ((int)(occ_get(ver) - want) < 0) {
     if (occ_get(abort_w) != 0u) return false;
-    if (__builtin_amdgcn_s_memrealtime() - t0 > TOPT_OCC_SPIN_TICKS) {
+    if (__builtin_amdgcn_s_memrealtime() - t0 > lim) {
       occ_put(abort_w, 1u);
       return false;
     }
@@ -542,6 +543,9 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_coop(const ProbDev* tab, cons
   const int G = (int)gridDim.x, b = (int)blockIdx.x;
   const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
   const int K = (nvb - b + G - 1) / G;   // logical blocks held (G <= nvb, so K >= 1)
+  // the plan may lower the spin limit (tests of the abort path)
+  const unsigned long long lset = *(const unsigned long long*)(bar + 2);
+  const unsigned long long lim = lset != 0ull ? lset : TOPT_OCC_SPIN_TICKS;
   if (t < K) {
     int lb;
     PD& P = occ_prob(tab, bmap, b + t * G, lb);
@@ -598,7 +602,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_coop(const ProbDev* tab, cons
         int lb;
         PD& P = occ_prob(tab, bmap, b + k * G, lb);
         OcState* o = P.oc;
-        if (!occ_wait(&o->ver, s_v0[k] + (unsigned)pass + 1u, bar)) {
+        if (!occ_wait(&o->ver, s_v0[k] + (unsigned)pass + 1u, bar, lim)) {
           s_ok = 0;
           break;
         }
@@ -617,10 +621,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_oc_coop(const ProbDev* tab, cons
     int lb;
     PD& P = occ_prob(tab, bmap, b + k * G, lb);
     OcState* o = P.oc;
-    if (!s_ok) {
-      if (lb == 0 && t == 0) o->steps = -1;
-      continue;
-    }
+    if (!s_ok) continue;   // aborted: rho stays as it was
     const long ne = (long)P.nelx * P.nely;
     const double lam = s_mid[k], mv = P.move;
     double vsum = 0.0, vmax = 0.0;
