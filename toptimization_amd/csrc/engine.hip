@@ -567,8 +567,12 @@ struct topt_plan {
   // status snapshot into dstat[slot]
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  hipGraph_t mgraph[2] = {nullptr, nullptr};       // multigrid plans: one 2-iteration chunk
+  hipGraph_t mgraph[2] = {nullptr, nullptr};       // multigrid plans: one TOPT_MG_CHUNK-iteration chunk
   hipGraphExec_t mgexec[2] = {nullptr, nullptr};
+  hipGraph_t mgraph2[2] = {nullptr, nullptr};      // ... and a 2-iteration chunk (near the expected stop)
+  hipGraphExec_t mgexec2[2] = {nullptr, nullptr};
+  // CG iterations of each problem's last lockstep solve (-1: none yet)
+  std::vector<int> last_cg;
   // lockstep SIMP iteration (topt_plan_iteration), per filter mode: the
   // launches before the PCG (density filter, solve setup) and after it
   // (sensitivity, dc filter, OC, statistics gather)
@@ -816,14 +820,17 @@ int enqueue_chunk(topt_plan* pl, int count, int slot, hipStream_t s) {
   const bool whole = (pl->q & 1) == 0 && (pl->na & 1) == 0 && pl->pcur == pl->dpfull;
   const bool user_graph = pl->use_graph && count == TOPT_GRAPH_CHUNK && whole;
   // a multigrid iteration is ~28 launches: replaying them cuts the host's share
-  const bool mg_graph = pl->mg && count == TOPT_MG_CHUNK && whole;
+  const bool mg_graph = pl->mg && (count == TOPT_MG_CHUNK || count == 2) && whole;
   if (user_graph || mg_graph) {
     if (user_graph) {
       if ((rc = chunk_graph(pl, count, slot, &pl->graph[slot], &pl->gexec[slot]))) return rc;
       HIP_TRY(hipGraphLaunch(pl->gexec[slot], s));
-    } else {
+    } else if (count == TOPT_MG_CHUNK) {
       if ((rc = chunk_graph(pl, count, slot, &pl->mgraph[slot], &pl->mgexec[slot]))) return rc;
       HIP_TRY(hipGraphLaunch(pl->mgexec[slot], s));
+    } else {
+      if ((rc = chunk_graph(pl, count, slot, &pl->mgraph2[slot], &pl->mgexec2[slot]))) return rc;
+      HIP_TRY(hipGraphLaunch(pl->mgexec2[slot], s));
     }
     const int per_it = pl->three_slot ? 3 : 2;
     pl->q += (int64_t)per_it * count;
@@ -935,11 +942,22 @@ int plan_solve_chunks(topt_plan* pl, hipStream_t s) {
   // the overshoot past convergence small
   int chunk = pl->mg ? TOPT_MG_CHUNK : 4;
   const int chunk_max = pl->mg ? TOPT_MG_CHUNK : TOPT_GRAPH_CHUNK;
+  // Multigrid: the host sees a stop one chunk late, and the launches of the
+  // chunk in flight then run empty (~20 us per iteration). Once the chunk in
+  // flight is expected to reach the CG count of the problems' previous
+  // solves, chunks of 2 iterations halve that overshoot.
+  int expect = -1;
+  if (pl->mg)
+    for (int i = 0; i < pl->n; ++i)
+      if (pl->last_cg[i] >= 0) expect = expect < 0 ? pl->last_cg[i] : std::min(expect, pl->last_cg[i]);
+  std::vector<int> seen(pl->n, 0);   // CG iterations of each problem as of the snapshot read last
+  int inflight = 0;                  // CG iterations enqueued after the chunk whose snapshot was read
   for (int k = 0; k < 2; ++k) {
     rc = enqueue_chunk(pl, chunk, k, s);
     if (rc) return rc;
     issued += chunk;
   }
+  inflight = chunk;
   int wait = 0;
   for (;;) {
     HIP_TRY(wait_event(pl->ev[wait]));
@@ -947,11 +965,18 @@ int plan_solve_chunks(topt_plan* pl, hipStream_t s) {
     // still covered by the chunk in flight, so its final state is in
     // cg[q & 1] for the current q; from the next chunk on it is left out
     bool running = false, dropped = false;
+    int it_min = -1;   // least CG count among the problems still running
     for (int i = 0; i < pl->n; ++i) {
       if (!pl->live[i]) continue;
+      const int it_i = pl->hstat[wait][2 * i + 1];
       if (pl->hstat[wait][2 * i] == ST_RUN) {
         running = true;
-      } else if (!pl->use_graph) {
+        it_min = it_min < 0 ? it_i : std::min(it_min, it_i);
+      } else if (!seen[i]) {
+        pl->last_cg[i] = it_i;
+      }
+      if (pl->hstat[wait][2 * i] != ST_RUN) seen[i] = 1;
+      if (pl->hstat[wait][2 * i] != ST_RUN && !pl->use_graph) {
         pl->live[i] = 0;
         pl->hpar[i] = (int)(pl->q & 1);
         dropped = true;
@@ -970,9 +995,11 @@ int plan_solve_chunks(topt_plan* pl, hipStream_t s) {
       pl->npcur = m;
     }
     chunk = std::min(chunk * 2, chunk_max);
+    if (pl->mg && expect >= 0 && it_min >= 0 && it_min + inflight + chunk > expect) chunk = 2;
     rc = enqueue_chunk(pl, chunk, wait, s);
     if (rc) return rc;
     issued += chunk;
+    inflight = chunk;
     wait ^= 1;
   }
   // problems still in the map end with the chunk in flight, at q & 1 (hpar
@@ -1416,6 +1443,7 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
   // PCG grid: (problem, local tile) pairs in the order of the block map
   const int npcg = pl->nblk[G_PCG];
   pl->live.assign(nprob, 1);
+  pl->last_cg.assign(nprob, -1);
   pl->dpfull = pl->dmap[G_PCG];
   for (int k = 0; k < 2; ++k) {
     if (hipMalloc(&pl->dpmap[k], sizeof(int2) * npcg) != hipSuccess)
@@ -1520,6 +1548,8 @@ int topt_plan_destroy(topt_plan* pl) {
     if (pl->graph[k]) (void)hipGraphDestroy(pl->graph[k]);
     if (pl->mgexec[k]) (void)hipGraphExecDestroy(pl->mgexec[k]);
     if (pl->mgraph[k]) (void)hipGraphDestroy(pl->mgraph[k]);
+    if (pl->mgexec2[k]) (void)hipGraphExecDestroy(pl->mgexec2[k]);
+    if (pl->mgraph2[k]) (void)hipGraphDestroy(pl->mgraph2[k]);
     for (int j = 0; j < 2; ++j) {
       if (pl->iexec[k][j]) (void)hipGraphExecDestroy(pl->iexec[k][j]);
       if (pl->igraph[k][j]) (void)hipGraphDestroy(pl->igraph[k][j]);
