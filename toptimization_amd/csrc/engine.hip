@@ -532,6 +532,9 @@ struct topt_plan {
   int n_dn[TOPT_MG_MAXLEV] = {}, n_rs[TOPT_MG_MAXLEV] = {};
   int2* mg_tail = nullptr;
   int n_tail = 0;
+  // one-problem plans: the last two down sweeps above the tail (levels dr2_l
+  // and dr2_l + 1) in one launch of n_dr2 workgroups (k_mg_down2); -1: off
+  int dr2_l = -1, n_dr2 = 0;
   // setup maps: Galerkin l -> l+1 (level l+1 elements), diagonal of level l
   int2* mg_ga[TOPT_MG_MAXLEV] = {};
   int2* mg_dg[TOPT_MG_MAXLEV] = {};
@@ -679,10 +682,17 @@ void mg_vcycle_k(topt_plan* pl, int par, bool fused, hipStream_t s, bool down0) 
       hipLaunchKernelGGL((k_mg_down_restrict0<T, false, DIRECT>), dim3(pl->n_rs[0]), dim3(TOPT_BLOCK), 0, s, cgd,
                          tab, lmap(pl->mg_rs[0]), par);
   }
-  for (int l = 1; l < pl->mg_lt_max; ++l)
-    if (pl->n_rs[l] > 0)
-      hipLaunchKernelGGL((k_mg_down_restrict<T, DIRECT>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
-                         lmap(pl->mg_rs[l]), l, par);
+  for (int l = 1; l < pl->mg_lt_max; ++l) {
+    if (pl->n_rs[l] == 0) continue;
+    if (l == pl->dr2_l) {   // levels l and l + 1 together
+      hipLaunchKernelGGL((k_mg_down2<T, DIRECT>), dim3(pl->n_dr2), dim3(TOPT_BLOCK), 0, s, cgd, tab,
+                         (const int2*)nullptr, l, par);
+      ++l;
+      continue;
+    }
+    hipLaunchKernelGGL((k_mg_down_restrict<T, DIRECT>), dim3(pl->n_rs[l]), dim3(TOPT_BLOCK), 0, s, cgd, tab,
+                       lmap(pl->mg_rs[l]), l, par);
+  }
   hipLaunchKernelGGL((k_mg_tail<T, DIRECT>), dim3(pl->n_tail), dim3(TOPT_MG_TAIL_BLOCK), 0, s, cgd, tab,
                      lmap(pl->mg_tail), par);
   for (int l = pl->mg_lt_max - 1; l >= 1; --l)
@@ -1493,6 +1503,14 @@ int topt_plan_create(const topt_problem* probs, int32_t nprob, int32_t dtype, to
           (rc = upload(ga[l], &pl->mg_ga[l], &pl->n_ga[l])) || (rc = upload(dg[l], &pl->mg_dg[l], &pl->n_dg[l])))
         return cleanup(rc);
     if ((rc = upload(tail, &pl->mg_tail, &pl->n_tail))) return cleanup(rc);
+    // the two-level down sweep: one-problem plans whose tail starts at level
+    // 3 or deeper (TOPT_MG_NODOWN2 keeps the separate sweeps)
+    const ProbDev& P0 = pl->host[0];
+    if (nprob == 1 && P0.precond == PC_MG && P0.mg_lt >= 3 &&
+        std::getenv("TOPT_MG_NODOWN2") == nullptr) {
+      pl->dr2_l = P0.mg_lt - 2;
+      pl->n_dr2 = mg_dr2_tiles(P0.mg[P0.mg_lt].nx, P0.mg[P0.mg_lt].ny);
+    }
   }
   // one-launch OC: at least one of its blocks resident per CU
   {

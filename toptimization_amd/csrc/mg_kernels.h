@@ -755,6 +755,182 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down_restrict(const CgState* 
                                   gp((vec2<T>*)bc));
 }
 
+// Two down sweeps in one launch: levels l and l+1 (the last two above the
+// tail of a one-problem plan), b_{l+2} from b_l. A workgroup covers a tile of
+// Dr2Tile::CX x CY level-(l+2) nodes. It forms b_{l+1} on the level-(l+1)
+// patch its own sweep reads (XX2 x XY2 nodes) as k_mg_down_restrict forms it
+// at level l (x_l on the level-l patch, the residuals from the node records,
+// the restriction sums, rounded to T), stores the level-(l+1) nodes it owns
+// (rows [2 I0, 2 I0 + 2 CX), columns [2 J0, 2 J0 + 2 CY) of that level: every
+// node once over the tiles) for the up sweep, then sweeps level l+1 from the
+// patch. Each value is formed by the same operations on the same operands as
+// in the two separate sweeps, so the bits are theirs; patch positions
+// outside level l+1 hold 0 where the separate sweep holds a clamped node's
+// value, and both meet only zero coefficients there.
+struct Dr2Tile {
+  static constexpr int CX = 2, CY = 8;                       // level l+2 nodes
+  static constexpr int RX2 = 2 * CX + 1, RY2 = 2 * CY + 1;   // level l+1 residuals (5 x 17)
+  static constexpr int XX2 = RX2 + 2, XY2 = RY2 + 2;         // level l+1 patch (7 x 19)
+  static constexpr int RX1 = 2 * XX2 + 1, RY1 = 2 * XY2 + 1; // level l residuals (15 x 39)
+  static constexpr int XX1 = RX1 + 2, XY1 = RY1 + 2;         // level l patch (17 x 41)
+};
+static_assert(Dr2Tile::RX2 * Dr2Tile::RY2 <= TOPT_BLOCK && Dr2Tile::XX2 * Dr2Tile::XY2 <= TOPT_BLOCK,
+              "one level-(l+1) residual and patch node per thread");
+// Workgroups of the two-level sweep for a level l+2 of cnx x cny elements.
+__host__ __device__ inline int mg_dr2_tiles(int cnx, int cny) {
+  return ((cnx + Dr2Tile::CX) / Dr2Tile::CX) * ((cny + Dr2Tile::CY) / Dr2Tile::CY);
+}
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down2(const CgState* cgd, const ProbDev* tab, const int2* map,
+                                                         int l, int par) {
+  using D = Dr2Tile;
+  constexpr int N1R = (D::RX1 * D::RY1 + TOPT_BLOCK - 1) / TOPT_BLOCK;   // level-l residuals per thread
+  constexpr int N1X = (D::XX1 * D::XY1 + TOPT_BLOCK - 1) / TOPT_BLOCK;   // level-l patch nodes per thread
+  __shared__ d2v xs1[D::XX1 * D::XY1];
+  __shared__ d2v rs1[D::RX1 * D::RY1];
+  __shared__ vec2<T> b2[D::XX2 * D::XY2];
+  __shared__ d2v xs2[D::XX2 * D::XY2];
+  __shared__ d2v rs2[D::RX2 * D::RY2];
+  int lb;
+  PD& P = mg_entry(tab, map, xcd_remap(blockIdx.x, gridDim.x), lb);
+  const int n0x = P.mg[l].nx, n0y = P.mg[l].ny, n1x = P.mg[l + 1].nx, n1y = P.mg[l + 1].ny;
+  const int n2x = P.mg[l + 2].nx, n2y = P.mg[l + 2].ny;
+  const void *st0 = P.mg[l].st, *b0 = P.mg[l].b, *dv0 = P.mg[l].dinv;
+  const void *st1 = P.mg[l + 1].st, *b1 = P.mg[l + 1].b, *dv1 = P.mg[l + 1].dinv, *bo = P.mg[l + 2].b;
+  if ((int)mg_stopped<DIRECT>(P, par, cgd) | (int)(st0 == nullptr) | (int)(b0 == nullptr) | (int)(dv0 == nullptr) |
+      (int)(st1 == nullptr) | (int)(b1 == nullptr) | (int)(dv1 == nullptr) | (int)(bo == nullptr) |
+      (int)((lb | n0x | n0y | n1x | n1y | n2x | n2y) < 0))
+    return;
+  const GLB vec2<T>* bl = gp((const vec2<T>*)b0);
+  const GLB vec2<T>* dl = gp((const vec2<T>*)dv0);
+  const GLB vec2<T>* dm = gp((const vec2<T>*)dv1);
+  const int nty = (n2y + D::CY) / D::CY;
+  const int tx = lb / nty, ty = lb - tx * nty;
+  const int I0 = tx * D::CX, J0 = ty * D::CY;                  // level l+2
+  const int t = (int)threadIdx.x;
+  const int n0ny = n0y + 1, n1ny = n1y + 1;
+  // level l+1 residual node of this thread (its record goes first: it does
+  // not depend on the level-l stage)
+  const bool has2 = t < D::RX2 * D::RY2;
+  const int q2x = has2 ? t / D::RY2 : 0, q2y = has2 ? t - (t / D::RY2) * D::RY2 : 0;
+  const int i2 = 2 * I0 - 1 + q2x, j2 = 2 * J0 - 1 + q2y;
+  const bool in2 = has2 && i2 >= 0 && i2 <= n1x && j2 >= 0 && j2 <= n1y;
+  const long m2 = (long)(i2 < 0 ? 0 : (i2 > n1x ? n1x : i2)) * n1ny + (j2 < 0 ? 0 : (j2 > n1y ? n1y : j2));
+  T rec2[TOPT_MG_ST];
+  mg_rec_load<T>(st1, m2, rec2);
+  // level-(l+1) patch node of this thread: its 1/diag
+  const bool hasp = t < D::XX2 * D::XY2;
+  const int ppx = hasp ? t / D::XY2 : 0, ppy = hasp ? t - (t / D::XY2) * D::XY2 : 0;
+  const int kx = 2 * I0 - 2 + ppx, ky = 2 * J0 - 2 + ppy;      // level l+1
+  const bool inp = hasp && kx >= 0 && kx <= n1x && ky >= 0 && ky <= n1y;
+  const vec2<T> pd2 = dm[(long)(kx < 0 ? 0 : (kx > n1x ? n1x : kx)) * n1ny + (ky < 0 ? 0 : (ky > n1y ? n1y : ky))];
+  // level-l residual nodes of this thread: records and b
+  T rec1[N1R][TOPT_MG_ST];
+  vec2<T> bn1[N1R];
+  int q1x[N1R], q1y[N1R];
+  bool in1[N1R];
+#pragma unroll
+  for (int k = 0; k < N1R; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < D::RX1 * D::RY1 ? u0 : D::RX1 * D::RY1 - 1;
+    q1x[k] = u / D::RY1;
+    q1y[k] = u - (u / D::RY1) * D::RY1;
+    const int i = 4 * I0 - 5 + q1x[k], j = 4 * J0 - 5 + q1y[k];
+    in1[k] = u0 < D::RX1 * D::RY1 && i >= 0 && i <= n0x && j >= 0 && j <= n0y;
+    const long mn = (long)(i < 0 ? 0 : (i > n0x ? n0x : i)) * n0ny + (j < 0 ? 0 : (j > n0y ? n0y : j));
+    bn1[k] = bl[mn];
+    mg_rec_load<T>(st0, mn, rec1[k]);
+  }
+  // x_l on the level-l patch (clamped into the grid, as the separate sweep)
+  vec2<T> pb[N1X], pd[N1X];
+#pragma unroll
+  for (int k = 0; k < N1X; ++k) {
+    const int u0 = t + k * TOPT_BLOCK;
+    const int u = u0 < D::XX1 * D::XY1 ? u0 : D::XX1 * D::XY1 - 1;
+    const int px = u / D::XY1, py = u - (u / D::XY1) * D::XY1;
+    int jx = 4 * I0 - 6 + px, jy = 4 * J0 - 6 + py;
+    jx = jx < 0 ? 0 : (jx > n0x ? n0x : jx);
+    jy = jy < 0 ? 0 : (jy > n0y ? n0y : jy);
+    const long m = (long)jx * n0ny + jy;
+    pb[k] = bl[m];
+    pd[k] = dl[m];
+  }
+#pragma unroll
+  for (int k = 0; k < N1X; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < D::XX1 * D::XY1)
+      xs1[u] = d2v{TOPT_MG_OMEGA * (double)pd[k].x * (double)pb[k].x, TOPT_MG_OMEGA * (double)pd[k].y * (double)pb[k].y};
+  }
+  __syncthreads();
+  // level-l residuals
+#pragma unroll
+  for (int k = 0; k < N1R; ++k) {
+    const int u = t + k * TOPT_BLOCK;
+    if (u < D::RX1 * D::RY1) {
+      const int qx = q1x[k], qy = q1y[k];
+      const d2v a2 = mg_rec_apply<T>(rec1[k], [&](int q) { return xs1[(qx + q / 3) * D::XY1 + qy + q % 3]; });
+      d2v r = d2v{(double)bn1[k].x - a2.x, (double)bn1[k].y - a2.y};
+      if (!in1[k]) r = d2v{0.0, 0.0};
+      rs1[u] = r;
+    }
+  }
+  __syncthreads();
+  // b_{l+1} on the level-(l+1) patch (rounded to T), stored where owned
+  if (hasp) {
+    vec2<T> o;
+    o.x = T(0);
+    o.y = T(0);
+    if (inp) {
+      double bx = 0.0, by = 0.0;
+#pragma unroll
+      for (int di = 0; di < 3; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          const double w = (di == 1 ? 1.0 : 0.5) * (dj == 1 ? 1.0 : 0.5);
+          const d2v r = rs1[(2 * ppx + di) * D::RY1 + 2 * ppy + dj];
+          bx = fma(w, r.x, bx);
+          by = fma(w, r.y, by);
+        }
+      o.x = (T)bx;
+      o.y = (T)by;
+      if (ppx >= 2 && ppx < 2 + 2 * D::CX && ppy >= 2 && ppy < 2 + 2 * D::CY)
+        gp((vec2<T>*)b1)[(long)kx * n1ny + ky] = o;
+    }
+    b2[t] = o;
+    xs2[t] = d2v{TOPT_MG_OMEGA * (double)pd2.x * (double)o.x, TOPT_MG_OMEGA * (double)pd2.y * (double)o.y};
+  }
+  __syncthreads();
+  // level-(l+1) residuals
+  if (has2) {
+    const vec2<T> bn = b2[(q2x + 1) * D::XY2 + q2y + 1];
+    const d2v a2 = mg_rec_apply<T>(rec2, [&](int q) { return xs2[(q2x + q / 3) * D::XY2 + q2y + q % 3]; });
+    d2v r = d2v{(double)bn.x - a2.x, (double)bn.y - a2.y};
+    if (!in2) r = d2v{0.0, 0.0};
+    rs2[t] = r;
+  }
+  __syncthreads();
+  if (t < D::CX * D::CY) {
+    const int cx = t / D::CY, cy = t - (t / D::CY) * D::CY;
+    const int I = I0 + cx, J = J0 + cy;
+    if (I <= n2x && J <= n2y) {
+      double bx = 0.0, by = 0.0;
+#pragma unroll
+      for (int di = 0; di < 3; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          const double w = (di == 1 ? 1.0 : 0.5) * (dj == 1 ? 1.0 : 0.5);
+          const d2v r = rs2[(2 * cx + di) * D::RY2 + 2 * cy + dj];
+          bx = fma(w, r.x, bx);
+          by = fma(w, r.y, by);
+        }
+      vec2<T> o;
+      o.x = (T)bx;
+      o.y = (T)by;
+      gp((vec2<T>*)bo)[(long)I * (n2y + 1) + J] = o;
+    }
+  }
+}
+
 // Level 0: b_1 from the PCG residual r (x0 = omega D^-1 r; K(s) has fixed
 // rows identity, so r1 = 0 on fixed DOFs, and r, hence x0, is zero there).
 template <typename T, bool KST, bool DIRECT>
