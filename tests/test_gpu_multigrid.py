@@ -48,6 +48,29 @@ def test_mg_pcg_matches_oracle(shape, precision, tol, utol):
     assert it * 5 < it_ref
 
 
+# The smoother's damping follows the element's Jacobi bound (omega = 0.97 *
+# 2 / lambda_max(D_e^-1 Ke): 0.97 at nu = 0, 0.672 at 0.3, 0.569 at 0.45), so
+# the V-cycle stays SPD for any Poisson ratio; a fixed 0.65 left the
+# guaranteed range from nu = 0.35 on.
+@pytest.mark.parametrize("nu", [0.0, 0.45])
+@pytest.mark.parametrize("precision,tol,utol", [("fp64", 1e-10, 1e-7), ("mixed", 1e-8, 1e-5)])
+def test_mg_pcg_other_poisson_ratios_match_oracle(nu, precision, tol, utol):
+    nelx, nely = 96, 32
+    print(f"multigrid PCG {precision} {nelx}x{nely} nu={nu}", flush=True)
+    prob = mbb(nelx, nely, preconditioner="multigrid", cg_tolerance=tol, max_cg_iterations=20000, nu=nu)
+    rho = rng_density(prob.n_elem, seed=7)
+    u, it, status, c = _solve(prob, precision, rho)
+    Ke, edof = O.ke_q4(prob.E, prob.nu), O.edof_q4(nelx, nely)
+    fixed = O.fixed_mask(prob.n_dofs, prob.fixed_dofs)
+    f = O.force_vector(prob.n_dofs, prob.force_dofs, prob.force_values, prob.fixed_dofs)
+    u_ref, it_ref = O.pcg(f, O.scale(rho, prob.E_min, prob.penalty), Ke, edof, fixed,
+                          max_iter=200000, tol=1e-12, preconditioner="jacobi")
+    assert status == 1
+    assert np.abs(u - u_ref).max() <= utol * np.abs(u_ref).max()
+    assert c == pytest.approx(float(f @ u_ref), rel=10 * utol)
+    assert it * 5 < it_ref
+
+
 def test_mg_far_fewer_iterations_than_jacobi():
     prob_j = bridge(240, 80, preconditioner="jacobi", cg_tolerance=1e-6, max_cg_iterations=50000,
                     volume_fraction=0.3)

@@ -247,6 +247,44 @@ bool invert8(const double* A, double* Ainv) {
 }
 
 // Material part of a descriptor: Ke (FP32 + FP64) and the regularised
+// lambda_max(D^-1 Ke) of an element stiffness (D its diagonal): the largest
+// eigenvalue of D^-1/2 Ke D^-1/2 by cyclic Jacobi rotations (8 x 8, to
+// machine precision). It bounds lambda_max(D^-1 K) of every assembled level
+// operator, whatever the element scales, masks and Galerkin agglomeration.
+double jacobi_bound(const double* Ke) {
+  double M[8][8];
+  for (int i = 0; i < 8; ++i) {
+    if (!(Ke[i * 9] > 0.0)) return -1.0;
+  }
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) M[i][j] = Ke[i * 8 + j] / std::sqrt(Ke[i * 9] * Ke[j * 9]);
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0;
+    for (int i = 0; i < 8; ++i)
+      for (int j = i + 1; j < 8; ++j) off += M[i][j] * M[i][j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < 8; ++p)
+      for (int q = p + 1; q < 8; ++q) {
+        if (std::fabs(M[p][q]) < 1e-300) continue;
+        const double th = 0.5 * std::atan2(2.0 * M[p][q], M[q][q] - M[p][p]);
+        const double c = std::cos(th), sn = std::sin(th);
+        for (int k = 0; k < 8; ++k) {   // columns p, q
+          const double mkp = M[k][p], mkq = M[k][q];
+          M[k][p] = c * mkp - sn * mkq;
+          M[k][q] = sn * mkp + c * mkq;
+        }
+        for (int k = 0; k < 8; ++k) {   // rows p, q
+          const double mpk = M[p][k], mqk = M[q][k];
+          M[p][k] = c * mpk - sn * mqk;
+          M[q][k] = sn * mpk + c * mqk;
+        }
+      }
+  }
+  double lam = M[0][0];
+  for (int i = 1; i < 8; ++i) lam = std::max(lam, M[i][i]);
+  return lam;
+}
+
 // inverse used by the element-block preconditioner,
 // Ki = inv(Ke + a I), a = 0.1 mean|diag Ke| (fem/fields.py:139-143).
 int fill_material(ProbDev& P, const double* Ke, double E_min, double penal) {
@@ -263,6 +301,9 @@ int fill_material(ProbDev& P, const double* Ke, double E_min, double penal) {
     Kr[i] = Ke[i] + ((i % 9) == 0 ? a : 0.0);
   }
   if (!invert8(Kr, Ki)) return fail(TOPT_EINVAL, "Ke + alpha I is singular");
+  // multigrid smoother damping: a fixed fraction of the guaranteed range
+  const double mu = jacobi_bound(Ke);
+  P.mg_omega = mu > 0.0 ? TOPT_MG_OMEGA_FRAC * 2.0 / mu : 0.5;
   for (int i = 0; i < 64; ++i) {
     P.Ki64[i] = Ki[i];
     P.Ki32[i] = (float)Ki[i];

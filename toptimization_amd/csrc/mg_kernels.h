@@ -27,16 +27,19 @@ namespace topt {
 #ifndef TOPT_MG_DENSE
 #define TOPT_MG_DENSE 128      // max DOFs of the coarsest level (inverted in LDS, FP64: 128 KB)
 #endif
-// Jacobi damping. Every element matrix (s_e Ke, with or without fixed-DOF
-// masking, and every Galerkin agglomerate of them) has lambda_max(D_e^-1
-// K_e) = 2.889 for the unit-square Q4 with nu = 0.3, so lambda_max(D^-1 K)
-// <= 2.889 on every level whatever the densities, and a damped Jacobi step
-// converges in the energy norm for omega < 2 / 2.889 = 0.69. 0.65 keeps the
-// V-cycle SPD with margin (omega lambda_max <= 1.88; against 0.6 it saves
-// 2 % of the iterations of a 1200x400 bridge over 30 OC iterations; 0.7 and
-// 0.8 leave the guaranteed range, 0.9 diverges).
-#ifndef TOPT_MG_OMEGA
-#define TOPT_MG_OMEGA 0.65
+// Jacobi damping, per problem (P.mg_omega). Every element matrix (s_e Ke,
+// with or without fixed-DOF masking, and every Galerkin agglomerate of them)
+// has lambda_max(D_e^-1 K_e) = mu, the value of Ke itself (2.889 for the
+// unit-square Q4 with nu = 0.3, 2.0 at nu = 0, 3.41 at nu = 0.45), so
+// lambda_max(D^-1 K) <= mu on every level whatever the densities, and a
+// damped Jacobi step converges in the energy norm for omega < 2 / mu. The
+// host sets omega = TOPT_MG_OMEGA_FRAC * 2 / mu (0.672 at nu = 0.3), which
+// keeps the V-cycle SPD for any Poisson ratio. On the 1200x400 bridge,
+// 200 OC iterations took 10,146 / 9,980 / 9,869 / 9,853 CG iterations at
+// omega = 0.62 / 0.65 / 0.67 / 0.68; a fixed 0.65 would leave the guaranteed
+// range from nu = 0.35 on.
+#ifndef TOPT_MG_OMEGA_FRAC
+#define TOPT_MG_OMEGA_FRAC 0.97
 #endif
 
 // Packed index of entry (i, j) of a symmetric 8x8 matrix (upper triangle,
@@ -522,10 +525,10 @@ __device__ __forceinline__ d2v mg_prolong(VC xc, int cny, int i, int j) {
 
 // Pre-smoothed iterate x = omega D^-1 b (from zero) at a node with 1/diag dd.
 template <typename T>
-__device__ __forceinline__ vec2<T> mg_jac(vec2<T> dd, vec2<T> bb) {
+__device__ __forceinline__ vec2<T> mg_jac(vec2<T> dd, vec2<T> bb, double om) {
   vec2<T> o;
-  o.x = (T)(TOPT_MG_OMEGA * (double)dd.x * (double)bb.x);
-  o.y = (T)(TOPT_MG_OMEGA * (double)dd.y * (double)bb.y);
+  o.x = (T)(om * (double)dd.x * (double)bb.x);
+  o.y = (T)(om * (double)dd.y * (double)bb.y);
   return o;
 }
 
@@ -679,7 +682,7 @@ __device__ __forceinline__ void mg_down_restrict_tile(PD& P, int lb, int fnx, in
   for (int k = 0; k < NPU; ++k) {
     const int u = t + k * TOPT_BLOCK;
     if (u < D::XX * D::XY)
-      xs[u] = d2v{TOPT_MG_OMEGA * (double)pd[k].x * (double)pb[k].x, TOPT_MG_OMEGA * (double)pd[k].y * (double)pb[k].y};
+      xs[u] = d2v{P.mg_omega * (double)pd[k].x * (double)pb[k].x, P.mg_omega * (double)pd[k].y * (double)pb[k].y};
   }
   __syncthreads();
   if (hasr) {
@@ -859,7 +862,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down2(const CgState* cgd, con
   for (int k = 0; k < N1X; ++k) {
     const int u = t + k * TOPT_BLOCK;
     if (u < D::XX1 * D::XY1)
-      xs1[u] = d2v{TOPT_MG_OMEGA * (double)pd[k].x * (double)pb[k].x, TOPT_MG_OMEGA * (double)pd[k].y * (double)pb[k].y};
+      xs1[u] = d2v{P.mg_omega * (double)pd[k].x * (double)pb[k].x, P.mg_omega * (double)pd[k].y * (double)pb[k].y};
   }
   __syncthreads();
   // level-l residuals
@@ -897,7 +900,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_down2(const CgState* cgd, con
         gp((vec2<T>*)b1)[(long)kx * n1ny + ky] = o;
     }
     b2[t] = o;
-    xs2[t] = d2v{TOPT_MG_OMEGA * (double)pd2.x * (double)o.x, TOPT_MG_OMEGA * (double)pd2.y * (double)o.y};
+    xs2[t] = d2v{P.mg_omega * (double)pd2.x * (double)o.x, P.mg_omega * (double)pd2.y * (double)o.y};
   }
   __syncthreads();
   // level-(l+1) residuals
@@ -1019,7 +1022,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const CgState* cgd, const 
   for (int k = 0; k < NPU; ++k) {
     const int u = t + k * TOPT_BLOCK;
     if (u < U::PX * U::PY)
-      xs[u] = d2v{TOPT_MG_OMEGA * (double)pd[k].x * (double)pb[k].x, TOPT_MG_OMEGA * (double)pd[k].y * (double)pb[k].y} +
+      xs[u] = d2v{P.mg_omega * (double)pd[k].x * (double)pb[k].x, P.mg_omega * (double)pd[k].y * (double)pb[k].y} +
               pc[k];
   }
   __syncthreads();
@@ -1028,8 +1031,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up(const CgState* cgd, const 
   const d2v ax = mg_rec_apply<T>(rec, [&](int q) { return xs[(ox + q / 3) * U::PY + oy + q % 3]; });
   const d2v xo = xs[(ox + 1) * U::PY + oy + 1];
   vec2<T> o;
-  o.x = (T)(xo.x + TOPT_MG_OMEGA * (double)rec[36] * ((double)bn.x - ax.x));
-  o.y = (T)(xo.y + TOPT_MG_OMEGA * (double)rec[37] * ((double)bn.y - ax.y));
+  o.x = (T)(xo.x + P.mg_omega * (double)rec[36] * ((double)bn.x - ax.x));
+  o.y = (T)(xo.y + P.mg_omega * (double)rec[37] * ((double)bn.y - ax.y));
   gp((vec2<T>*)rl)[n] = o;
 }
 
@@ -1188,7 +1191,7 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const CgState* c
     each_slot(lt, [&](int g, const TailSlot& s) {
       const vec2<T> bb = blt[s.n];
       sb[g] = bb;
-      sx[g] = mg_jac<T>(dinv_of(g, s), bb);
+      sx[g] = mg_jac<T>(dinv_of(g, s), bb, P.mg_omega);
     });
   } else {
     for (int n = t; n < nL; n += NB) sb[n] = blt[n];
@@ -1220,7 +1223,7 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const CgState* c
       each_slot(l + 1, [&](int g, const TailSlot& s) {
         const vec2<T> bc = mg_restrict_p<T>(P, l, s.n, rl);
         sb[g] = bc;
-        sx[g] = mg_jac<T>(dinv_of(g, s), bc);
+        sx[g] = mg_jac<T>(dinv_of(g, s), bc, P.mg_omega);
       });
     } else {
       for (int n = t; n < nL; n += NB) sb[tot + n] = mg_restrict_p<T>(P, l, n, rl);
@@ -1274,8 +1277,8 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const CgState* c
         const d2v ax = tail_apply<T, decltype(seq)::value>(s, ra, sx + ol);
         const vec2<T> bn = sb[g], xn = sx[g];
         vec2<T> o;
-        o.x = (T)((double)xn.x + TOPT_MG_OMEGA * (double)ra(36) * ((double)bn.x - ax.x));
-        o.y = (T)((double)xn.y + TOPT_MG_OMEGA * (double)ra(37) * ((double)bn.y - ax.y));
+        o.x = (T)((double)xn.x + P.mg_omega * (double)ra(36) * ((double)bn.x - ax.x));
+        o.y = (T)((double)xn.y + P.mg_omega * (double)ra(37) * ((double)bn.y - ax.y));
         sr[g] = o;
       });
     });
@@ -1291,8 +1294,8 @@ __global__ __launch_bounds__(TOPT_MG_TAIL_BLOCK) void k_mg_tail(const CgState* c
 
 // Level 0's pre-smoothed iterate at a node from r and 1/diag.
 template <typename T>
-__device__ __forceinline__ d2v mg_x0(vec2<T> rr, vec2<T> dd) {
-  return d2v{TOPT_MG_OMEGA * (double)dd.x * (double)rr.x, TOPT_MG_OMEGA * (double)dd.y * (double)rr.y};
+__device__ __forceinline__ d2v mg_x0(vec2<T> rr, vec2<T> dd, double om) {
+  return d2v{om * (double)dd.x * (double)rr.x, om * (double)dd.y * (double)rr.y};
 }
 // x' = x0 + (masked) P x_1 at the tile's ring node m.
 template <typename T>
@@ -1304,7 +1307,7 @@ __device__ __forceinline__ d2v mg_ring_x(PD& P, long m, const GLB vec2<T>* r, co
   const unsigned fb = fixbits(P.fix, m);
   if (fb & 1u) c.x = 0.0;
   if (fb & 2u) c.y = 0.0;
-  return mg_x0<T>(r[m], dv[m]) + c;
+  return mg_x0<T>(r[m], dv[m], P.mg_omega) + c;
 }
 
 // Level 0 up sweep: x' = x0 + (masked) P x_1, z = x' + omega D^-1 (r - K x'),
@@ -1335,7 +1338,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const CgState* cgd, const
     d2v c = mg_prolong<T>(xc, cny, S.ids[j].ix, S.ids[j].iy);
     if (S.fb[j] & 1u) c.x = 0.0;
     if (S.fb[j] & 2u) c.y = 0.0;
-    hv[j] = mg_x0<T>(orr[j], od[j]) + c;
+    hv[j] = mg_x0<T>(orr[j], od[j], P.mg_omega) + c;
   }
   const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   tile_store<T>(S, hv, rv, hal, esc);
@@ -1347,8 +1350,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_mg_up0(const CgState* cgd, const
   for (int j = 0; j < NPT; ++j) {
     if (!S.ids[j].act) continue;
     vec2<T> zo;
-    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + TOPT_MG_OMEGA * (double)od[j].x * ((double)orr[j].x - ax[j]));
-    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + TOPT_MG_OMEGA * (double)od[j].y * ((double)orr[j].y - ay[j]));
+    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + P.mg_omega * (double)od[j].x * ((double)orr[j].x - ax[j]));
+    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + P.mg_omega * (double)od[j].y * ((double)orr[j].y - ay[j]));
     z[S.ids[j].n] = zo;
   }
 }
@@ -1402,7 +1405,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
     d2v c = mg_prolong<T>(xc, cny, S.ids[j].ix, S.ids[j].iy);
     if (S.fb[j] & 1u) c.x = 0.0;
     if (S.fb[j] & 2u) c.y = 0.0;
-    hv[j] = mg_x0<T>(orr[j], od[j]) + c;
+    hv[j] = mg_x0<T>(orr[j], od[j], P.mg_omega) + c;
   }
   const d2v rv = mg_ring_x<T>(P, S.rn, r, dv, xc, cny);
   bool upd;
@@ -1421,8 +1424,8 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_c_mg(const ProbDev* tab, co
   for (int j = 0; j < NPT; ++j) {
     if (!S.ids[j].act) continue;
     vec2<T> zo;
-    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + TOPT_MG_OMEGA * (double)od[j].x * ((double)orr[j].x - ax[j]));
-    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + TOPT_MG_OMEGA * (double)od[j].y * ((double)orr[j].y - ay[j]));
+    zo.x = (S.fb[j] & 1u) ? T(0) : (T)(hv[j].x + P.mg_omega * (double)od[j].x * ((double)orr[j].x - ax[j]));
+    zo.y = (S.fb[j] & 2u) ? T(0) : (T)(hv[j].y + P.mg_omega * (double)od[j].y * ((double)orr[j].y - ay[j]));
     z[S.ids[j].n] = zo;
     v += (double)orr[j].x * (double)zo.x + (double)orr[j].y * (double)zo.y;
   }
@@ -1572,7 +1575,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_slot_b_mg(const ProbDev* tab, co
     const int u = t + k * TOPT_BLOCK;
     if (u < XX * XY) {
       const vec2<T> rn = mg_rnew<T>(pr[k], pa[k], alpha, cg);
-      xs[u] = mg_x0<T>(rn, pd[k]);
+      xs[u] = mg_x0<T>(rn, pd[k], P.mg_omega);
       const int px = u / XY, py = u - (u / XY) * XY;
       if (px >= 1 && px <= RX && py >= 1 && py <= RY) rs[(px - 1) * RY + (py - 1)] = d2v{(double)rn.x, (double)rn.y};
     }

@@ -146,6 +146,7 @@ struct ProbDev {
   // the mixed path, FP64 on the plain one)
   int mg_nlev, mg_ncd;
   int mg_lt, mg_pad;          // first level of the single-workgroup tail
+  double mg_omega;            // Jacobi damping of the smoother (set from Ke)
   void* mg_inv;
   MgLevel mg[TOPT_MG_MAXLEV];
 };
