@@ -12,7 +12,7 @@
 // interpolation from the coarse element's corners to child c's corners and
 // K_c = s_c Ke (fixed rows and columns zeroed) at level 0. The coarsest level
 // (<= TOPT_MG_DENSE DOFs) is solved with a dense inverse formed once per
-// solve. Smoother: damped Jacobi (omega 0.65), one sweep before and one after
+// solve. Smoother: damped Jacobi (omega from Ke, below), one sweep before and one after
 // the coarse correction, so M is symmetric positive definite and plain PCG
 // applies. The iteration count depends little on the grid size (with
 // omega 0.6, a 300x100 bridge after 20 OC iterations took 69 iterations
