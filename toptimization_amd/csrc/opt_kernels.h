@@ -406,7 +406,7 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k_volume(ProbDev single, const dou
 #define TOPT_OCC_MAXG 256
 #define TOPT_OCC_MAXK 16
 #define TOPT_OCC_PASSES 12                 // >= 11: one chain step and ten 5-step passes reach 50
-#define TOPT_OCC_SPIN_TICKS 20000000ull    // 0.2 s at 100 MHz
+#define TOPT_OCC_SPIN_TICKS 100000000ull   // 1 s at 100 MHz: far beyond any wait of a resident grid
 #define TOPT_OCC_WORDS 64                  // plan-level words: [0] abort, [2..3] spin limit (0: default)
 
 template <typename X>
