@@ -431,10 +431,12 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k3_cg_update(Grid3 G, double* u, d
   if (st->status != ST_RUN) return;
   const double alpha = st->alpha;
   double v2[2] = {0.0, 0.0};
-  const long i0 = ((long)blockIdx.x * TOPT_BLOCK + threadIdx.x) * 3;
+  // a workgroup takes 3 x TOPT_BLOCK consecutive DOFs, a thread every
+  // TOPT_BLOCK-th of them (each load instruction of a wave contiguous)
+  const long i0 = (long)blockIdx.x * (3 * TOPT_BLOCK) + threadIdx.x;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const long i = i0 + c;
+    const long i = i0 + c * TOPT_BLOCK;
     if (i >= nd) break;
     gp(u)[i] = fma(alpha, gp(p)[i], gp(u)[i]);
     const double ri = fma(-alpha, gp(Ap)[i], gp(r)[i]);
@@ -457,10 +459,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k3_cg_pupdate(Grid3 G, const doubl
   const Cg3* st = G.cg;
   if (st->status != ST_RUN) return;
   const double beta = st->beta;
-  const long i0 = ((long)blockIdx.x * TOPT_BLOCK + threadIdx.x) * 3;
+  const long i0 = (long)blockIdx.x * (3 * TOPT_BLOCK) + threadIdx.x;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const long i = i0 + c;
+    const long i = i0 + c * TOPT_BLOCK;
     if (i >= nd) break;
     gp(p)[i] = fma(beta, gp(p)[i], gp(z)[i]);
   }
