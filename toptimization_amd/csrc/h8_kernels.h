@@ -385,10 +385,10 @@ __global__ __launch_bounds__(TOPT_BLOCK) void k3_cg_init(Grid3 G, const double* 
   __shared__ double sh[8];
   const long nd = 3 * (long)(G.nelx + 1) * (G.nely + 1) * (G.nelz + 1);
   double v2[2] = {0.0, 0.0};
-  const long i0 = ((long)blockIdx.x * TOPT_BLOCK + threadIdx.x) * 3;
+  const long i0 = (long)blockIdx.x * (3 * TOPT_BLOCK) + threadIdx.x;   // as in k3_cg_update
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const long i = i0 + c;
+    const long i = i0 + c * TOPT_BLOCK;
     if (i >= nd) break;
     const double fi = gp(f)[i];
     const double ri = fi - gp(Ap)[i];
